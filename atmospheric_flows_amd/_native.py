@@ -1,0 +1,123 @@
+"""ctypes binding of libatmo_hip.so (declarations in include/atmo_hip.h).
+
+Loading fails loudly when the shared library is missing: there is no CPU
+fallback for any operator of this package.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libatmo_hip.so")
+
+ATMO_OK = 0
+ATMO_INVALID_ARG = 1
+ATMO_NO_CONVERGENCE = 2
+ATMO_NONFINITE = 3
+ATMO_OOM = 4
+ATMO_DEVICE = 5
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class MeshDesc(ctypes.Structure):
+    _fields_ = [("degree", ctypes.c_int), ("nx", ctypes.c_int), ("nz", ctypes.c_int), ("periodic", ctypes.c_int),
+                ("Lx", ctypes.c_double), ("H", ctypes.c_double), ("h", _dp)]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("Ma", ctypes.c_double), ("Fr", ctypes.c_double), ("dt", ctypes.c_double),
+                ("gamma_gas", ctypes.c_double), ("grav_x", ctypes.c_double), ("grav_z", ctypes.c_double),
+                ("grav_factor", ctypes.c_double), ("eps", ctypes.c_double), ("max_iterations", ctypes.c_int),
+                ("schur_inner_tol", ctypes.c_double), ("schur_inner_max", ctypes.c_int),
+                ("a_solve_tol", ctypes.c_double), ("fp_max_iter", ctypes.c_int), ("fp_tol", ctypes.c_double),
+                ("fp_lock", ctypes.c_int), ("mg_min_cells", ctypes.c_int), ("cheb_degree", ctypes.c_int),
+                ("cheb_range", ctypes.c_double), ("cheb_eig_its", ctypes.c_int), ("gmres_restart", ctypes.c_int),
+                ("precision", ctypes.c_int)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_long) for n in ("cg_its", "gmres_its", "inner_cg_its", "fp_its", "mg_cycles",
+                                               "coarse_its", "schur_matvecs", "host_syncs")]
+
+
+# (name, restype, argtypes) of every exported symbol
+SIGNATURES = [
+    ("atmo_params_default", None, [ctypes.POINTER(Params)]),
+    ("atmo_version", ctypes.c_char_p, []),
+    ("atmo_last_error", ctypes.c_char_p, [ctypes.c_void_p]),
+    ("atmo_ctx_create", ctypes.c_int, [ctypes.POINTER(MeshDesc), ctypes.POINTER(Params),
+                                       ctypes.POINTER(ctypes.c_void_p)]),
+    ("atmo_ctx_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("atmo_set_params", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Params)]),
+    ("atmo_ctx_info", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_long),
+                                     ctypes.POINTER(ctypes.c_int)]),
+    ("atmo_sync", ctypes.c_int, [ctypes.c_void_p]),
+    ("atmo_get_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Stats)]),
+    ("atmo_reset_stats", ctypes.c_int, [ctypes.c_void_p]),
+    ("atmo_vec_alloc", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    ("atmo_vec_free", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("atmo_vec_size", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)]),
+    ("atmo_vec_upload", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, _dp, ctypes.c_size_t]),
+    ("atmo_vec_download", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, _dp, ctypes.c_size_t]),
+    ("atmo_vec_device_ptr", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
+    ("atmo_set_stages", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    ("atmo_set_rho_for_fixed", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("atmo_set_pres_fixed", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("atmo_rhs_rho", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), ctypes.c_int]),
+    ("atmo_rhs_pressure", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p),
+                                         ctypes.c_int]),
+    ("atmo_rhs_velocity", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p),
+                                         ctypes.c_int]),
+    ("atmo_vmult_pressure", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    ("atmo_vmult_enthalpy", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    ("atmo_vmult", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    ("atmo_inverse_diagonal", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("atmo_solve_cg_mg", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_double, ctypes.POINTER(ctypes.c_int)]),
+    ("atmo_solve_schur_gmres", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                                              ctypes.POINTER(ctypes.c_int)]),
+    ("atmo_state_upload", ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp]),
+    ("atmo_state_download", ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp]),
+    ("atmo_imex_step", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("atmo_init_hill_state", ctypes.c_int, [ctypes.POINTER(MeshDesc)] + [ctypes.c_double] * 8 +
+     [ctypes.c_uint64, _dp, _dp, _dp]),
+    ("atmo_time_kernel", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp]),
+]
+
+_lib = None
+
+
+class AtmoError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"atmo status {status}: {msg}")
+        self.status = status
+
+
+class NoConvergence(AtmoError):
+    pass
+
+
+def load():
+    """Loads libatmo_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m atmospheric_flows_amd.build` "
+                          "(the package has no CPU fallback)")
+    L = ctypes.CDLL(LIB_PATH)
+    for name, res, args in SIGNATURES:
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(status, ctx=None):
+    if status == ATMO_OK:
+        return
+    msg = load().atmo_last_error(ctx).decode(errors="replace")
+    if status == ATMO_NO_CONVERGENCE:
+        raise NoConvergence(status, msg)
+    raise AtmoError(status, msg)

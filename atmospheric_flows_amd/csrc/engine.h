@@ -1,0 +1,92 @@
+// Device engine: the EULEROperator state (euler_operator.h:24-211) and the
+// EulerSolver stage sequence (Atmospheric_Flow.cc:430-871, 1100-1257) with
+// all fields resident in HBM.
+#pragma once
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "ops.h"
+
+namespace atmo {
+
+struct EngineParams {
+  double Ma = 1.0, Fr = 1.0, dt = 1.0;
+  double gamma_gas = 1.4;          // EquationData::Cp_Cv (equation_data.h:24)
+  double grav[2] = {0.0, 1.0};     // gravity unit vector (constant for the terrain case)
+  double grav_factor = 1.0;        // radial factor a^2/r^2 (1 for constant gravity)
+  double eps = 1e-10;              // Data solve / eps (parameter-file.prm:25)
+  int max_its = 30000;             // Data solve / max_iterations (:24)
+  double schur_inner_tol = 1e-12;  // euler_operator.h:2391
+  int schur_inner_max = 10000;     // euler_operator.h:2391
+  double a_solve_tol = 1e-12;      // Atmospheric_Flow.cc:571
+  int fp_max_iter = 100;           // Atmospheric_Flow.cc:1135
+  double fp_tol = 1e-10;           // Atmospheric_Flow.cc:1149
+  int fp_lock = 0;                 // >0: run exactly this many fixed-point iterations
+  int mg_min_cells = 8;            // coarsest level keeps >= this many cells
+  int cheb_degree = 3;             // Atmospheric_Flow.cc:492
+  double cheb_range = 15.0;        // :491
+  int cheb_eig_its = 10;           // :493
+  int gmres_restart = 28;          // deal.II SolverGMRES default (30 temporary vectors)
+  int precision = 0;               // 0: FP64 ; 1: mixed (FP32 MG levels, MG-preconditioned inner solve)
+};
+
+struct MeshDesc {
+  int nx = 0, nz = 0, periodic = 1;
+  double Lx = 0, H = 0;
+  std::vector<double> h;  // terrain height at the nx+1 vertex columns
+};
+
+struct EngineStats {
+  long cg_its = 0, gmres_its = 0, inner_cg_its = 0, fp_its = 0, mg_cycles = 0, coarse_its = 0, schur_matvecs = 0;
+  long host_syncs = 0;
+};
+
+// Block selected by the reference's NS_stage: 1/4 -> M_rho, 3/5 -> A, 2/6 -> D.
+enum BlockKind { BLK_RHO = 0, BLK_A = 1, BLK_D = 2 };
+
+class Engine {
+ public:
+  virtual ~Engine() = default;
+  static std::unique_ptr<Engine> create(int k, const MeshDesc& mesh, const EngineParams& p);
+
+  virtual int degree() const = 0;
+  virtual int ncells() const = 0;
+  virtual long ns() const = 0;  // scalar dofs per field
+  virtual int nlevels() const = 0;
+  virtual hipStream_t stream() const = 0;
+  virtual void set_params(const EngineParams& p) = 0;
+  virtual const EngineParams& params() const = 0;
+  virtual EngineStats& stats() = 0;
+
+  // ---- EULEROperator interface on device pointers
+  virtual void set_stages(int hyperbolic_stage, int ns_stage) = 0;
+  virtual void set_rho_for_fixed(const double* rho) = 0;  // copies, refreshes level densities
+  virtual void set_pres_fixed(const double* p) = 0;       // copies
+  virtual void rhs_rho(const double* const* src, int nsrc, double* dst) = 0;
+  virtual void rhs_pressure(const double* const* src, int nsrc, double* dst) = 0;
+  virtual void rhs_velocity(const double* const* src, int nsrc, double* dst) = 0;
+  virtual void vmult_pressure(const double* p, double* dst_u) = 0;   // B
+  virtual void vmult_enthalpy(const double* u, double* dst_p) = 0;   // C
+  virtual void vmult(const double* src, double* dst) = 0;            // per NS_stage (2: Schur)
+  virtual void inverse_diagonal(double* dst) = 0;                    // per NS_stage
+  virtual int solve_mass_mg(int ns, double* x, const double* b, double tol_factor) = 0;
+  virtual int solve_schur(double* x, const double* b, double tol_factor) = 0;
+
+  // ---- IMEX driver (state is the engine's own)
+  virtual void load_state(const double* rho, const double* u, const double* p) = 0;  // device pointers
+  virtual void store_state(double* rho, double* u, double* p) = 0;
+  virtual void step() = 0;
+  virtual double* state_ptr(int which) = 0;  // 0 rho, 1 u, 2 p (device)
+
+  // ---- measurement: mean duration of one kernel on the engine's data
+  virtual void time_kernel(int which, int nrep, double* ms_per_launch, double* bytes_per_launch) = 0;
+};
+
+// Host-side helpers shared by the C ABI and the driver.
+void hill_initial_state(int k, const MeshDesc& m, double theta0, double p0, double Nbv, double U, double g,
+                        double Rgas, double gamma_gas, double pert, uint64_t seed, double* rho, double* u,
+                        double* p);
+
+}  // namespace atmo

@@ -1,0 +1,950 @@
+// Device engine (see engine.h). Host code here only orchestrates launches on
+// one HIP stream; every vector and every Krylov scalar stays in device
+// memory. The host reads back a few bytes per Krylov iteration (the
+// convergence flag) and the GMRES Hessenberg column.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <type_traits>
+
+#include "engine.h"
+#include "krylov.h"
+#include "mg.h"
+
+namespace atmo {
+
+struct NoConvergence : std::runtime_error {
+  explicit NoConvergence(const std::string& s) : std::runtime_error(s) {}
+};
+
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  void alloc(size_t n_) {
+    release();
+    n = n_;
+    if (n) ATMO_HIP_CHECK(hipMalloc(&p, n * sizeof(T)));
+  }
+  void zero(hipStream_t s) {
+    if (n) ATMO_HIP_CHECK(hipMemsetAsync(p, 0, n * sizeof(T), s));
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+struct Level {
+  int nx = 0, nz = 0, ncell = 0;
+  long Ns = 0;
+  DBuf<double> h;
+  MeshDev m{};
+  DBuf<double> rho64;
+  DBuf<float> rho32;
+  DBuf<double> inv64[3];
+  DBuf<float> inv32[3];
+  double lmax[3] = {1, 1, 1}, lmin[3] = {1, 1, 1};
+  DBuf<double> x64, b64, t64, d64;
+  DBuf<float> x32, b32, t32, d32;
+  DBuf<double> eigb1, eigb2;
+  DBuf<double> cg_g, cg_d, cg_h;  // coarse-level CG work
+};
+
+// symmetric tridiagonal extreme eigenvalues (Sturm bisection)
+static void tridiag_extreme(const std::vector<double>& a, const std::vector<double>& b, double& lmin, double& lmax) {
+  const int n = (int)a.size();
+  double lo = 1e300, hi = -1e300;
+  for (int i = 0; i < n; ++i) {
+    double r = (i > 0 ? std::abs(b[i - 1]) : 0) + (i + 1 < n ? std::abs(b[i]) : 0);
+    lo = std::min(lo, a[i] - r);
+    hi = std::max(hi, a[i] + r);
+  }
+  auto count = [&](double x) {
+    int c = 0;
+    double q = 1;
+    for (int i = 0; i < n; ++i) {
+      double bb = i > 0 ? b[i - 1] * b[i - 1] : 0;
+      q = a[i] - x - (i > 0 ? bb / q : 0);
+      if (q == 0) q = -1e-300;
+      if (q < 0) ++c;
+    }
+    return c;
+  };
+  auto kth = [&](int k) {
+    double l = lo, h = hi;
+    for (int it = 0; it < 200; ++it) {
+      double mid = 0.5 * (l + h);
+      if (count(mid) > k) h = mid;
+      else l = mid;
+    }
+    return 0.5 * (l + h);
+  };
+  lmin = kth(0);
+  lmax = kth(n - 1);
+}
+
+__global__ void k_set_cg(CGState* st, double tol, int max_its, double tol_factor) {
+  st->tol = tol;
+  st->max_its = max_its;
+  st->tol_factor = tol_factor;
+  st->it = 0;
+  st->done = 0;
+  st->beta = 0;
+}
+
+// st->tol = factor * |b| (deal.II: SolverControl(max_its, eps*rhs.l2_norm()))
+__global__ void k_tol_from_norm(CGState* st, const double* nrm, double factor) { st->tol = factor * nrm[0]; }
+
+template <int K>
+class EngineK : public Engine {
+ public:
+  EngineK(const MeshDesc& mesh, const EngineParams& p) : P(p) {
+    ATMO_HIP_CHECK(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
+    oc_ = consts();
+    build_levels(mesh);
+    const long Ns = fine().Ns;
+    red_partials.alloc((size_t)RED_MAX_SLOTS * RED_MAX_BLOCKS);
+    red_counter.alloc(1);
+    red_counter.zero(s_);
+    rb.partials = red_partials.p;
+    rb.counter = red_counter.p;
+    cgst.alloc(4);
+    cgst.zero(s_);
+    scal.alloc(256);
+    scal.zero(s_);
+    coarse_its.alloc(1);
+    ATMO_HIP_CHECK(hipHostMalloc((void**)&hst, sizeof(CGState) * 4, hipHostMallocDefault));
+    ATMO_HIP_CHECK(hipHostMalloc((void**)&hscal, sizeof(double) * 256, hipHostMallocDefault));
+    ATMO_HIP_CHECK(hipHostMalloc((void**)&hint, sizeof(int) * 4, hipHostMallocDefault));
+    rho_fix.alloc(Ns);
+    pres_fix.alloc(Ns);
+    rho_fix.zero(s_);
+    pres_fix.zero(s_);
+    for (auto* b : {&cg_g, &cg_d, &cg_h, &s_t1, &s_t2, &in_g, &in_d, &in_h}) b->alloc(2 * Ns);
+    s_t3.alloc(Ns);
+    gm_V.alloc((size_t)(P.gmres_restart + 2) * Ns);
+    gm_p.alloc(Ns);
+    gm_coef.alloc(MAX_GMRES + 2);
+    for (auto* b : {&rho_old, &rho2, &rho3, &rho_curr, &p_old, &p2, &p3, &p_fixed, &p_fixed_old, &rhs_p}) b->alloc(Ns);
+    for (auto* b : {&u_old, &u2, &u3, &u_curr, &u_fixed, &tmp1, &rhs_u}) b->alloc(2 * Ns);
+    for (auto* b : {&rho_old, &rho2, &rho3, &rho_curr, &p_old, &p2, &p3, &p_fixed, &p_fixed_old, &rhs_p, &u_old, &u2,
+                    &u3, &u_curr, &u_fixed, &tmp1, &rhs_u})
+      b->zero(s_);
+    // geometry-only diagonals and eigenvalue bounds (M_rho and D blocks)
+    for (auto& L : lv) {
+      level_diag(*L, BLK_RHO);
+      level_diag(*L, BLK_D);
+    }
+    for (int l = 1; l < (int)lv.size(); ++l) {
+      estimate_eig(l, BLK_RHO);
+      estimate_eig(l, BLK_D);
+    }
+    sync();
+  }
+
+  ~EngineK() override {
+    if (hst) (void)hipHostFree(hst);
+    if (hscal) (void)hipHostFree(hscal);
+    if (hint) (void)hipHostFree(hint);
+    (void)hipStreamSynchronize(s_);
+    (void)hipStreamDestroy(s_);
+  }
+
+  int degree() const override { return K; }
+  int ncells() const override { return fine().ncell; }
+  long ns() const override { return fine().Ns; }
+  int nlevels() const override { return (int)lv.size(); }
+  hipStream_t stream() const override { return s_; }
+  const EngineParams& params() const override { return P; }
+  EngineStats& stats() override { return st_; }
+
+  void set_params(const EngineParams& p) override {
+    const bool prec_change = p.precision != P.precision;
+    const bool rebuild = p.gmres_restart != P.gmres_restart;
+    P = p;
+    oc_ = consts();
+    if (rebuild) gm_V.alloc((size_t)(P.gmres_restart + 2) * fine().Ns);
+    if (prec_change) refresh_levels_rho();
+  }
+
+  // ----------------------------------------------------------- operator API
+  void set_stages(int hs, int ns) override {
+    if (hs < 1 || hs > 3) throw std::invalid_argument("HYPERBOLIC_stage must be 1..3");
+    if (ns < 1 || ns > 6) throw std::invalid_argument("NS_stage must be 1..6");
+    hyper = hs;
+    nsst = ns;
+  }
+
+  void set_rho_for_fixed(const double* rho) override {
+    vec_copy(fine().Ns, rho, rho_fix.p, s_);
+    refresh_levels_rho();
+  }
+
+  void set_pres_fixed(const double* p) override { vec_copy(fine().Ns, p, pres_fix.p, s_); }
+
+  void rhs_rho(const double* const* src, int nsrc, double* dst) override {
+    if (nsrc != 2 * hyper) throw std::invalid_argument("vmult_rhs_rho_update: expected 2*stage sources");
+    launch_rhs_rho<K>(fine().m, oc_, tb_, hyper, src, dst, fine().Ns, s_);
+  }
+  void rhs_pressure(const double* const* src, int nsrc, double* dst) override {
+    const int need = hyper == 1 ? 5 : (hyper == 2 ? 8 : 11);
+    if (nsrc != need) throw std::invalid_argument("vmult_rhs_pressure: wrong number of sources");
+    launch_rhs_pressure<K>(fine().m, oc_, tb_, hyper, src, dst, fine().Ns, s_);
+  }
+  void rhs_velocity(const double* const* src, int nsrc, double* dst) override {
+    const int need = hyper == 1 ? 4 : (hyper == 2 ? 7 : 9);
+    if (nsrc != need) throw std::invalid_argument("vmult_rhs_velocity_fixed: wrong number of sources");
+    launch_rhs_velocity<K>(fine().m, oc_, tb_, hyper, src, dst, fine().Ns, s_);
+  }
+  double bc_coeff() const { return hyper == 1 ? tb_.a22t : tb_.a33t; }
+  void vmult_pressure(const double* p, double* dst_u) override {
+    launch_B<K>(fine().m, oc_, bc_coeff(), p, dst_u, fine().Ns, s_);
+  }
+  void vmult_enthalpy(const double* u, double* dst_p) override {
+    launch_C<K>(fine().m, oc_, bc_coeff(), pres_fix.p, u, dst_p, fine().Ns, s_);
+  }
+
+  void vmult(const double* src, double* dst) override {
+    if (nsst == 2) schur_vmult(src, dst);
+    else fine_apply(block_of(nsst), src, dst);
+  }
+
+  void inverse_diagonal(double* dst) override {
+    const int blk = block_of(nsst);
+    vec_copy((long)ncomp(blk) * fine().Ns, fine().inv64[blk].p, dst, s_);
+  }
+
+  int solve_mass_mg(int ns, double* x, const double* b, double tol_factor) override {
+    return pcg_mg(block_of(ns), x, b, tol_factor, P.max_its);
+  }
+
+  int solve_schur(double* x, const double* b, double tol_factor) override { return gmres(x, b, tol_factor); }
+
+  // ----------------------------------------------------------- IMEX driver
+  void load_state(const double* rho, const double* u, const double* p) override {
+    const long Ns = fine().Ns;
+    vec_copy(Ns, rho, rho_old.p, s_);
+    vec_copy(2 * Ns, u, u_old.p, s_);
+    vec_copy(Ns, p, p_old.p, s_);
+    tmp1.zero(s_);
+  }
+  void store_state(double* rho, double* u, double* p) override {
+    const long Ns = fine().Ns;
+    vec_copy(Ns, rho_old.p, rho, s_);
+    vec_copy(2 * Ns, u_old.p, u, s_);
+    vec_copy(Ns, p_old.p, p, s_);
+  }
+  double* state_ptr(int which) override { return which == 0 ? rho_old.p : (which == 1 ? u_old.p : p_old.p); }
+
+  void time_kernel(int which, int nrep, double* ms, double* bytes) override {
+    const long Ns = fine().Ns;
+    const MeshDev& m = fine().m;
+    double fields = 0;
+    hipEvent_t e0, e1;
+    ATMO_HIP_CHECK(hipEventCreate(&e0));
+    ATMO_HIP_CHECK(hipEventCreate(&e1));
+    CGState* st = cgst.p + 3;
+    if (which == 1) {
+      hipLaunchKernelGGL(k_set_cg, dim3(1), dim3(1), 0, s_, st, 0.0, 1 << 30, 0.0);
+      vec_copy(2 * Ns, u_old.p, in_g.p, s_);
+      in_d.zero(s_);
+    }
+    auto launch = [&]() {
+      switch (which) {
+        case 0:
+          launch_mass<K, double>(m, 2, true, u_old.p, rho_fix.p, in_h.p, 1.0, nullptr, 0.0, Ns, s_);
+          fields = 5;
+          break;
+        case 1:
+          jcg_apply<K>(m, rho_fix.p, fine().inv64[BLK_A].p, in_g.p, in_d.p, in_h.p, st, rb, Ns, s_);
+          fields = 11;
+          break;
+        case 2:
+          launch_mass<K, double>(m, 1, false, rho_old.p, nullptr, gm_p.p, 1.0, nullptr, 0.0, Ns, s_);
+          fields = 2;
+          break;
+        case 3:
+          launch_B<K>(m, oc_, bc_coeff(), p_old.p, in_h.p, Ns, s_);
+          fields = 3;
+          break;
+        case 4:
+          launch_C<K>(m, oc_, bc_coeff(), pres_fix.p, u_old.p, gm_p.p, Ns, s_);
+          fields = 4;
+          break;
+        case 5: {
+          const double* sp[11] = {rho_old.p, u_old.p, p_old.p, rho_old.p, u_old.p, p_old.p,
+                                  rho_old.p, u_old.p, p_old.p, rho_old.p, u_old.p};
+          launch_rhs_pressure<K>(m, oc_, tb_, 3, sp, gm_p.p, Ns, s_);
+          fields = 16;
+          break;
+        }
+        default:
+          throw std::invalid_argument("time_kernel: unknown kernel id");
+      }
+    };
+    launch();  // warm-up
+    ATMO_HIP_CHECK(hipEventRecord(e0, s_));
+    for (int r = 0; r < nrep; ++r) launch();
+    ATMO_HIP_CHECK(hipEventRecord(e1, s_));
+    ATMO_HIP_CHECK(hipEventSynchronize(e1));
+    float total = 0;
+    ATMO_HIP_CHECK(hipEventElapsedTime(&total, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = (double)total / nrep;
+    *bytes = fields * 8.0 * (double)Ns;
+  }
+
+  // one IMEX step (Atmospheric_Flow.cc:1115-1222)
+  void step() override {
+    const long Ns = fine().Ns;
+    // stage 1
+    update_density(1);
+    set_rho_for_fixed(rho2.p);
+    vec_copy(Ns, p_old.p, p_fixed_old.p, s_);
+    vec_copy(2 * Ns, u_old.p, u_fixed.p, s_);
+    fixed_point_loop(1);
+    vec_copy(Ns, p_fixed.p, p2.p, s_);
+    vec_copy(2 * Ns, u_fixed.p, u2.p, s_);
+    // stage 2
+    update_density(2);
+    set_rho_for_fixed(rho3.p);
+    vec_copy(Ns, p2.p, p_fixed_old.p, s_);
+    vec_copy(2 * Ns, u2.p, u_fixed.p, s_);
+    fixed_point_loop(2);
+    vec_copy(Ns, p_fixed.p, p3.p, s_);
+    vec_copy(2 * Ns, u_fixed.p, u3.p, s_);
+    // stage 3
+    update_density(3);
+    set_rho_for_fixed(rho_curr.p);
+    update_velocity(3);
+    update_pressure();
+    vec_copy(Ns, rho_curr.p, rho_old.p, s_);
+    vec_copy(2 * Ns, u_curr.p, u_old.p, s_);
+  }
+
+ private:
+  EngineParams P;
+  Tableau tb_;
+  OpConsts oc_{};
+  hipStream_t s_ = nullptr;
+  std::vector<std::unique_ptr<Level>> lv;
+  RedBuf rb;
+  DBuf<double> red_partials;
+  DBuf<unsigned> red_counter;
+  DBuf<CGState> cgst;
+  DBuf<double> scal;
+  DBuf<int> coarse_its;
+  CGState* hst = nullptr;
+  double* hscal = nullptr;
+  int* hint = nullptr;
+  int hyper = 1, nsst = 1;
+  DBuf<double> rho_fix, pres_fix;
+  DBuf<double> cg_g, cg_d, cg_h, s_t1, s_t2, s_t3, in_g, in_d, in_h, gm_V, gm_p, gm_coef;
+  DBuf<double> rho_old, rho2, rho3, rho_curr, u_old, u2, u3, u_curr, u_fixed, p_old, p2, p3, p_fixed, p_fixed_old,
+      tmp1, rhs_u, rhs_p;
+  EngineStats st_;
+  int last_inner = 8;
+
+  OpConsts consts() const {
+    OpConsts o;
+    o.Ma = P.Ma;
+    o.Fr = P.Fr;
+    o.dt = P.dt;
+    o.gamma_gas = P.gamma_gas;
+    o.grav[0] = P.grav[0];
+    o.grav[1] = P.grav[1];
+    o.grav_factor = P.grav_factor;
+    return o;
+  }
+
+  Level& fine() { return *lv.back(); }
+  const Level& fine() const { return *lv.back(); }
+  static int block_of(int ns) { return (ns == 1 || ns == 4) ? BLK_RHO : ((ns == 3 || ns == 5) ? BLK_A : BLK_D); }
+  static int ncomp(int blk) { return blk == BLK_A ? 2 : 1; }
+  double coeff(int blk) const { return blk == BLK_D ? 1.0 / (P.gamma_gas - 1.0) : 1.0; }
+
+  void sync() {
+    ATMO_HIP_CHECK(hipStreamSynchronize(s_));
+    ++st_.host_syncs;
+  }
+
+  void build_levels(const MeshDesc& mesh) {
+    if (mesh.nx < 1 || mesh.nz < 1 || (int)mesh.h.size() != mesh.nx + 1)
+      throw std::invalid_argument("mesh: need nx, nz >= 1 and nx+1 terrain heights");
+    struct LD {
+      int nx, nz;
+      std::vector<double> h;
+    };
+    std::vector<LD> ls;
+    ls.push_back({mesh.nx, mesh.nz, mesh.h});
+    while (ls.back().nx % 2 == 0 && ls.back().nz % 2 == 0 && ls.back().nx >= 4 &&
+           (ls.back().nx * ls.back().nz) / 4 >= P.mg_min_cells) {
+      LD c;
+      c.nx = ls.back().nx / 2;
+      c.nz = ls.back().nz / 2;
+      c.h.resize(c.nx + 1);
+      for (int i = 0; i <= c.nx; ++i) c.h[i] = ls.back().h[2 * i];
+      ls.push_back(c);
+    }
+    const int nlev = (int)ls.size();
+    for (int l = 0; l < nlev; ++l) {
+      const LD& d = ls[nlev - 1 - l];
+      auto L = std::make_unique<Level>();
+      L->nx = d.nx;
+      L->nz = d.nz;
+      L->ncell = d.nx * d.nz;
+      L->Ns = (long)L->ncell * (K + 1) * (K + 1);
+      L->h.alloc(d.nx + 1);
+      ATMO_HIP_CHECK(hipMemcpy(L->h.p, d.h.data(), sizeof(double) * (d.nx + 1), hipMemcpyHostToDevice));
+      L->m.h = L->h.p;
+      L->m.nx = d.nx;
+      L->m.nz = d.nz;
+      L->m.dx = mesh.Lx / d.nx;
+      L->m.H = mesh.H;
+      L->m.periodic = mesh.periodic;
+      L->m.ncell = L->ncell;
+      const long n2 = 2 * L->Ns;
+      L->rho64.alloc(L->Ns);
+      L->rho32.alloc(L->Ns);
+      L->rho64.zero(s_);
+      L->rho32.zero(s_);
+      for (int b = 0; b < 3; ++b) {
+        L->inv64[b].alloc(b == BLK_A ? n2 : L->Ns);
+        L->inv32[b].alloc(b == BLK_A ? n2 : L->Ns);
+      }
+      for (auto* b : {&L->x64, &L->b64, &L->t64, &L->d64}) b->alloc(n2);
+      for (auto* b : {&L->x32, &L->b32, &L->t32, &L->d32}) b->alloc(n2);
+      if (l == 0)
+        for (auto* b : {&L->cg_g, &L->cg_d, &L->cg_h}) b->alloc(n2);
+      // Chebyshev eigenvalue start vectors: (i % 11) - mean
+      for (int nc = 1; nc <= 2; ++nc) {
+        const long n = nc * L->Ns;
+        std::vector<double> v(n);
+        double mean = 0;
+        for (long i = 0; i < n; ++i) {
+          v[i] = (double)(i % 11);
+          mean += v[i];
+        }
+        mean /= (double)n;
+        for (auto& x : v) x -= mean;
+        DBuf<double>& dst = nc == 1 ? L->eigb1 : L->eigb2;
+        dst.alloc(n);
+        ATMO_HIP_CHECK(hipMemcpy(dst.p, v.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+      }
+      lv.push_back(std::move(L));
+    }
+  }
+
+  // level operator in precision T: dst = Op_blk src (+ add_scale*add)
+  template <typename T>
+  void level_apply(Level& L, int blk, const T* src, T* dst, T cf = 1, const T* add = nullptr, T add_scale = 0);
+
+  void fine_apply(int blk, const double* src, double* dst) { level_apply<double>(fine(), blk, src, dst); }
+
+  void level_diag(Level& L, int blk) {
+    const bool w = blk == BLK_A;
+    launch_inv_diag<K, double>(L.m, ncomp(blk), w, L.rho64.p, L.inv64[blk].p, coeff(blk), L.Ns, s_);
+    if (P.precision == 1) vec_cast<float, double>((long)ncomp(blk) * L.Ns, L.inv64[blk].p, L.inv32[blk].p, s_);
+  }
+
+  void refresh_levels_rho() {
+    Level& F = fine();
+    vec_copy(F.Ns, rho_fix.p, F.rho64.p, s_);
+    for (int l = (int)lv.size() - 1; l > 0; --l) launch_project_down<K>(lv[l]->m, lv[l - 1]->m, lv[l]->rho64.p, lv[l - 1]->rho64.p, s_);
+    for (auto& L : lv) {
+      if (P.precision == 1) vec_cast<float, double>(L->Ns, L->rho64.p, L->rho32.p, s_);
+      level_diag(*L, BLK_A);
+    }
+    for (int l = 1; l < (int)lv.size(); ++l) estimate_eig(l, BLK_A);
+  }
+
+  // Chebyshev eigenvalue estimate: <= cheb_eig_its CG iterations with the
+  // Jacobi preconditioner from b = (i % 11) - mean, Lanczos tridiagonal,
+  // lmax = 1.2 * estimate, lmin = lmax / range.
+  void estimate_eig(int l, int blk) {
+    Level& L = *lv[l];
+    const long n = (long)ncomp(blk) * L.Ns;
+    const double* b = ncomp(blk) == 1 ? L.eigb1.p : L.eigb2.p;
+    CGState* st = cgst.p + 2;
+    double *g = L.x64.p, *d = L.b64.p, *h = L.t64.p, *x = L.d64.p;
+    const double* Pd = L.inv64[blk].p;
+    hipLaunchKernelGGL(k_set_cg, dim3(1), dim3(1), 0, s_, st, 1e-2, P.cheb_eig_its, 0.0);
+    cg_residual(n, g, b, false, s_);
+    vec_fill(n, 0.0, x, s_);
+    vec_mul(n, g, Pd, h, s_);
+    vec_axpby(n, -1.0, h, 0.0, d, s_);
+    cg_dot(n, g, h, st, DOT_GH_INIT, rb, s_);
+    std::vector<double> alphas, betas;
+    for (int it = 0; it < P.cheb_eig_its; ++it) {
+      level_apply<double>(L, blk, d, h);
+      cg_dot(n, d, h, st, DOT_ALPHA, rb, s_);
+      cg_update(n, x, g, d, h, st, rb, s_);
+      read_state(2);
+      alphas.push_back(hst[2].alpha);
+      if (hst[2].done) break;
+      vec_mul(n, g, Pd, h, s_);
+      cg_dot(n, g, h, st, DOT_BETA, rb, s_);
+      cg_dir(n, d, h, st, s_);
+      read_state(2);
+      betas.push_back(hst[2].beta);
+    }
+    const int m = (int)alphas.size();
+    std::vector<double> a(m), o(m > 0 ? m - 1 : 0);
+    for (int j = 0; j < m; ++j) {
+      a[j] = 1.0 / alphas[j] + (j > 0 ? betas[j - 1] / alphas[j - 1] : 0.0);
+      if (j + 1 < m) o[j] = std::sqrt(betas[j]) / alphas[j];
+    }
+    double lmin, lmax;
+    tridiag_extreme(a, o, lmin, lmax);
+    L.lmax[blk] = 1.2 * lmax;
+    L.lmin[blk] = L.lmax[blk] / P.cheb_range;
+  }
+
+  void read_state(int slot) {
+    ATMO_HIP_CHECK(hipMemcpyAsync(&hst[slot], cgst.p + slot, sizeof(CGState), hipMemcpyDeviceToHost, s_));
+    sync();
+  }
+
+  // --------------------------------------------------------------- MG
+  template <typename T>
+  static T* LX(Level& L) {
+    if constexpr (std::is_same<T, double>::value) return L.x64.p; else return L.x32.p;
+  }
+  template <typename T>
+  static T* LB(Level& L) {
+    if constexpr (std::is_same<T, double>::value) return L.b64.p; else return L.b32.p;
+  }
+  template <typename T>
+  static T* LT(Level& L) {
+    if constexpr (std::is_same<T, double>::value) return L.t64.p; else return L.t32.p;
+  }
+  template <typename T>
+  static T* LD(Level& L) {
+    if constexpr (std::is_same<T, double>::value) return L.d64.p; else return L.d32.p;
+  }
+  template <typename T>
+  static const T* LP(Level& L, int blk) {
+    if constexpr (std::is_same<T, double>::value) return L.inv64[blk].p; else return L.inv32[blk].p;
+  }
+
+  template <typename T>
+  void cheb(int l, int blk, T* x, const T* b, bool zero_guess) {
+    Level& L = *lv[l];
+    const long n = (long)ncomp(blk) * L.Ns;
+    const double theta = 0.5 * (L.lmax[blk] + L.lmin[blk]), delta = 0.5 * (L.lmax[blk] - L.lmin[blk]);
+    const double sigma = theta / delta;
+    double rho = 1.0 / sigma;
+    T* t = LT<T>(L);
+    T* d = LD<T>(L);
+    const T* Pd = LP<T>(L, blk);
+    if (zero_guess) {
+      cheb_first<T>(n, b, Pd, x, d, (T)(1.0 / theta), s_);
+    } else {
+      level_apply<T>(L, blk, x, t);
+      cheb_step<T>(n, b, t, Pd, x, d, (T)0, (T)(1.0 / theta), s_);
+    }
+    for (int k = 0; k < P.cheb_degree - 1; ++k) {
+      level_apply<T>(L, blk, x, t);
+      const double rn = 1.0 / (2.0 * sigma - rho);
+      cheb_step<T>(n, b, t, Pd, x, d, (T)(rn * rho), (T)(2.0 * rn / delta), s_);
+      rho = rn;
+    }
+  }
+
+  // V-cycle on level l: x = MG(b); x, b are level buffers of type T
+  template <typename T>
+  void vcycle(int l, int blk, T* x, const T* b, const double* tol, int max_its) {
+    Level& L = *lv[l];
+    const int nc = ncomp(blk);
+    if (l == 0) {
+      const double* b64;
+      double* x64;
+      if constexpr (std::is_same<T, double>::value) {
+        b64 = b;
+        x64 = x;
+      } else {
+        vec_cast<double, T>((long)nc * L.Ns, b, L.b64.p, s_);
+        b64 = L.b64.p;
+        x64 = L.x64.p;
+      }
+      launch_coarse_cg<K>(L.m, nc, blk == BLK_A, L.rho64.p, coeff(blk), b64, x64, L.cg_g.p, L.cg_d.p, L.cg_h.p, tol,
+                          max_its, coarse_its.p, L.Ns, s_);
+      if constexpr (!std::is_same<T, double>::value) vec_cast<T, double>((long)nc * L.Ns, x64, x, s_);
+      return;
+    }
+    cheb<T>(l, blk, x, b, true);
+    T* t = LT<T>(L);
+    level_apply<T>(L, blk, x, t, (T)-1, b, (T)1);  // t = b - A x
+    Level& C = *lv[l - 1];
+    launch_restrict<K, T>(L.m, C.m, t, LB<T>(C), nc, L.Ns, C.Ns, s_);
+    vcycle<T>(l - 1, blk, LX<T>(C), LB<T>(C), tol, max_its);
+    launch_prolongate_add<K, T>(L.m, C.m, LX<T>(C), x, nc, L.Ns, C.Ns, s_);
+    cheb<T>(l, blk, x, b, false);
+  }
+
+  // PreconditionMG::vmult: dst = V-cycle(src) on the fine level
+  void mg_prec(int blk, const double* src, double* dst, const double* tol, int max_its) {
+    ++st_.mg_cycles;
+    const int top = (int)lv.size() - 1;
+    if (P.precision == 1) {
+      Level& F = fine();
+      const long n = (long)ncomp(blk) * F.Ns;
+      vec_cast<float, double>(n, src, F.b32.p, s_);
+      vcycle<float>(top, blk, F.x32.p, F.b32.p, tol, max_its);
+      vec_cast<double, float>(n, F.x32.p, dst, s_);
+    } else {
+      vcycle<double>(top, blk, dst, src, tol, max_its);
+    }
+  }
+
+  // CG + MG on a mass-type block (SolverCG + PreconditionMG). Tolerance
+  // tol_factor*|b|; the coarse CG shares the tolerance and max_its
+  // (Atmospheric_Flow.cc:452/506 share one SolverControl).
+  int pcg_mg(int blk, double* x, const double* b, double tol_factor, int max_its) {
+    const long n = (long)ncomp(blk) * fine().Ns;
+    CGState* st = cgst.p + 0;
+    hipLaunchKernelGGL(k_set_cg, dim3(1), dim3(1), 0, s_, st, 0.0, max_its, tol_factor);
+    vec_dot(n, b, b, scal.p + 0, true, rb, s_);
+    hipLaunchKernelGGL(k_tol_from_norm, dim3(1), dim3(1), 0, s_, st, scal.p + 0, tol_factor);
+    double *g = cg_g.p, *d = cg_d.p, *h = cg_h.p;
+    fine_apply(blk, x, g);
+    cg_residual(n, g, b, true, s_);
+    cg_init_check(n, g, st, false, rb, s_);
+    read_state(0);
+    if (hst[0].done == 1) return 0;
+    if (hst[0].done == -1) throw NoConvergence("CG+MG: no convergence");
+    const double* tol = &st->tol;
+    mg_prec(blk, g, h, tol, max_its);
+    vec_axpby(n, -1.0, h, 0.0, d, s_);
+    cg_dot(n, g, h, st, DOT_GH_INIT, rb, s_);
+    int it = 0;
+    while (true) {
+      fine_apply(blk, d, h);
+      cg_dot(n, d, h, st, DOT_ALPHA, rb, s_);
+      cg_update(n, x, g, d, h, st, rb, s_);
+      ATMO_HIP_CHECK(hipMemcpyAsync(hint, coarse_its.p, sizeof(int), hipMemcpyDeviceToHost, s_));
+      read_state(0);
+      if (hint[0] < 0) throw NoConvergence("MG coarse CG: no convergence");
+      st_.coarse_its += hint[0];
+      it = hst[0].it;
+      if (hst[0].done == 1) break;
+      if (hst[0].done == -1) throw NoConvergence("CG+MG: no convergence");
+      mg_prec(blk, g, h, tol, max_its);
+      cg_dot(n, g, h, st, DOT_BETA, rb, s_);
+      cg_dir(n, d, h, st, s_);
+    }
+    st_.cg_its += it;
+    return it;
+  }
+
+  // inner A^{-1} of the Schur complement (euler_operator.h:2391-2397):
+  // Jacobi CG to 1e-12*|b| from a zero guess, fused kernels, device-side
+  // convergence flag, host polls once per batch of iterations.
+  int inner_solve(double* x, const double* b) {
+    const long n = 2 * fine().Ns;
+    if (P.precision == 1) {
+      vec_fill(n, 0.0, x, s_);
+      const int its = pcg_mg(BLK_A, x, b, P.schur_inner_tol, P.schur_inner_max);
+      st_.cg_its -= its;
+      st_.inner_cg_its += its;
+      return its;
+    }
+    CGState* st = cgst.p + 1;
+    const double* PA = fine().inv64[BLK_A].p;
+    hipLaunchKernelGGL(k_set_cg, dim3(1), dim3(1), 0, s_, st, 0.0, P.schur_inner_max, P.schur_inner_tol);
+    vec_fill(n, 0.0, x, s_);
+    jcg_init(n, b, PA, in_g.p, in_d.p, st, rb, s_);
+    int batch = std::max(1, last_inner);
+    int its = 0;
+    while (true) {
+      for (int i = 0; i < batch; ++i) {
+        jcg_apply<K>(fine().m, rho_fix.p, PA, in_g.p, in_d.p, in_h.p, st, rb, fine().Ns, s_);
+        jcg_update(n, x, in_g.p, in_d.p, in_h.p, PA, st, rb, s_);
+      }
+      read_state(1);
+      its = hst[1].it;
+      if (hst[1].done == 1) break;
+      if (hst[1].done == -1) throw NoConvergence("Schur inner CG: no convergence");
+      batch = 2;
+    }
+    last_inner = its;
+    st_.inner_cg_its += its;
+    return its;
+  }
+
+  // S p = D p - C A^{-1} B p (apply_add NS 2, euler_operator.h:2374-2408)
+  void schur_vmult(const double* src, double* dst) {
+    ++st_.schur_matvecs;
+    const long Ns = fine().Ns;
+    launch_B<K>(fine().m, oc_, bc_coeff(), src, s_t1.p, Ns, s_);
+    inner_solve(s_t2.p, s_t1.p);
+    launch_C<K>(fine().m, oc_, bc_coeff(), pres_fix.p, s_t2.p, s_t3.p, Ns, s_);
+    launch_mass<K, double>(fine().m, 1, false, src, nullptr, dst, coeff(BLK_D), s_t3.p, -1.0, Ns, s_);
+  }
+
+  // GMRES, left Jacobi (diag D) preconditioning, convergence on the
+  // preconditioned residual against tol_factor*|b| (deal.II SolverGMRES
+  // defaults). Orthogonalization: classical Gram-Schmidt with one
+  // re-orthogonalization when the norm drops below 10*sqrt(eps) of its
+  // start (same criterion as deal.II's iterated Gram-Schmidt).
+  int gmres(double* x, const double* b, double tol_factor) {
+    const long Ns = fine().Ns;
+    const int m = P.gmres_restart;
+    const double* PD = fine().inv64[BLK_D].p;
+    double* V = gm_V.p;
+    double* p = gm_p.p;
+    double* coef = gm_coef.p;
+    vec_dot(Ns, b, b, scal.p + 0, true, rb, s_);
+    ATMO_HIP_CHECK(hipMemcpyAsync(hscal, scal.p, sizeof(double), hipMemcpyDeviceToHost, s_));
+    sync();
+    const double tol = tol_factor * hscal[0];
+    auto check = [&](int step, double res) {
+      if (res <= tol) return 1;
+      if (step >= P.max_its || std::isnan(res)) return -1;
+      return 0;
+    };
+    std::vector<double> H((size_t)(m + 1) * m, 0.0), gam(m + 1), cs(m), sn(m), hcol(m + 2);
+    int acc = 0, state = 0;
+    while (true) {
+      schur_vmult(x, p);
+      vec_axpby(Ns, 1.0, b, -1.0, p, s_);  // p = b - S x
+      vec_mul(Ns, p, PD, V, s_);
+      vec_dot(Ns, V, V, scal.p + 0, true, rb, s_);
+      ATMO_HIP_CHECK(hipMemcpyAsync(hscal, scal.p, sizeof(double), hipMemcpyDeviceToHost, s_));
+      sync();
+      double rho = hscal[0];
+      state = check(acc, rho);
+      if (state != 0) break;
+      std::fill(gam.begin(), gam.end(), 0.0);
+      gam[0] = rho;
+      vec_axpby(Ns, 1.0 / rho, V, 0.0, V, s_);
+      int dim = 0;
+      for (int j = 0; j < m && state == 0; ++j) {
+        ++acc;
+        dim = j + 1;
+        double* w = V + (long)(j + 1) * Ns;
+        schur_vmult(V + (long)j * Ns, p);
+        vec_mul(Ns, p, PD, w, s_);
+        // h_i = w.V_i (i <= j) and |w|^2 in one pass
+        gm_mdot(Ns, w, V, Ns, j + 2, coef, rb, s_);
+        gm_maxpy(Ns, w, V, Ns, j + 1, coef, s_);
+        ATMO_HIP_CHECK(hipMemcpyAsync(hcol.data(), coef, sizeof(double) * (j + 2), hipMemcpyDeviceToHost, s_));
+        vec_dot(Ns, w, w, scal.p + 1, true, rb, s_);
+        ATMO_HIP_CHECK(hipMemcpyAsync(hscal, scal.p, sizeof(double) * 2, hipMemcpyDeviceToHost, s_));
+        sync();
+        const double nstart = std::sqrt(hcol[j + 1]);
+        double nw = hscal[1];
+        double* hc = &H[(size_t)j * (m + 1)];
+        for (int i = 0; i <= j; ++i) hc[i] = hcol[i];
+        if (nw < 10.0 * nstart * std::sqrt(2.220446049250313e-16)) {
+          gm_mdot(Ns, w, V, Ns, j + 1, coef, rb, s_);
+          gm_maxpy(Ns, w, V, Ns, j + 1, coef, s_);
+          ATMO_HIP_CHECK(hipMemcpyAsync(hcol.data(), coef, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, s_));
+          vec_dot(Ns, w, w, scal.p + 1, true, rb, s_);
+          ATMO_HIP_CHECK(hipMemcpyAsync(hscal, scal.p, sizeof(double) * 2, hipMemcpyDeviceToHost, s_));
+          sync();
+          for (int i = 0; i <= j; ++i) hc[i] += hcol[i];
+          nw = hscal[1];
+        }
+        hc[j + 1] = nw;
+        if (nw != 0.0) vec_axpby(Ns, 1.0 / nw, w, 0.0, w, s_);
+        for (int i = 0; i < j; ++i) {
+          const double t = hc[i];
+          hc[i] = cs[i] * t + sn[i] * hc[i + 1];
+          hc[i + 1] = -sn[i] * t + cs[i] * hc[i + 1];
+        }
+        const double r = 1.0 / std::sqrt(hc[j] * hc[j] + hc[j + 1] * hc[j + 1]);
+        sn[j] = hc[j + 1] * r;
+        cs[j] = hc[j] * r;
+        hc[j] = cs[j] * hc[j] + sn[j] * hc[j + 1];
+        gam[j + 1] = -sn[j] * gam[j];
+        gam[j] *= cs[j];
+        rho = std::abs(gam[j + 1]);
+        state = check(acc, rho);
+      }
+      std::vector<double> y(dim);
+      for (int i = dim - 1; i >= 0; --i) {
+        double sacc = gam[i];
+        for (int l = i + 1; l < dim; ++l) sacc -= H[(size_t)l * (m + 1) + i] * y[l];
+        y[i] = sacc / H[(size_t)i * (m + 1) + i];
+      }
+      ATMO_HIP_CHECK(hipMemcpyAsync(coef, y.data(), sizeof(double) * dim, hipMemcpyHostToDevice, s_));
+      gm_update(Ns, x, V, Ns, dim, coef, s_);
+      sync();  // y lives on the host stack
+      if (state != 0) break;
+    }
+    if (state == -1) throw NoConvergence("GMRES: no convergence");
+    st_.gmres_its += acc;
+    return acc;
+  }
+
+  // ---------------------------------------------------- solver stages
+  // update_density (Atmospheric_Flow.cc:430-537)
+  void update_density(int hs) {
+    set_stages(hs, hs == 3 ? 4 : 1);
+    const long Ns = fine().Ns;
+    double* rhs = gm_p.p;  // scratch of size Ns
+    if (hs == 1) {
+      const double* s[2] = {rho_old.p, u_old.p};
+      rhs_rho(s, 2, rhs);
+      vec_copy(Ns, rho_old.p, rho2.p, s_);
+      pcg_mg(BLK_RHO, rho2.p, rhs, P.eps, P.max_its);
+    } else if (hs == 2) {
+      const double* s[4] = {rho_old.p, u_old.p, rho2.p, u2.p};
+      rhs_rho(s, 4, rhs);
+      vec_copy(Ns, rho2.p, rho3.p, s_);
+      pcg_mg(BLK_RHO, rho3.p, rhs, P.eps, P.max_its);
+    } else {
+      const double* s[6] = {rho_old.p, u_old.p, rho2.p, u2.p, rho3.p, u3.p};
+      rhs_rho(s, 6, rhs);
+      vec_copy(Ns, rho3.p, rho_curr.p, s_);
+      pcg_mg(BLK_RHO, rho_curr.p, rhs, P.eps, P.max_its);
+    }
+  }
+
+  // pressure_fixed_point (Atmospheric_Flow.cc:544-673)
+  void pressure_fixed_point(int hs) {
+    set_stages(hs, 2);
+    set_pres_fixed(p_fixed_old.p);
+    if (hs == 1) {
+      const double* sp[5] = {rho_old.p, u_old.p, p_old.p, rho2.p, u_fixed.p};
+      rhs_pressure(sp, 5, rhs_p.p);
+      const double* su[4] = {rho_old.p, u_old.p, p_old.p, rho2.p};
+      rhs_velocity(su, 4, rhs_u.p);
+    } else {
+      const double* sp[8] = {rho_old.p, u_old.p, p_old.p, rho2.p, u2.p, p2.p, rho3.p, u_fixed.p};
+      rhs_pressure(sp, 8, rhs_p.p);
+      const double* su[7] = {rho_old.p, u_old.p, p_old.p, rho2.p, u2.p, p2.p, rho3.p};
+      rhs_velocity(su, 7, rhs_u.p);
+    }
+    pcg_mg(BLK_A, tmp1.p, rhs_u.p, P.a_solve_tol, P.max_its);
+    const long Ns = fine().Ns;
+    vmult_enthalpy(tmp1.p, s_t3.p);
+    vec_axpby(Ns, -1.0, s_t3.p, 1.0, rhs_p.p, s_);
+    vec_copy(Ns, p_fixed_old.p, p_fixed.p, s_);
+    gmres(p_fixed.p, rhs_p.p, P.eps);
+  }
+
+  // update_velocity (Atmospheric_Flow.cc:680-781)
+  void update_velocity(int hs) {
+    const long Ns = fine().Ns;
+    if (hs == 1 || hs == 2) {
+      set_stages(hs, 3);
+      vmult_pressure(p_fixed.p, s_t1.p);
+      vec_axpby(2 * Ns, -1.0, s_t1.p, 1.0, rhs_u.p, s_);
+      pcg_mg(BLK_A, u_fixed.p, rhs_u.p, P.eps, P.max_its);
+    } else {
+      set_stages(3, 5);
+      const double* su[9] = {rho_old.p, u_old.p, p_old.p, rho2.p, u2.p, p2.p, rho3.p, u3.p, p3.p};
+      rhs_velocity(su, 9, rhs_u.p);
+      vec_copy(2 * Ns, u3.p, u_curr.p, s_);
+      pcg_mg(BLK_A, u_curr.p, rhs_u.p, P.eps, P.max_its);
+    }
+  }
+
+  // update_pressure (Atmospheric_Flow.cc:788-871)
+  void update_pressure() {
+    set_stages(3, 6);
+    const double* sp[11] = {rho_old.p, u_old.p, p_old.p, rho2.p, u2.p, p2.p, rho3.p, u3.p, p3.p, rho_curr.p, u_curr.p};
+    rhs_pressure(sp, 11, rhs_p.p);
+    vec_copy(fine().Ns, p3.p, p_old.p, s_);
+    pcg_mg(BLK_D, p_old.p, rhs_p.p, P.eps, P.max_its);
+  }
+
+  // fixed-point loop with the relative L-infinity stop test on nodal
+  // pressure values (Atmospheric_Flow.cc:1135-1153)
+  void fixed_point_loop(int hs) {
+    const long Ns = fine().Ns;
+    for (int iter = 0; iter < P.fp_max_iter; ++iter) {
+      pressure_fixed_point(hs);
+      update_velocity(hs);
+      ++st_.fp_its;
+      vec_linf(Ns, p_fixed.p, nullptr, scal.p + 2, rb, s_);
+      vec_linf(Ns, p_fixed.p, p_fixed_old.p, scal.p + 3, rb, s_);
+      ATMO_HIP_CHECK(hipMemcpyAsync(hscal + 2, scal.p + 2, 2 * sizeof(double), hipMemcpyDeviceToHost, s_));
+      sync();
+      const double err = hscal[3] / (hscal[2] + 1e-10);
+      if (P.fp_lock > 0) {
+        if (iter + 1 >= P.fp_lock) break;
+      } else if (err < P.fp_tol) {
+        break;
+      }
+      vec_copy(Ns, p_fixed.p, p_fixed_old.p, s_);
+    }
+  }
+};
+
+template <int K>
+template <typename T>
+void EngineK<K>::level_apply(Level& L, int blk, const T* src, T* dst, T cf, const T* add, T add_scale) {
+  const T* rho = nullptr;
+  if (blk == BLK_A) rho = std::is_same<T, double>::value ? (const T*)L.rho64.p : (const T*)L.rho32.p;
+  launch_mass<K, T>(L.m, ncomp(blk), blk == BLK_A, src, rho, dst, (T)(cf * coeff(blk)), add, add_scale, L.Ns, s_);
+}
+
+std::unique_ptr<Engine> Engine::create(int k, const MeshDesc& mesh, const EngineParams& p) {
+  switch (k) {
+    case 1: return std::make_unique<EngineK<1>>(mesh, p);
+    case 2: return std::make_unique<EngineK<2>>(mesh, p);
+    case 3: return std::make_unique<EngineK<3>>(mesh, p);
+    default: throw std::invalid_argument("degree must be 1, 2 or 3");
+  }
+}
+
+// ------------------------------------------------------- initial state
+static uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+// Hydrostatic background with constant buoyancy frequency, interpolated at
+// the FE_DGQ support points (VectorTools::interpolate, Atmospheric_Flow.cc:420-422),
+// density perturbed by 1 + pert*(2U-1), U = top 53 bits of
+// splitmix64(seed ^ global dof id) / 2^53.
+void hill_initial_state(int k, const MeshDesc& m, double theta0, double p0, double Nbv, double U, double g,
+                        double Rgas, double gamma_gas, double pert, uint64_t seed, double* rho, double* u,
+                        double* p) {
+  constexpr int KMAX = 3;
+  if (k < 1 || k > KMAX) throw std::invalid_argument("degree must be 1..3");
+  double gll[KMAX + 1];
+  if (k == 1) { gll[0] = 0; gll[1] = 1; }
+  else if (k == 2) { for (int i = 0; i < 3; ++i) gll[i] = basis::kT<2>.gll[i]; }
+  else { for (int i = 0; i < 4; ++i) gll[i] = basis::kT<3>.gll[i]; }
+  const int nd = k + 1, nd2 = nd * nd;
+  const long Ns = (long)m.nx * m.nz * nd2;
+  const double cp = gamma_gas * Rgas / (gamma_gas - 1.0);
+  for (int c = 0; c < m.nx * m.nz; ++c) {
+    const int i = c % m.nx, j = c / m.nx;
+    const double hl = m.h[i], hr = m.h[i + 1];
+    const double z[4] = {hl + (m.H - hl) * (double)j / m.nz, hr + (m.H - hr) * (double)j / m.nz,
+                         hl + (m.H - hl) * (double)(j + 1) / m.nz, hr + (m.H - hr) * (double)(j + 1) / m.nz};
+    for (int jj = 0; jj < nd; ++jj)
+      for (int ii = 0; ii < nd; ++ii) {
+        const double xi = gll[ii], eta = gll[jj];
+        const double zz = (1 - xi) * (1 - eta) * z[0] + xi * (1 - eta) * z[1] + (1 - xi) * eta * z[2] + xi * eta * z[3];
+        const double th = theta0 * std::exp(Nbv * Nbv * zz / g);
+        const double ex = 1.0 + g * g / (cp * theta0 * Nbv * Nbv) * (std::exp(-Nbv * Nbv * zz / g) - 1.0);
+        const double pr = p0 * std::pow(ex, cp / Rgas);
+        const double rh = pr / (Rgas * th * ex);
+        const long id = (long)c * nd2 + jj * nd + ii;
+        const uint64_t r = splitmix64(seed ^ (uint64_t)id);
+        const double U01 = (double)(r >> 11) * (1.0 / 9007199254740992.0);
+        rho[id] = rh * (1.0 + pert * (2.0 * U01 - 1.0));
+        p[id] = pr;
+        u[id] = U;
+        u[Ns + id] = 0.0;
+      }
+  }
+}
+
+}  // namespace atmo

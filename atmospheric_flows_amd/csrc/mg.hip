@@ -1,0 +1,307 @@
+// Multigrid kernels on the uniformly coarsened terrain mesh: child (a,b) of
+// coarse cell (I,J) is fine cell (2I+a, 2J+b).
+#include "mg.h"
+#include "wave.h"
+
+namespace atmo {
+
+template <int K>
+__device__ inline double trP(int a, int i, int n) {
+  constexpr auto& tr = basis::kTr<K>;
+  double v = 0;
+#pragma unroll
+  for (int aa = 0; aa < 2; ++aa)
+#pragma unroll
+    for (int ii = 0; ii < K + 1; ++ii)
+#pragma unroll
+      for (int nn = 0; nn < K + 1; ++nn)
+        if (aa == a && ii == i && nn == n) v = tr.P[aa][ii][nn];
+  return v;
+}
+
+template <int K>
+__device__ inline double trR(int a, int mm, int i) {
+  constexpr auto& tr = basis::kTr<K>;
+  double v = 0;
+#pragma unroll
+  for (int aa = 0; aa < 2; ++aa)
+#pragma unroll
+    for (int m2 = 0; m2 < K + 1; ++m2)
+#pragma unroll
+      for (int ii = 0; ii < K + 1; ++ii)
+        if (aa == a && m2 == mm && ii == i) v = tr.R[aa][m2][ii];
+  return v;
+}
+
+// one thread per coarse dof
+template <int K, typename T>
+__global__ void k_restrict(MeshDev mf, MeshDev mc, const T* __restrict__ fine, T* __restrict__ coarse, int ncomp,
+                           long Nf, long Nc) {
+  constexpr int ND = K + 1, ND2 = ND * ND;
+  const long tot = (long)ncomp * Nc;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < tot; t += (long)gridDim.x * blockDim.x) {
+    const int comp = (int)(t / Nc);
+    const long r = t - comp * Nc;
+    const int cc = (int)(r / ND2), o = (int)(r - (long)cc * ND2), m = o / ND, n = o - m * ND;
+    const int I = cc % mc.nx, J = cc / mc.nx;
+    double acc = 0;
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        const long cf = (long)(2 * J + b) * mf.nx + 2 * I + a;
+        const T* u = fine + comp * Nf + cf * ND2;
+#pragma unroll
+        for (int jj = 0; jj < ND; ++jj) {
+          const double pb = trP<K>(b, jj, m);
+#pragma unroll
+          for (int ii = 0; ii < ND; ++ii) acc += pb * trP<K>(a, ii, n) * (double)u[jj * ND + ii];
+        }
+      }
+    coarse[t] = (T)acc;
+  }
+}
+
+// one thread per fine dof
+template <int K, typename T>
+__global__ void k_prolongate_add(MeshDev mf, MeshDev mc, const T* __restrict__ coarse, T* __restrict__ fine,
+                                 int ncomp, long Nf, long Nc) {
+  constexpr int ND = K + 1, ND2 = ND * ND;
+  const long tot = (long)ncomp * Nf;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < tot; t += (long)gridDim.x * blockDim.x) {
+    const int comp = (int)(t / Nf);
+    const long r = t - comp * Nf;
+    const int cf = (int)(r / ND2), o = (int)(r - (long)cf * ND2), jj = o / ND, ii = o - jj * ND;
+    const int i = cf % mf.nx, j = cf / mf.nx;
+    const int cc = (j >> 1) * mc.nx + (i >> 1), a = i & 1, b = j & 1;
+    const T* u = coarse + comp * Nc + (long)cc * ND2;
+    double acc = 0;
+#pragma unroll
+    for (int m = 0; m < ND; ++m) {
+      const double pb = trP<K>(b, jj, m);
+#pragma unroll
+      for (int n = 0; n < ND; ++n) acc += pb * trP<K>(a, ii, n) * (double)u[m * ND + n];
+    }
+    fine[t] += (T)acc;
+  }
+}
+
+template <int K>
+__global__ void k_project_down(MeshDev mf, MeshDev mc, const double* __restrict__ fine, double* __restrict__ coarse) {
+  constexpr int ND = K + 1, ND2 = ND * ND;
+  const long tot = (long)mc.ncell * ND2;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < tot; t += (long)gridDim.x * blockDim.x) {
+    const int cc = (int)(t / ND2), o = (int)(t - (long)cc * ND2), m = o / ND, n = o - m * ND;
+    const int I = cc % mc.nx, J = cc / mc.nx;
+    double acc = 0;
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        const long cf = (long)(2 * J + b) * mf.nx + 2 * I + a;
+        const double* u = fine + cf * ND2;
+#pragma unroll
+        for (int jj = 0; jj < ND; ++jj) {
+          const double rb = trR<K>(b, m, jj);
+#pragma unroll
+          for (int ii = 0; ii < ND; ++ii) acc += rb * trR<K>(a, n, ii) * u[jj * ND + ii];
+        }
+      }
+    coarse[t] = acc;
+  }
+}
+
+static int grid_for(long n) {
+  long b = (n + 255) / 256;
+  if (b > 8192) b = 8192;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
+template <int K, typename T>
+void launch_restrict(const MeshDev& mf, const MeshDev& mc, const T* fine, T* coarse, int ncomp, long Nf, long Nc,
+                     hipStream_t s) {
+  hipLaunchKernelGGL((k_restrict<K, T>), dim3(grid_for(ncomp * Nc)), dim3(256), 0, s, mf, mc, fine, coarse, ncomp,
+                     Nf, Nc);
+  ATMO_HIP_CHECK(hipGetLastError());
+}
+
+template <int K, typename T>
+void launch_prolongate_add(const MeshDev& mf, const MeshDev& mc, const T* coarse, T* fine, int ncomp, long Nf,
+                           long Nc, hipStream_t s) {
+  hipLaunchKernelGGL((k_prolongate_add<K, T>), dim3(grid_for(ncomp * Nf)), dim3(256), 0, s, mf, mc, coarse, fine,
+                     ncomp, Nf, Nc);
+  ATMO_HIP_CHECK(hipGetLastError());
+}
+
+template <int K>
+void launch_project_down(const MeshDev& mf, const MeshDev& mc, const double* fine, double* coarse, hipStream_t s) {
+  hipLaunchKernelGGL((k_project_down<K>), dim3(grid_for((long)mc.ncell * (K + 1) * (K + 1))), dim3(256), 0, s, mf,
+                     mc, fine, coarse);
+  ATMO_HIP_CHECK(hipGetLastError());
+}
+
+// ------------------------------------------------------------ Chebyshev
+template <typename T>
+__global__ void k_cheb_first(long n, const T* __restrict__ b, const T* __restrict__ P, T* __restrict__ x,
+                             T* __restrict__ d, T inv_theta) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const T v = P[i] * b[i] * inv_theta;
+    d[i] = v;
+    x[i] = v;
+  }
+}
+
+template <typename T>
+__global__ void k_cheb_step(long n, const T* __restrict__ b, const T* __restrict__ t, const T* __restrict__ P,
+                            T* __restrict__ x, T* __restrict__ d, T f1, T f2) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const T v = f1 * d[i] + f2 * P[i] * (b[i] - t[i]);
+    d[i] = v;
+    x[i] += v;
+  }
+}
+
+template <typename T>
+void cheb_first(long n, const T* b, const T* P, T* x, T* d, T inv_theta, hipStream_t s) {
+  hipLaunchKernelGGL(k_cheb_first<T>, dim3(grid_for(n)), dim3(256), 0, s, n, b, P, x, d, inv_theta);
+  ATMO_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void cheb_step(long n, const T* b, const T* t, const T* P, T* x, T* d, T f1, T f2, hipStream_t s) {
+  hipLaunchKernelGGL(k_cheb_step<T>, dim3(grid_for(n)), dim3(256), 0, s, n, b, t, P, x, d, f1, f2);
+  ATMO_HIP_CHECK(hipGetLastError());
+}
+
+// ------------------------------------------------------------ coarse CG
+__device__ inline double wsum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;  // all lanes hold the sum
+}
+
+template <int K, int NC, bool W>
+__device__ void coarse_apply(const MeshDev& m, const double* rho, double coeff, const double* src, double* dst,
+                             long Ns, double* s_u, double* s_r, double* s_red) {
+  using C = Cfg<K>;
+  constexpr int ND2 = C::ND2, CPW = C::CPW;
+  const int passes = (m.ncell + CPW - 1) / CPW;
+  for (int p = 0; p < passes; ++p) {
+    const int cell0 = p * CPW;
+    const long base = (long)cell0 * ND2, lim = (long)m.ncell * ND2;
+    for (int t = threadIdx.x; t < CPW * ND2; t += 64) {
+      const long gi = base + t;
+#pragma unroll
+      for (int comp = 0; comp < NC; ++comp) s_u[comp * CPW * ND2 + t] = gi < lim ? src[comp * Ns + gi] : 0.0;
+      if (W) s_r[t] = gi < lim ? rho[gi] : 0.0;
+    }
+    __syncthreads();
+    wave_mass_core<K, double, NC, W>(m, cell0, s_u, s_r, coeff, s_red);
+    __syncthreads();
+    for (int t = threadIdx.x; t < CPW * ND2; t += 64) {
+      const long gi = base + t;
+      if (gi >= lim) break;
+#pragma unroll
+      for (int comp = 0; comp < NC; ++comp)
+        dst[comp * Ns + gi] = wave_mass_out<K, double>(s_red + comp * CPW * C::ND * C::NQ, t);
+    }
+    __syncthreads();
+  }
+}
+
+template <int K, int NC, bool W>
+__global__ __launch_bounds__(64) void k_coarse_cg(MeshDev m, const double* __restrict__ rho, double coeff,
+                                                  const double* __restrict__ b, double* __restrict__ x,
+                                                  double* __restrict__ g, double* __restrict__ d,
+                                                  double* __restrict__ h, const double* tolp, int max_its, int* its,
+                                                  long Ns) {
+  using C = Cfg<K>;
+  __shared__ double s_u[NC * C::CPW * C::ND2];
+  __shared__ double s_r[W ? C::CPW * C::ND2 : 1];
+  __shared__ double s_red[NC * C::CPW * C::ND * C::NQ];
+  const long n = (long)NC * Ns;
+  const double tol = *tolp;
+  double gg = 0;
+  for (long i = threadIdx.x; i < n; i += 64) {
+    const double gi = -b[i];
+    x[i] = 0.0;
+    g[i] = gi;
+    d[i] = -gi;
+    gg += gi * gi;
+  }
+  gg = wsum(gg);
+  double res = sqrt(gg);
+  int it = 0;
+  int status = 0;
+  if (res <= tol) status = 1;
+  else if (max_its <= 0) status = -1;
+  double gh = gg;
+  __syncthreads();
+  while (status == 0) {
+    ++it;
+    coarse_apply<K, NC, W>(m, rho, coeff, d, h, Ns, s_u, s_r, s_red);
+    double dh = 0;
+    for (long i = threadIdx.x; i < n; i += 64) dh += d[i] * h[i];
+    dh = wsum(dh);
+    const double alpha = gh / dh;
+    gg = 0;
+    for (long i = threadIdx.x; i < n; i += 64) {
+      x[i] += alpha * d[i];
+      const double gi = g[i] + alpha * h[i];
+      g[i] = gi;
+      gg += gi * gi;
+    }
+    gg = wsum(gg);
+    res = sqrt(gg);
+    if (res <= tol) { status = 1; break; }
+    if (it >= max_its || isnan(res)) { status = -1; break; }
+    const double beta = gg / gh;
+    gh = gg;
+    for (long i = threadIdx.x; i < n; i += 64) d[i] = -g[i] + beta * d[i];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *its = status == 1 ? it : -1;
+}
+
+template <int K>
+void launch_coarse_cg(const MeshDev& m, int ncomp, bool weighted, const double* rho, double coeff, const double* b,
+                      double* x, double* g, double* d, double* h, const double* tol, int max_its, int* its, long Ns,
+                      hipStream_t s) {
+  if (weighted)
+    hipLaunchKernelGGL((k_coarse_cg<K, 2, true>), dim3(1), dim3(64), 0, s, m, rho, coeff, b, x, g, d, h, tol, max_its,
+                       its, Ns);
+  else if (ncomp == 1)
+    hipLaunchKernelGGL((k_coarse_cg<K, 1, false>), dim3(1), dim3(64), 0, s, m, rho, coeff, b, x, g, d, h, tol,
+                       max_its, its, Ns);
+  else
+    hipLaunchKernelGGL((k_coarse_cg<K, 2, false>), dim3(1), dim3(64), 0, s, m, rho, coeff, b, x, g, d, h, tol,
+                       max_its, its, Ns);
+  ATMO_HIP_CHECK(hipGetLastError());
+}
+
+#define ATMO_MG_INST(K)                                                                                          \
+  template void launch_restrict<K, double>(const MeshDev&, const MeshDev&, const double*, double*, int, long, long, \
+                                           hipStream_t);                                                        \
+  template void launch_restrict<K, float>(const MeshDev&, const MeshDev&, const float*, float*, int, long, long,   \
+                                          hipStream_t);                                                         \
+  template void launch_prolongate_add<K, double>(const MeshDev&, const MeshDev&, const double*, double*, int, long, \
+                                                 long, hipStream_t);                                            \
+  template void launch_prolongate_add<K, float>(const MeshDev&, const MeshDev&, const float*, float*, int, long,    \
+                                                long, hipStream_t);                                             \
+  template void launch_project_down<K>(const MeshDev&, const MeshDev&, const double*, double*, hipStream_t);      \
+  template void launch_coarse_cg<K>(const MeshDev&, int, bool, const double*, double, const double*, double*,     \
+                                    double*, double*, double*, const double*, int, int*, long, hipStream_t);
+
+ATMO_MG_INST(1)
+ATMO_MG_INST(2)
+ATMO_MG_INST(3)
+
+template void cheb_first<double>(long, const double*, const double*, double*, double*, double, hipStream_t);
+template void cheb_first<float>(long, const float*, const float*, float*, float*, float, hipStream_t);
+template void cheb_step<double>(long, const double*, const double*, const double*, double*, double*, double, double,
+                                hipStream_t);
+template void cheb_step<float>(long, const float*, const float*, const float*, float*, float*, float, float,
+                               hipStream_t);
+
+}  // namespace atmo

@@ -1,0 +1,713 @@
+// Operator kernels of the IMEX-DG hot path (euler_operator.h).
+//
+// Two kernel families:
+//  * k_mass / k_inv_diag: the cell-local mass-type blocks M_rho, D and A
+//    (euler_operator.h:717-740, 1384-1408, 2328-2357) and their diagonals
+//    (:2456-2626), templated on the arithmetic type (FP64 and FP32 levels).
+//  * k_cellface<Phys>: cell integral plus the cell's own four faces for the
+//    operators with face terms (B, C and the explicit residuals). Each cell
+//    evaluates the flux of each of its faces with itself as the "+" side and
+//    its outward normal. All the reference's face fluxes are antisymmetric
+//    under exchanging the sides (n -> -n), so this equals the reference's
+//    face-loop contribution to the cell without any atomics; every face flux
+//    is computed twice (once per side).
+#include "ops.h"
+#include "wave.h"
+
+namespace atmo {
+
+Tableau::Tableau() {
+  g = 2.0 - std::sqrt(2.0);
+  a21 = g;
+  a22 = 0.0;
+  a31 = 0.5;
+  a32 = 0.5;
+  a33 = 0.0;
+  a21t = 0.5 * g;
+  a22t = 0.5 * g;
+  a31t = std::sqrt(2.0) / 4.0;
+  a32t = std::sqrt(2.0) / 4.0;
+  a33t = 1.0 - std::sqrt(2.0) / 2.0;
+  b1 = a31t;
+  b2 = a32t;
+  b3 = a33t;
+}
+
+// ------------------------------------------------------------ mass family
+template <int K, typename T, int NC, bool W>
+__global__ __launch_bounds__(64) void k_mass(MeshDev m, const T* __restrict__ src, const T* __restrict__ rho,
+                                             T* __restrict__ dst, T coeff, const T* __restrict__ add, T add_scale,
+                                             long Ns) {
+  using C = Cfg<K>;
+  constexpr int ND2 = C::ND2, CPW = C::CPW;
+  __shared__ T s_u[NC][CPW * ND2];
+  __shared__ T s_r[W ? CPW * ND2 : 1];
+  __shared__ T s_red[NC][CPW][C::ND][C::NQ];
+#pragma unroll
+  for (int comp = 0; comp < NC; ++comp) stage_cells<K>(s_u[comp], src + comp * Ns, m.ncell);
+  if (W) stage_cells<K>(s_r, rho, m.ncell);
+  __syncthreads();
+  wave_mass_core<K, T, NC, W>(m, blockIdx.x * CPW, &s_u[0][0], s_r, coeff, &s_red[0][0][0][0]);
+  __syncthreads();
+  const long base = (long)blockIdx.x * CPW * ND2;
+  const long lim = (long)m.ncell * ND2;
+  for (int t = threadIdx.x; t < CPW * ND2; t += 64) {
+    const long gi = base + t;
+    if (gi >= lim) break;
+#pragma unroll
+    for (int comp = 0; comp < NC; ++comp) {
+      T acc = wave_mass_out<K, T>(&s_red[comp][0][0][0], t);
+      if (add) acc += add_scale * add[comp * Ns + gi];
+      dst[comp * Ns + gi] = acc;
+    }
+  }
+}
+
+template <int K, typename T, bool W>
+__global__ __launch_bounds__(64) void k_inv_diag(MeshDev m, int ncomp, const T* __restrict__ rho, T* __restrict__ dst,
+                                                 T coeff, long Ns) {
+  using C = Cfg<K>;
+  constexpr int ND = C::ND, NQ = C::NQ, ND2 = C::ND2, CPW = C::CPW;
+  constexpr auto& tb = basis::kT<K>;
+  __shared__ T s_r[W ? CPW * ND2 : 1];
+  __shared__ T s_red[CPW][ND][NQ];
+  LaneMap<K> L(m.ncell);
+  if (W) stage_cells<K>(s_r, rho, m.ncell);
+  __syncthreads();
+  if (L.active) {
+    const int ci = L.cell % m.nx, cj = L.cell / m.nx;
+    const CellVerts v = cell_verts(m, ci, cj);
+    const ColGeo<T> g = col_geo<K, T>(v, L.c);
+    T Sc[ND], Dc[ND];
+    lane_rows<K, T>(L.c, Sc, Dc);
+    T wq[NQ];
+#pragma unroll
+    for (int r = 0; r < NQ; ++r) {
+      const T eta = (T)tb.gq[r];
+      const T xs = g.a0 + eta * g.a1, zs = g.b0 + eta * g.b1;
+      wq[r] = (xs * g.ze - g.xe * zs) * g.wc * (T)tb.gw[r] * coeff;
+    }
+    if (W) {
+      T tv[ND];
+#pragma unroll
+      for (int j = 0; j < ND; ++j) {
+        T s = 0;
+#pragma unroll
+        for (int i = 0; i < ND; ++i) s += Sc[i] * s_r[L.cw * ND2 + j * ND + i];
+        tv[j] = s;
+      }
+#pragma unroll
+      for (int r = 0; r < NQ; ++r) {
+        T s = 0;
+#pragma unroll
+        for (int j = 0; j < ND; ++j) s += (T)tb.S[r][j] * tv[j];
+        wq[r] *= s;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < ND; ++j) {
+      T s = 0;
+#pragma unroll
+      for (int r = 0; r < NQ; ++r) s += (T)(tb.S[r][j] * tb.S[r][j]) * wq[r];
+      s_red[L.cw][j][L.c] = s;
+    }
+  }
+  __syncthreads();
+  const long base = (long)blockIdx.x * CPW * ND2;
+  const long lim = (long)m.ncell * ND2;
+  for (int t = threadIdx.x; t < CPW * ND2; t += 64) {
+    const long gi = base + t;
+    if (gi >= lim) break;
+    const int cw = t / ND2, o = t - cw * ND2, jj = o / ND, ii = o - jj * ND;
+    T Si[NQ], Di[NQ];
+    lane_cols<K, T>(ii, Si, Di);
+    T acc = 0;
+#pragma unroll
+    for (int c = 0; c < NQ; ++c) acc += Si[c] * Si[c] * s_red[cw][jj][c];
+    const T inv = (T)1 / acc;
+    for (int comp = 0; comp < ncomp; ++comp) dst[comp * Ns + gi] = inv;
+  }
+}
+
+template <int K, typename T>
+void launch_mass(const MeshDev& m, int ncomp, bool weighted, const T* src, const T* rho, T* dst, T coeff,
+                 const T* add, T add_scale, long Ns, hipStream_t s) {
+  const int grid = ceil_div(m.ncell, Cfg<K>::CPW);
+  if (grid == 0) return;
+  if (weighted) {
+    if (ncomp != 2) throw std::invalid_argument("weighted mass operator expects 2 components");
+    hipLaunchKernelGGL((k_mass<K, T, 2, true>), dim3(grid), dim3(64), 0, s, m, src, rho, dst, coeff, add,
+                       add_scale, Ns);
+  } else if (ncomp == 1) {
+    hipLaunchKernelGGL((k_mass<K, T, 1, false>), dim3(grid), dim3(64), 0, s, m, src, rho, dst, coeff, add,
+                       add_scale, Ns);
+  } else {
+    hipLaunchKernelGGL((k_mass<K, T, 2, false>), dim3(grid), dim3(64), 0, s, m, src, rho, dst, coeff, add,
+                       add_scale, Ns);
+  }
+  ATMO_HIP_CHECK(hipGetLastError());
+}
+
+template <int K, typename T>
+void launch_inv_diag(const MeshDev& m, int ncomp, bool weighted, const T* rho, T* dst, T coeff, long Ns,
+                     hipStream_t s) {
+  const int grid = ceil_div(m.ncell, Cfg<K>::CPW);
+  if (grid == 0) return;
+  if (weighted)
+    hipLaunchKernelGGL((k_inv_diag<K, T, true>), dim3(grid), dim3(64), 0, s, m, ncomp, rho, dst, coeff, Ns);
+  else
+    hipLaunchKernelGGL((k_inv_diag<K, T, false>), dim3(grid), dim3(64), 0, s, m, ncomp, rho, dst, coeff, Ns);
+  ATMO_HIP_CHECK(hipGetLastError());
+}
+
+// ------------------------------------------------------- cell+face family
+template <int NF>
+struct FieldPtrs {
+  const double* p[NF];
+};
+
+// B: euler_operator.h:1510-1617 (cell: submit_divergence, face: {p}[v.n],
+// wall: p_D = p)
+struct PhysB {
+  static constexpr int NF = 1, NO = 2;
+  static constexpr bool BDRY = true;
+  double cf;  // coeff*dt/Ma^2
+  __device__ void cell(const double (&v)[NF], double (&V)[NO], double (&Gx)[NO], double (&Gz)[NO]) const {
+    V[0] = V[1] = 0;
+    Gx[0] = -cf * v[0];
+    Gz[0] = 0;
+    Gx[1] = 0;
+    Gz[1] = -cf * v[0];
+  }
+  __device__ void face(const double (&a)[NF], const double (&b)[NF], double n0, double n1, double (&F)[NO]) const {
+    const double avg = 0.5 * (a[0] + b[0]);
+    F[0] = cf * avg * n0;
+    F[1] = cf * avg * n1;
+  }
+  __device__ void bdry(const double (&a)[NF], double n0, double n1, double (&F)[NO]) const {
+    const double pD = a[0];
+    F[0] = cf * 0.5 * (a[0] + pD) * n0;
+    F[1] = cf * 0.5 * (a[0] + pD) * n1;
+  }
+};
+
+// C: euler_operator.h:1411-1507 (nonlinear in u through lambda)
+struct PhysC {
+  static constexpr int NF = 3, NO = 1;  // pfix, ux, uz
+  static constexpr bool BDRY = false;
+  double cdt, gm;
+  __device__ void cell(const double (&v)[NF], double (&V)[NO], double (&Gx)[NO], double (&Gz)[NO]) const {
+    const double s = -cdt * gm / (gm - 1.0) * v[0];
+    V[0] = 0;
+    Gx[0] = s * v[1];
+    Gz[0] = s * v[2];
+  }
+  __device__ void face(const double (&a)[NF], const double (&b)[NF], double n0, double n1, double (&F)[NO]) const {
+    const double ax = 0.5 * gm / (gm - 1.0) * (a[0] * a[1] + b[0] * b[1]);
+    const double az = 0.5 * gm / (gm - 1.0) * (a[0] * a[2] + b[0] * b[2]);
+    const double lam = fmax(fabs(a[1] * n0 + a[2] * n1), fabs(b[1] * n0 + b[2] * n1));
+    const double jump = 1.0 / (gm - 1.0) * (a[0] - b[0]);
+    F[0] = cdt * (ax * n0 + az * n1 + 0.5 * lam * jump);
+  }
+  __device__ void bdry(const double (&)[NF], double, double, double (&F)[NO]) const { F[0] = 0; }
+};
+
+// density residual: euler_operator.h:344-692
+template <int ST>
+struct PhysRhsRho {
+  static constexpr int NF = 3 * ST, NO = 1;  // (rho, ux, uz) per stage
+  static constexpr bool BDRY = false;
+  double wdt[3];
+  __device__ void cell(const double (&v)[NF], double (&V)[NO], double (&Gx)[NO], double (&Gz)[NO]) const {
+    double gx = 0, gz = 0;
+#pragma unroll
+    for (int s = 0; s < ST; ++s) {
+      gx += wdt[s] * v[3 * s] * v[3 * s + 1];
+      gz += wdt[s] * v[3 * s] * v[3 * s + 2];
+    }
+    V[0] = v[0];
+    Gx[0] = gx;
+    Gz[0] = gz;
+  }
+  __device__ void face(const double (&a)[NF], const double (&b)[NF], double n0, double n1, double (&F)[NO]) const {
+    double acc = 0;
+#pragma unroll
+    for (int s = 0; s < ST; ++s) {
+      const double ra = a[3 * s], rb = b[3 * s];
+      const double ax = 0.5 * (ra * a[3 * s + 1] + rb * b[3 * s + 1]);
+      const double az = 0.5 * (ra * a[3 * s + 2] + rb * b[3 * s + 2]);
+      const double lam = fmax(fabs(a[3 * s + 1] * n0 + a[3 * s + 2] * n1), fabs(b[3 * s + 1] * n0 + b[3 * s + 2] * n1));
+      acc += wdt[s] * (ax * n0 + az * n1 + 0.5 * lam * (ra - rb));
+    }
+    F[0] = -acc;
+  }
+  __device__ void bdry(const double (&)[NF], double, double, double (&F)[NO]) const { F[0] = 0; }
+};
+
+// energy (pressure) residual: euler_operator.h:745-1361
+template <int ST>
+struct PhysRhsP {
+  static constexpr int NF = 4 * ST + 3, NO = 1;  // (rho,ux,uz,p) per stage, then rho_k, uk_x, uk_z
+  static constexpr bool BDRY = false;
+  double wa[3], wt[3], gwt[3];  // advective, enthalpy and gravity weights
+  double dt, Ma2, gm, iFr2, G, e0, e1;
+  __device__ void cell(const double (&v)[NF], double (&V)[NO], double (&Gx)[NO], double (&Gz)[NO]) const {
+    constexpr int KB = 4 * ST;
+    const double rk = v[KB], ukx = v[KB + 1], ukz = v[KB + 2];
+    double E[ST], uu[ST];
+#pragma unroll
+    for (int s = 0; s < ST; ++s) {
+      uu[s] = v[4 * s + 1] * v[4 * s + 1] + v[4 * s + 2] * v[4 * s + 2];
+      E[s] = 1.0 / (gm - 1.0) * (v[4 * s + 3] / v[4 * s]) + 0.5 * Ma2 * uu[s];
+    }
+    double val = v[0] * E[0] - 0.5 * rk * Ma2 * (ukx * ukx + ukz * ukz);
+    // gravity work: explicit stages use their own (rho, u); the implicit
+    // weight uses (rho_k, u_k) in stages 1 and 2
+#pragma unroll
+    for (int s = 0; s < ST; ++s)
+      if (ST == 3 || s < ST)
+        val -= gwt[s] * dt * Ma2 * iFr2 * v[4 * s] * (v[4 * s + 1] * e0 + v[4 * s + 2] * e1) * G;
+    if (ST < 3) val -= gwt[ST] * dt * Ma2 * iFr2 * rk * (ukx * e0 + ukz * e1) * G;
+    double gx = 0, gz = 0;
+#pragma unroll
+    for (int s = 0; s < ST; ++s) {
+      const double kin = 0.5 * wa[s] * dt * Ma2 * uu[s] * v[4 * s];
+      const double ent = wt[s] * dt * (v[4 * s] * (E[s] - 0.5 * Ma2 * uu[s]) + v[4 * s + 3]);
+      gx += kin * v[4 * s + 1] + ent * v[4 * s + 1];
+      gz += kin * v[4 * s + 2] + ent * v[4 * s + 2];
+    }
+    V[0] = val;
+    Gx[0] = gx;
+    Gz[0] = gz;
+  }
+  __device__ void face(const double (&a)[NF], const double (&b)[NF], double n0, double n1, double (&F)[NO]) const {
+    double acc = 0;
+#pragma unroll
+    for (int s = 0; s < ST; ++s) {
+      const double ra = a[4 * s], rb = b[4 * s];
+      const double uxa = a[4 * s + 1], uza = a[4 * s + 2], uxb = b[4 * s + 1], uzb = b[4 * s + 2];
+      const double pa = a[4 * s + 3], pb = b[4 * s + 3];
+      const double ua2 = uxa * uxa + uza * uza, ub2 = uxb * uxb + uzb * uzb;
+      const double akx = 0.5 * (0.5 * ua2 * ra * uxa + 0.5 * ub2 * rb * uxb);
+      const double akz = 0.5 * (0.5 * ua2 * ra * uza + 0.5 * ub2 * rb * uzb);
+      const double Ea = 1.0 / (gm - 1.0) * pa / ra + 0.5 * Ma2 * ua2;
+      const double Eb = 1.0 / (gm - 1.0) * pb / rb + 0.5 * Ma2 * ub2;
+      const double ha = (Ea - 0.5 * Ma2 * ua2) * ra + pa;
+      const double hb = (Eb - 0.5 * Ma2 * ub2) * rb + pb;
+      const double aex = 0.5 * (ha * uxa + hb * uxb);
+      const double aez = 0.5 * (ha * uza + hb * uzb);
+      const double lam = fmax(fabs(uxa * n0 + uza * n1), fabs(uxb * n0 + uzb * n1));
+      if (ST < 3) {
+        double jkin;
+        if (ST == 2 && s == 1) {
+          // euler_operator.h:1182-1183: u_tmp_2 . u_old, as written
+          jkin = ra * 0.5 * (uxa * a[1] + uza * a[2]) - rb * 0.5 * (uxb * b[1] + uzb * b[2]);
+        } else {
+          jkin = ra * 0.5 * ua2 - rb * 0.5 * ub2;
+        }
+        const double je = ra * (Ea - 0.5 * Ma2 * ua2) - rb * (Eb - 0.5 * Ma2 * ub2);
+        acc += -wa[s] * dt * Ma2 * (akx * n0 + akz * n1 + 0.5 * lam * jkin) - wt[s] * dt * (aex * n0 + aez * n1 + 0.5 * lam * je);
+      } else {
+        const double jrE = ra * Ea - rb * Eb;
+        acc += -wa[s] * dt * Ma2 * (akx * n0 + akz * n1) - wa[s] * dt * (aex * n0 + aez * n1) - wa[s] * dt * 0.5 * lam * jrE;
+      }
+    }
+    F[0] = acc;
+  }
+  __device__ void bdry(const double (&)[NF], double, double, double (&F)[NO]) const { F[0] = 0; }
+};
+
+// momentum (velocity) residual: euler_operator.h:1620-2305
+template <int ST>
+struct PhysRhsU {
+  static constexpr int NX = ST < 3 ? 1 : 0;  // extra gravity density (rho_2 resp. rho_3)
+  static constexpr int NF = 4 * ST + NX, NO = 2;
+  static constexpr bool BDRY = true;
+  double wa[3], wt[3], gwt[3];
+  double dt, Ma2, iFr2, G, e0, e1;
+  __device__ void cell(const double (&v)[NF], double (&V)[NO], double (&Gx)[NO], double (&Gz)[NO]) const {
+    double grho = 0;  // sum of gravity weight * density
+#pragma unroll
+    for (int s = 0; s < ST; ++s) grho += gwt[s] * v[4 * s];
+    if (NX) grho += gwt[ST] * v[4 * ST];
+    (void)grho;
+    double gs[NO];
+#pragma unroll
+    for (int comp = 0; comp < NO; ++comp) {
+      const double e = comp == 0 ? e0 : e1;
+      double val = v[0] * v[1 + comp];
+#pragma unroll
+      for (int s = 0; s < ST; ++s) val -= gwt[s] * dt * iFr2 * v[4 * s] * e * G;
+      if (NX) val -= gwt[ST] * dt * iFr2 * v[4 * ST] * e * G;
+      V[comp] = val;
+      double gx = 0, gz = 0;
+#pragma unroll
+      for (int s = 0; s < ST; ++s) {
+        const double uc = v[4 * s + 1 + comp];
+        gx += wa[s] * dt * v[4 * s] * uc * v[4 * s + 1] + (comp == 0 ? wt[s] * dt / Ma2 * v[4 * s + 3] : 0.0);
+        gz += wa[s] * dt * v[4 * s] * uc * v[4 * s + 2] + (comp == 1 ? wt[s] * dt / Ma2 * v[4 * s + 3] : 0.0);
+      }
+      Gx[comp] = gx;
+      Gz[comp] = gz;
+      gs[comp] = 0;
+    }
+    (void)gs;
+  }
+  __device__ void face(const double (&a)[NF], const double (&b)[NF], double n0, double n1, double (&F)[NO]) const {
+#pragma unroll
+    for (int comp = 0; comp < NO; ++comp) {
+      const double nc = comp == 0 ? n0 : n1;
+      double acc = 0;
+#pragma unroll
+      for (int s = 0; s < ST; ++s) {
+        const double ra = a[4 * s], rb = b[4 * s];
+        const double uxa = a[4 * s + 1], uza = a[4 * s + 2], uxb = b[4 * s + 1], uzb = b[4 * s + 2];
+        const double uca = a[4 * s + 1 + comp], ucb = b[4 * s + 1 + comp];
+        const double tn = 0.5 * (ra * uca * (uxa * n0 + uza * n1) + rb * ucb * (uxb * n0 + uzb * n1));
+        const double avgp = 0.5 * (a[4 * s + 3] + b[4 * s + 3]);
+        const double jr = ra * uca - rb * ucb;
+        const double lam = fmax(fabs(uxa * n0 + uza * n1), fabs(uxb * n0 + uzb * n1));
+        acc += -wa[s] * dt * tn - wt[s] * dt / Ma2 * avgp * nc - wa[s] * dt * 0.5 * lam * jr;
+      }
+      F[comp] = acc;
+    }
+  }
+  __device__ void bdry(const double (&a)[NF], double n0, double n1, double (&F)[NO]) const {
+#pragma unroll
+    for (int comp = 0; comp < NO; ++comp) {
+      const double nc = comp == 0 ? n0 : n1;
+      double acc = 0;
+#pragma unroll
+      for (int s = 0; s < ST; ++s) {
+        const double pD = a[4 * s + 3];
+        acc += -wt[s] * dt / Ma2 * 0.5 * (a[4 * s + 3] + pD) * nc;
+      }
+      F[comp] = acc;
+    }
+  }
+};
+
+template <int K, class Ph>
+__global__ __launch_bounds__(64) void k_cellface(MeshDev m, Ph ph, FieldPtrs<Ph::NF> in, double* __restrict__ dst,
+                                                 long Ns) {
+  using C = Cfg<K>;
+  constexpr int ND = C::ND, NQ = C::NQ, ND2 = C::ND2, CPW = C::CPW;
+  constexpr int NF = Ph::NF, NO = Ph::NO;
+  constexpr auto& tb = basis::kT<K>;
+  __shared__ double s_in[NF][CPW * ND2];
+  __shared__ double s_T[NO][2][CPW][ND][NQ];
+  __shared__ double s_F[NO][2][CPW][NQ];
+  LaneMap<K> L(m.ncell);
+#pragma unroll
+  for (int f = 0; f < NF; ++f) stage_cells<K>(s_in[f], in.p[f], m.ncell);
+  __syncthreads();
+  if (L.active) {
+    const int ci = L.cell % m.nx, cj = L.cell / m.nx;
+    const CellVerts v = cell_verts(m, ci, cj);
+    const ColGeo<double> g = col_geo<K, double>(v, L.c);
+    double Sc[ND], Dc[ND];
+    lane_rows<K, double>(L.c, Sc, Dc);
+    const int off = L.cw * ND2;
+    double tv[NF][ND];
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int j = 0; j < ND; ++j) {
+        double s = 0;
+#pragma unroll
+        for (int i = 0; i < ND; ++i) s += Sc[i] * s_in[f][off + j * ND + i];
+        tv[f][j] = s;
+      }
+    double Tv[NO][ND], Td[NO][ND];
+#pragma unroll
+    for (int o = 0; o < NO; ++o)
+#pragma unroll
+      for (int j = 0; j < ND; ++j) Tv[o][j] = Td[o][j] = 0;
+#pragma unroll
+    for (int r = 0; r < NQ; ++r) {
+      double val[NF];
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        double s = 0;
+#pragma unroll
+        for (int j = 0; j < ND; ++j) s += tb.S[r][j] * tv[f][j];
+        val[f] = s;
+      }
+      double V[NO], Gx[NO], Gz[NO];
+      ph.cell(val, V, Gx, Gz);
+      const double eta = tb.gq[r];
+      const double xs = g.a0 + eta * g.a1, zs = g.b0 + eta * g.b1;
+      const double w = g.wc * tb.gw[r];
+      const double JxW = (xs * g.ze - g.xe * zs) * w;
+#pragma unroll
+      for (int o = 0; o < NO; ++o) {
+        const double Vq = V[o] * JxW;
+        const double Gxi = w * (g.ze * Gx[o] - g.xe * Gz[o]);
+        const double Geta = w * (-zs * Gx[o] + xs * Gz[o]);
+#pragma unroll
+        for (int j = 0; j < ND; ++j) {
+          Tv[o][j] += tb.S[r][j] * Vq + tb.D[r][j] * Geta;
+          Td[o][j] += tb.S[r][j] * Gxi;
+        }
+      }
+    }
+    // bottom (j-1) and top (j+1) faces: this lane is face point c
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      const int f = 2 + side;
+      const FaceGeo fg = face_geo(v, f);
+      const double wl = fg.len * g.wc;
+      double own[NF], nb[NF], F[NO];
+#pragma unroll
+      for (int q = 0; q < NF; ++q) own[q] = tv[q][side ? K : 0];
+      const int nj = side ? cj + 1 : cj - 1;
+      if (nj >= 0 && nj < m.nz) {
+        const long nbase = (long)(nj * m.nx + ci) * ND2 + (side ? 0 : K * ND);
+#pragma unroll
+        for (int q = 0; q < NF; ++q) {
+          double s = 0;
+#pragma unroll
+          for (int i = 0; i < ND; ++i) s += Sc[i] * in.p[q][nbase + i];
+          nb[q] = s;
+        }
+        ph.face(own, nb, fg.n0, fg.n1, F);
+      } else if (Ph::BDRY) {
+        ph.bdry(own, fg.n0, fg.n1, F);
+      } else {
+#pragma unroll
+        for (int o = 0; o < NO; ++o) F[o] = 0;
+      }
+#pragma unroll
+      for (int o = 0; o < NO; ++o) Tv[o][side ? K : 0] += F[o] * wl;
+    }
+    // left (i-1) and right (i+1) faces: this lane is face point s = c
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      const int f = side;
+      const FaceGeo fg = face_geo(v, f);
+      const double wl = fg.len * g.wc;
+      double own[NF], nb[NF], F[NO];
+      const int col = side ? K : 0;
+#pragma unroll
+      for (int q = 0; q < NF; ++q) {
+        double s = 0;
+#pragma unroll
+        for (int j = 0; j < ND; ++j) s += Sc[j] * s_in[q][off + j * ND + col];
+        own[q] = s;
+      }
+      int ni = side ? ci + 1 : ci - 1;
+      bool inner = true;
+      if (ni < 0 || ni >= m.nx) {
+        if (m.periodic) ni = (ni + m.nx) % m.nx;
+        else inner = false;
+      }
+      if (inner) {
+        const long nbase = (long)(cj * m.nx + ni) * ND2 + (side ? 0 : K);
+#pragma unroll
+        for (int q = 0; q < NF; ++q) {
+          double s = 0;
+#pragma unroll
+          for (int j = 0; j < ND; ++j) s += Sc[j] * in.p[q][nbase + j * ND];
+          nb[q] = s;
+        }
+        ph.face(own, nb, fg.n0, fg.n1, F);
+      } else if (Ph::BDRY) {
+        ph.bdry(own, fg.n0, fg.n1, F);
+      } else {
+#pragma unroll
+        for (int o = 0; o < NO; ++o) F[o] = 0;
+      }
+#pragma unroll
+      for (int o = 0; o < NO; ++o) s_F[o][side][L.cw][L.c] = F[o] * wl;
+    }
+#pragma unroll
+    for (int o = 0; o < NO; ++o)
+#pragma unroll
+      for (int j = 0; j < ND; ++j) {
+        s_T[o][0][L.cw][j][L.c] = Tv[o][j];
+        s_T[o][1][L.cw][j][L.c] = Td[o][j];
+      }
+  }
+  __syncthreads();
+  const long base = (long)blockIdx.x * CPW * ND2;
+  const long lim = (long)m.ncell * ND2;
+  for (int t = threadIdx.x; t < CPW * ND2; t += 64) {
+    const long gi = base + t;
+    if (gi >= lim) break;
+    const int cw = t / ND2, o2 = t - cw * ND2, jj = o2 / ND, ii = o2 - jj * ND;
+    double Si[NQ], Di[NQ], Sj[NQ], Dj[NQ];
+    lane_cols<K, double>(ii, Si, Di);
+    lane_cols<K, double>(jj, Sj, Dj);
+#pragma unroll
+    for (int o = 0; o < NO; ++o) {
+      double acc = 0;
+#pragma unroll
+      for (int c = 0; c < NQ; ++c) acc += Si[c] * s_T[o][0][cw][jj][c] + Di[c] * s_T[o][1][cw][jj][c];
+      if (ii == 0)
+#pragma unroll
+        for (int c = 0; c < NQ; ++c) acc += Sj[c] * s_F[o][0][cw][c];
+      if (ii == K)
+#pragma unroll
+        for (int c = 0; c < NQ; ++c) acc += Sj[c] * s_F[o][1][cw][c];
+      dst[o * Ns + gi] = acc;
+    }
+  }
+}
+
+template <int K, class Ph>
+static void launch_cf(const MeshDev& m, const Ph& ph, const FieldPtrs<Ph::NF>& in, double* dst, long Ns,
+                      hipStream_t s) {
+  const int grid = ceil_div(m.ncell, Cfg<K>::CPW);
+  if (grid == 0) return;
+  hipLaunchKernelGGL((k_cellface<K, Ph>), dim3(grid), dim3(64), 0, s, m, ph, in, dst, Ns);
+  ATMO_HIP_CHECK(hipGetLastError());
+}
+
+template <int K>
+void launch_B(const MeshDev& m, const OpConsts& oc, double coeff, const double* p, double* dst_u, long Ns,
+              hipStream_t s) {
+  PhysB ph;
+  ph.cf = coeff * oc.dt / (oc.Ma * oc.Ma);
+  FieldPtrs<1> in{{p}};
+  launch_cf<K>(m, ph, in, dst_u, Ns, s);
+}
+
+template <int K>
+void launch_C(const MeshDev& m, const OpConsts& oc, double coeff, const double* pfix, const double* u,
+              double* dst_p, long Ns, hipStream_t s) {
+  PhysC ph;
+  ph.cdt = coeff * oc.dt;
+  ph.gm = oc.gamma_gas;
+  FieldPtrs<3> in{{pfix, u, u + Ns}};
+  launch_cf<K>(m, ph, in, dst_p, Ns, s);
+}
+
+template <int K>
+void launch_rhs_rho(const MeshDev& m, const OpConsts& oc, const Tableau& tb, int stage, const double* const* src,
+                    double* dst, long Ns, hipStream_t s) {
+  if (stage == 1) {
+    PhysRhsRho<1> ph;
+    ph.wdt[0] = tb.a21 * oc.dt;
+    FieldPtrs<3> in{{src[0], src[1], src[1] + Ns}};
+    launch_cf<K>(m, ph, in, dst, Ns, s);
+  } else if (stage == 2) {
+    PhysRhsRho<2> ph;
+    ph.wdt[0] = tb.a31 * oc.dt;
+    ph.wdt[1] = tb.a32 * oc.dt;
+    FieldPtrs<6> in{{src[0], src[1], src[1] + Ns, src[2], src[3], src[3] + Ns}};
+    launch_cf<K>(m, ph, in, dst, Ns, s);
+  } else if (stage == 3) {
+    PhysRhsRho<3> ph;
+    ph.wdt[0] = tb.b1 * oc.dt;
+    ph.wdt[1] = tb.b2 * oc.dt;
+    ph.wdt[2] = tb.b3 * oc.dt;
+    FieldPtrs<9> in{{src[0], src[1], src[1] + Ns, src[2], src[3], src[3] + Ns, src[4], src[5], src[5] + Ns}};
+    launch_cf<K>(m, ph, in, dst, Ns, s);
+  } else {
+    throw std::invalid_argument("rhs_rho: stage must be 1..3");
+  }
+}
+
+template <int ST>
+static void fill_weights(const Tableau& tb, double (&wa)[3], double (&wt)[3], double (&gw)[3]) {
+  if (ST == 1) {
+    wa[0] = tb.a21; wt[0] = tb.a21t;
+    gw[0] = tb.a21t; gw[1] = tb.a22t;
+  } else if (ST == 2) {
+    wa[0] = tb.a31; wt[0] = tb.a31t; wa[1] = tb.a32; wt[1] = tb.a32t;
+    gw[0] = tb.a31t; gw[1] = tb.a32t; gw[2] = tb.a33t;
+  } else {
+    wa[0] = wt[0] = tb.b1; wa[1] = wt[1] = tb.b2; wa[2] = wt[2] = tb.b3;
+    gw[0] = tb.b1; gw[1] = tb.b2; gw[2] = tb.b3;
+  }
+}
+
+template <int K, int ST>
+static void rhs_p_stage(const MeshDev& m, const OpConsts& oc, const Tableau& tb, const double* const* src,
+                        double* dst, long Ns, hipStream_t s) {
+  PhysRhsP<ST> ph;
+  fill_weights<ST>(tb, ph.wa, ph.wt, ph.gwt);
+  ph.dt = oc.dt;
+  ph.Ma2 = oc.Ma * oc.Ma;
+  ph.gm = oc.gamma_gas;
+  ph.iFr2 = 1.0 / (oc.Fr * oc.Fr);
+  ph.G = oc.grav_factor;
+  ph.e0 = oc.grav[0];
+  ph.e1 = oc.grav[1];
+  FieldPtrs<PhysRhsP<ST>::NF> in;
+  for (int t = 0; t < ST; ++t) {
+    in.p[4 * t] = src[3 * t];
+    in.p[4 * t + 1] = src[3 * t + 1];
+    in.p[4 * t + 2] = src[3 * t + 1] + Ns;
+    in.p[4 * t + 3] = src[3 * t + 2];
+  }
+  in.p[4 * ST] = src[3 * ST];
+  in.p[4 * ST + 1] = src[3 * ST + 1];
+  in.p[4 * ST + 2] = src[3 * ST + 1] + Ns;
+  launch_cf<K>(m, ph, in, dst, Ns, s);
+}
+
+template <int K>
+void launch_rhs_pressure(const MeshDev& m, const OpConsts& oc, const Tableau& tb, int stage,
+                         const double* const* src, double* dst, long Ns, hipStream_t s) {
+  if (stage == 1) rhs_p_stage<K, 1>(m, oc, tb, src, dst, Ns, s);
+  else if (stage == 2) rhs_p_stage<K, 2>(m, oc, tb, src, dst, Ns, s);
+  else if (stage == 3) rhs_p_stage<K, 3>(m, oc, tb, src, dst, Ns, s);
+  else throw std::invalid_argument("rhs_pressure: stage must be 1..3");
+}
+
+template <int K, int ST>
+static void rhs_u_stage(const MeshDev& m, const OpConsts& oc, const Tableau& tb, const double* const* src,
+                        double* dst, long Ns, hipStream_t s) {
+  PhysRhsU<ST> ph;
+  fill_weights<ST>(tb, ph.wa, ph.wt, ph.gwt);
+  ph.dt = oc.dt;
+  ph.Ma2 = oc.Ma * oc.Ma;
+  ph.iFr2 = 1.0 / (oc.Fr * oc.Fr);
+  ph.G = oc.grav_factor;
+  ph.e0 = oc.grav[0];
+  ph.e1 = oc.grav[1];
+  FieldPtrs<PhysRhsU<ST>::NF> in;
+  for (int t = 0; t < ST; ++t) {
+    in.p[4 * t] = src[3 * t];
+    in.p[4 * t + 1] = src[3 * t + 1];
+    in.p[4 * t + 2] = src[3 * t + 1] + Ns;
+    in.p[4 * t + 3] = src[3 * t + 2];
+  }
+  if (PhysRhsU<ST>::NX) in.p[4 * ST] = src[3 * ST];
+  launch_cf<K>(m, ph, in, dst, Ns, s);
+}
+
+template <int K>
+void launch_rhs_velocity(const MeshDev& m, const OpConsts& oc, const Tableau& tb, int stage,
+                         const double* const* src, double* dst, long Ns, hipStream_t s) {
+  if (stage == 1) rhs_u_stage<K, 1>(m, oc, tb, src, dst, Ns, s);
+  else if (stage == 2) rhs_u_stage<K, 2>(m, oc, tb, src, dst, Ns, s);
+  else if (stage == 3) rhs_u_stage<K, 3>(m, oc, tb, src, dst, Ns, s);
+  else throw std::invalid_argument("rhs_velocity: stage must be 1..3");
+}
+
+#define ATMO_INST_K(K)                                                                                          \
+  template void launch_mass<K, double>(const MeshDev&, int, bool, const double*, const double*, double*, double, \
+                                       const double*, double, long, hipStream_t);                               \
+  template void launch_mass<K, float>(const MeshDev&, int, bool, const float*, const float*, float*, float,      \
+                                      const float*, float, long, hipStream_t);                                  \
+  template void launch_inv_diag<K, double>(const MeshDev&, int, bool, const double*, double*, double, long,      \
+                                           hipStream_t);                                                        \
+  template void launch_inv_diag<K, float>(const MeshDev&, int, bool, const float*, float*, float, long,          \
+                                          hipStream_t);                                                         \
+  template void launch_B<K>(const MeshDev&, const OpConsts&, double, const double*, double*, long, hipStream_t);   \
+  template void launch_C<K>(const MeshDev&, const OpConsts&, double, const double*, const double*, double*, long, \
+                            hipStream_t);                                                                       \
+  template void launch_rhs_rho<K>(const MeshDev&, const OpConsts&, const Tableau&, int, const double* const*,     \
+                                  double*, long, hipStream_t);                                                  \
+  template void launch_rhs_pressure<K>(const MeshDev&, const OpConsts&, const Tableau&, int,                     \
+                                       const double* const*, double*, long, hipStream_t);                       \
+  template void launch_rhs_velocity<K>(const MeshDev&, const OpConsts&, const Tableau&, int,                     \
+                                       const double* const*, double*, long, hipStream_t);
+
+ATMO_INST_K(1)
+ATMO_INST_K(2)
+ATMO_INST_K(3)
+
+}  // namespace atmo

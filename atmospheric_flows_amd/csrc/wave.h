@@ -1,0 +1,122 @@
+// Wave-level building blocks shared by the operator, Krylov and multigrid
+// kernels: the sum-factorized mass-type apply of CPW cells held in LDS.
+#pragma once
+#include "common.h"
+
+namespace atmo {
+
+// Per-lane table columns for a lane-dependent dof index m: Si[c] = S[c][m].
+template <int K, typename T>
+__device__ inline void lane_cols(int m, T (&Si)[Cfg<K>::NQ], T (&Di)[Cfg<K>::NQ]) {
+  constexpr auto& tb = basis::kT<K>;
+#pragma unroll
+  for (int c = 0; c < Cfg<K>::NQ; ++c) {
+    Si[c] = 0;
+    Di[c] = 0;
+  }
+#pragma unroll
+  for (int mm = 0; mm < Cfg<K>::ND; ++mm)
+    if (mm == m) {
+#pragma unroll
+      for (int c = 0; c < Cfg<K>::NQ; ++c) {
+        Si[c] = (T)tb.S[c][mm];
+        Di[c] = (T)tb.D[c][mm];
+      }
+    }
+}
+
+// Quadrature weights of the lane's column: wq[r] = JxW(c, r) * coeff, times
+// the interpolated density rho(c, r) when W (the A operator).
+template <int K, typename T, bool W>
+__device__ inline void column_weights(const MeshDev& m, int cell, int c, int cw, const T* s_r, T coeff,
+                                      T (&wq)[Cfg<K>::NQ], const T (&Sc)[Cfg<K>::ND]) {
+  constexpr int ND = Cfg<K>::ND, NQ = Cfg<K>::NQ, ND2 = Cfg<K>::ND2;
+  constexpr auto& tb = basis::kT<K>;
+  const int ci = cell % m.nx, cj = cell / m.nx;
+  const CellVerts v = cell_verts(m, ci, cj);
+  const ColGeo<T> g = col_geo<K, T>(v, c);
+#pragma unroll
+  for (int r = 0; r < NQ; ++r) {
+    const T eta = (T)tb.gq[r];
+    const T xs = g.a0 + eta * g.a1, zs = g.b0 + eta * g.b1;
+    wq[r] = (xs * g.ze - g.xe * zs) * g.wc * (T)tb.gw[r] * coeff;
+  }
+  if (W) {
+    T tv[ND];
+#pragma unroll
+    for (int j = 0; j < ND; ++j) {
+      T s = 0;
+#pragma unroll
+      for (int i = 0; i < ND; ++i) s += Sc[i] * s_r[cw * ND2 + j * ND + i];
+      tv[j] = s;
+    }
+#pragma unroll
+    for (int r = 0; r < NQ; ++r) {
+      T s = 0;
+#pragma unroll
+      for (int j = 0; j < ND; ++j) s += (T)tb.S[r][j] * tv[j];
+      wq[r] *= s;
+    }
+  }
+}
+
+// Column phase of the mass-type apply for the CPW cells starting at cell0:
+// s_u (LDS, [NC][CPW*ND2]) -> s_red (LDS, [NC][CPW][ND][NQ]) holding
+// Tv[j](c) = sum_r S[r][j] w(c,r) u(c,r). Callers barrier before reading s_red.
+template <int K, typename T, int NC, bool W>
+__device__ inline void wave_mass_core(const MeshDev& m, int cell0, const T* s_u, const T* s_r, T coeff, T* s_red) {
+  using C = Cfg<K>;
+  constexpr int ND = C::ND, NQ = C::NQ, ND2 = C::ND2, CPW = C::CPW;
+  constexpr auto& tb = basis::kT<K>;
+  const int lane = threadIdx.x & 63;
+  const int cw = lane / NQ, c = lane - cw * NQ;
+  const int cell = cell0 + cw;
+  if (cw >= CPW || cell >= m.ncell) return;
+  T Sc[ND], Dc[ND];
+  lane_rows<K, T>(c, Sc, Dc);
+  T wq[NQ];
+  column_weights<K, T, W>(m, cell, c, cw, s_r, coeff, wq, Sc);
+#pragma unroll
+  for (int comp = 0; comp < NC; ++comp) {
+    const T* u = s_u + comp * CPW * ND2 + cw * ND2;
+    T tv[ND];
+#pragma unroll
+    for (int j = 0; j < ND; ++j) {
+      T s = 0;
+#pragma unroll
+      for (int i = 0; i < ND; ++i) s += Sc[i] * u[j * ND + i];
+      tv[j] = s;
+    }
+    T Tv[ND];
+#pragma unroll
+    for (int j = 0; j < ND; ++j) Tv[j] = 0;
+#pragma unroll
+    for (int r = 0; r < NQ; ++r) {
+      T val = 0;
+#pragma unroll
+      for (int j = 0; j < ND; ++j) val += (T)tb.S[r][j] * tv[j];
+      val *= wq[r];
+#pragma unroll
+      for (int j = 0; j < ND; ++j) Tv[j] += (T)tb.S[r][j] * val;
+    }
+    T* red = s_red + comp * CPW * ND * NQ + cw * ND * NQ;
+#pragma unroll
+    for (int j = 0; j < ND; ++j) red[j * NQ + c] = Tv[j];
+  }
+}
+
+// Output phase: value of dof t (= cw*ND2 + jj*ND + ii) of one component.
+template <int K, typename T>
+__device__ inline T wave_mass_out(const T* s_red_comp, int t) {
+  constexpr int ND = Cfg<K>::ND, NQ = Cfg<K>::NQ, ND2 = Cfg<K>::ND2;
+  const int cw = t / ND2, o = t - cw * ND2, jj = o / ND, ii = o - jj * ND;
+  T Si[NQ], Di[NQ];
+  lane_cols<K, T>(ii, Si, Di);
+  const T* red = s_red_comp + cw * ND * NQ + jj * NQ;
+  T acc = 0;
+#pragma unroll
+  for (int c = 0; c < NQ; ++c) acc += Si[c] * red[c];
+  return acc;
+}
+
+}  // namespace atmo

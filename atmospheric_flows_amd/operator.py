@@ -1,0 +1,232 @@
+"""Host-side mirror of the reference's operator/solver interface over the C ABI.
+
+`DeviceContext` plays the role of EulerSolver's setup (mesh, MatrixFree
+storage, MG hierarchy, Atmospheric_Flow.cc:217-409); `EULEROperator` keeps the
+member names of euler_operator.h:24-67 (setters, vmult_rhs_*, vmult_pressure,
+vmult_enthalpy, vmult per NS_stage, compute_diagonal); `DeviceVector` is
+MatrixFree::initialize_dof_vector(v, block). All arithmetic runs in
+libatmo_hip.so on the GPU.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _native as N
+
+BLOCK_VELOCITY, BLOCK_PRESSURE, BLOCK_DENSITY = 0, 1, 2
+
+PARAM_NAMES = [f[0] for f in N.Params._fields_]
+
+
+def witch_of_agnesi(nx, Lx=40000.0, h0=400.0, a=1000.0, xc=20000.0):
+    """Terrain heights h(x_i) = h0/(1+((x_i-xc)/a)^2) at the nx+1 vertex columns."""
+    x = Lx * np.arange(nx + 1) / nx
+    return h0 / (1.0 + ((x - xc) / a) ** 2)
+
+
+def default_params(**overrides):
+    p = N.Params()
+    N.load().atmo_params_default(ctypes.byref(p))
+    for k, v in overrides.items():
+        if k not in PARAM_NAMES:
+            raise KeyError(f"unknown parameter {k}")
+        setattr(p, k, v)
+    return p
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class DeviceVector:
+    def __init__(self, ctx, block):
+        self.ctx, self.block = ctx, block
+        h = ctypes.c_void_p()
+        N.check(N.load().atmo_vec_alloc(ctx.handle, block, ctypes.byref(h)), ctx.handle)
+        self.handle = h
+        n = ctypes.c_size_t()
+        N.load().atmo_vec_size(h, ctypes.byref(n))
+        self.size = n.value
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        N.check(N.load().atmo_vec_upload(self.ctx.handle, self.handle, _ptr(a), a.size), self.ctx.handle)
+        return self
+
+    def download(self):
+        out = np.empty(self.size)
+        N.check(N.load().atmo_vec_download(self.ctx.handle, self.handle, _ptr(out), out.size), self.ctx.handle)
+        return out
+
+    def __del__(self):
+        if getattr(self, "handle", None) and getattr(self.ctx, "handle", None):
+            N.load().atmo_vec_free(self.ctx.handle, self.handle)
+            self.handle = None
+
+
+class DeviceContext:
+    def __init__(self, degree, nx, nz, Lx=40000.0, H=20000.0, h=None, periodic=True, **params):
+        L = N.load()
+        self.degree, self.nx, self.nz = degree, nx, nz
+        self._h = np.zeros(nx + 1) if h is None else np.ascontiguousarray(h, dtype=np.float64)
+        self.mesh = N.MeshDesc(degree, nx, nz, 1 if periodic else 0, Lx, H, _ptr(self._h))
+        self.params = default_params(**params)
+        h_ = ctypes.c_void_p()
+        N.check(L.atmo_ctx_create(ctypes.byref(self.mesh), ctypes.byref(self.params), ctypes.byref(h_)))
+        self.handle = h_
+        nc, ns, nl = ctypes.c_int(), ctypes.c_long(), ctypes.c_int()
+        L.atmo_ctx_info(self.handle, ctypes.byref(nc), ctypes.byref(ns), ctypes.byref(nl))
+        self.ncells, self.ns, self.nlevels = nc.value, ns.value, nl.value
+
+    def close(self):
+        if getattr(self, "handle", None):
+            N.load().atmo_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()
+
+    def set_params(self, **kw):
+        for k, v in kw.items():
+            if k not in PARAM_NAMES:
+                raise KeyError(f"unknown parameter {k}")
+            setattr(self.params, k, v)
+        N.check(N.load().atmo_set_params(self.handle, ctypes.byref(self.params)), self.handle)
+
+    def vector(self, block, data=None):
+        v = DeviceVector(self, block)
+        if data is not None:
+            v.upload(data)
+        return v
+
+    def sync(self):
+        N.check(N.load().atmo_sync(self.handle), self.handle)
+
+    def init_hill(self, theta0=300.0, p0=1e5, N_bv=0.01, U=10.0, g=9.81, R=287.0, gamma=1.4, pert=1e-4, seed=1234):
+        rho, u, p = np.zeros(self.ns), np.zeros(2 * self.ns), np.zeros(self.ns)
+        N.check(N.load().atmo_init_hill_state(ctypes.byref(self.mesh), theta0, p0, N_bv, U, g, R, gamma, pert, seed,
+                                              _ptr(rho), _ptr(u), _ptr(p)))
+        return rho, u, p
+
+    def upload_state(self, rho, u, p):
+        rho, u, p = (np.ascontiguousarray(a, dtype=np.float64) for a in (rho, u, p))
+        N.check(N.load().atmo_state_upload(self.handle, _ptr(rho), _ptr(u), _ptr(p)), self.handle)
+
+    def download_state(self):
+        rho, u, p = np.zeros(self.ns), np.zeros(2 * self.ns), np.zeros(self.ns)
+        N.check(N.load().atmo_state_download(self.handle, _ptr(rho), _ptr(u), _ptr(p)), self.handle)
+        return rho, u, p
+
+    def imex_step(self, nsteps=1):
+        N.check(N.load().atmo_imex_step(self.handle, nsteps), self.handle)
+
+    def stats(self):
+        s = N.Stats()
+        N.load().atmo_get_stats(self.handle, ctypes.byref(s))
+        return {n: getattr(s, n) for n, _ in N.Stats._fields_}
+
+    def reset_stats(self):
+        N.load().atmo_reset_stats(self.handle)
+
+    def time_kernel(self, which, nrep=20):
+        ms, b = ctypes.c_double(), ctypes.c_double()
+        N.check(N.load().atmo_time_kernel(self.handle, which, nrep, ctypes.byref(ms), ctypes.byref(b)), self.handle)
+        return ms.value, b.value
+
+
+class EULEROperator:
+    """euler_operator.h:24-67 over the device context."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        self.HYPERBOLIC_stage, self.NS_stage = 1, 1
+        self._inv_diag = None
+
+    def _L(self):
+        return N.load()
+
+    def set_dt(self, dt):
+        self.ctx.set_params(dt=dt)
+
+    def set_Mach(self, Ma):
+        self.ctx.set_params(Ma=Ma)
+
+    def set_Froude(self, Fr):
+        self.ctx.set_params(Fr=Fr)
+
+    def _stages(self):
+        N.check(self._L().atmo_set_stages(self.ctx.handle, self.HYPERBOLIC_stage, self.NS_stage), self.ctx.handle)
+
+    def set_HYPERBOLIC_stage(self, stage):
+        if stage not in (1, 2, 3):
+            raise ValueError("HYPERBOLIC_stage must be 1..3")
+        self.HYPERBOLIC_stage = stage
+        self._stages()
+
+    def set_NS_stage(self, stage):
+        if not 1 <= stage <= 6:
+            raise ValueError("NS_stage must be 1..6")
+        self.NS_stage = stage
+        self._stages()
+
+    def set_rho_for_fixed(self, v):
+        N.check(self._L().atmo_set_rho_for_fixed(self.ctx.handle, v.handle), self.ctx.handle)
+
+    def set_pres_fixed(self, v):
+        N.check(self._L().atmo_set_pres_fixed(self.ctx.handle, v.handle), self.ctx.handle)
+
+    def set_u_fixed(self, v):
+        """Kept for interface parity: the reference stores u_fixed but its
+        operators read the fixed-point velocity from the rhs source list."""
+        self._u_fixed = v
+
+    def _src(self, src):
+        return (ctypes.c_void_p * len(src))(*[s.handle.value for s in src])
+
+    def vmult_rhs_rho_update(self, dst, src):
+        self._stages()
+        N.check(self._L().atmo_rhs_rho(self.ctx.handle, dst.handle, self._src(src), len(src)), self.ctx.handle)
+
+    def vmult_rhs_pressure(self, dst, src):
+        self._stages()
+        N.check(self._L().atmo_rhs_pressure(self.ctx.handle, dst.handle, self._src(src), len(src)), self.ctx.handle)
+
+    def vmult_rhs_velocity_fixed(self, dst, src):
+        self._stages()
+        N.check(self._L().atmo_rhs_velocity(self.ctx.handle, dst.handle, self._src(src), len(src)), self.ctx.handle)
+
+    def vmult_pressure(self, dst, src):
+        self._stages()
+        N.check(self._L().atmo_vmult_pressure(self.ctx.handle, dst.handle, src.handle), self.ctx.handle)
+
+    def vmult_enthalpy(self, dst, src):
+        self._stages()
+        N.check(self._L().atmo_vmult_enthalpy(self.ctx.handle, dst.handle, src.handle), self.ctx.handle)
+
+    def vmult(self, dst, src):
+        self._stages()
+        N.check(self._L().atmo_vmult(self.ctx.handle, dst.handle, src.handle), self.ctx.handle)
+
+    def compute_diagonal(self):
+        self._stages()
+        blk = BLOCK_VELOCITY if self.NS_stage in (3, 5) else (BLOCK_DENSITY if self.NS_stage in (1, 4)
+                                                               else BLOCK_PRESSURE)
+        self._inv_diag = self.ctx.vector(blk)
+        N.check(self._L().atmo_inverse_diagonal(self.ctx.handle, self._inv_diag.handle), self.ctx.handle)
+
+    def get_matrix_diagonal_inverse(self):
+        return self._inv_diag
+
+    # solver entry points of EulerSolver's stage functions
+    def solve_cg_mg(self, ns_stage, x, b, tol_factor):
+        its = ctypes.c_int()
+        N.check(self._L().atmo_solve_cg_mg(self.ctx.handle, ns_stage, x.handle, b.handle, tol_factor,
+                                           ctypes.byref(its)), self.ctx.handle)
+        return its.value
+
+    def solve_schur_gmres(self, x, b, tol_factor):
+        self._stages()
+        its = ctypes.c_int()
+        N.check(self._L().atmo_solve_schur_gmres(self.ctx.handle, x.handle, b.handle, tol_factor, ctypes.byref(its)),
+                self.ctx.handle)
+        return its.value

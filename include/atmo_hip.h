@@ -1,0 +1,169 @@
+/* C ABI of the MI355X IMEX-DG hot path (libatmo_hip.so).
+ *
+ * Replaces, on one GPU with device-resident fields, the deal.II-based
+ * operator and solver sequence of Atmospheric_Flows:
+ *   - EULEROperator (euler_operator.h:24-211): setters, explicit residuals,
+ *     B/C actions, vmult per NS_stage (incl. the Schur complement) and
+ *     compute_diagonal;
+ *   - the EulerSolver stage solves and the IMEX time loop
+ *     (Atmospheric_Flow.cc:430-871, 1100-1257).
+ *
+ * Conventions: every function returns an atmo_status (0 = ATMO_OK); no C++
+ * exception crosses the ABI; the library owns device memory; one context
+ * owns one HIP stream; calls on one context must be serialized by the
+ * caller (like the reference operator, which mutates NS_stage inside
+ * apply_add and is not re-entrant, euler_operator.h:91, 2383-2406).
+ *
+ * Field layout (host and device): [component][cell][dof], cell = j*nx + i
+ * (x fastest), dof = jj*(k+1) + ii (x fastest). Velocity has 2 components.
+ * Block ids follow the reference's DoF-handler order (Atmospheric_Flow.cc:334-336):
+ * 0 = velocity, 1 = pressure, 2 = density.
+ */
+#ifndef ATMO_HIP_H
+#define ATMO_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int atmo_status;
+#define ATMO_OK 0
+#define ATMO_INVALID_ARG 1
+#define ATMO_NO_CONVERGENCE 2 /* deal.II SolverControl::NoConvergence */
+#define ATMO_NONFINITE 3
+#define ATMO_OOM 4
+#define ATMO_DEVICE 5
+
+#define ATMO_BLOCK_VELOCITY 0
+#define ATMO_BLOCK_PRESSURE 1
+#define ATMO_BLOCK_DENSITY 2
+
+typedef struct atmo_ctx atmo_ctx;
+typedef struct atmo_vec atmo_vec;
+
+/* Terrain-following structured mesh: vertex (i,j) at x = i*Lx/nx,
+ * z = h[i] + (H - h[i]) * j / nz, bilinear cells, walls at bottom and top,
+ * lateral boundaries periodic (h[nx] == h[0]) or walls. Replaces the
+ * reference's p4est sphere mesh (Atmospheric_Flow.cc:279-289). */
+typedef struct {
+  int degree; /* FE_DGQ degree k (1..3); quadrature QGauss(2k+1) */
+  int nx, nz;
+  int periodic;
+  double Lx, H;
+  const double* h; /* nx+1 terrain heights */
+} atmo_mesh_desc;
+
+/* Run-time parameters: the reference's Data_Storage (runtime_parameters.h:16-178)
+ * plus its hard-coded solver constants. */
+typedef struct {
+  double Ma, Fr, dt;           /* Physical data / Time step data */
+  double gamma_gas;            /* EquationData::Cp_Cv */
+  double grav_x, grav_z;       /* gravity unit vector e_r */
+  double grav_factor;          /* radial factor a^2/r^2 (1 for constant gravity) */
+  double eps;                  /* Data solve / eps */
+  int max_iterations;          /* Data solve / max_iterations */
+  double schur_inner_tol;      /* euler_operator.h:2391 (1e-12) */
+  int schur_inner_max;         /* euler_operator.h:2391 (10000) */
+  double a_solve_tol;          /* Atmospheric_Flow.cc:571 (1e-12) */
+  int fp_max_iter;             /* Atmospheric_Flow.cc:1135 (100) */
+  double fp_tol;               /* Atmospheric_Flow.cc:1149 (1e-10) */
+  int fp_lock;                 /* >0: exactly this many fixed-point iterations */
+  int mg_min_cells;            /* coarsest MG level keeps >= this many cells */
+  int cheb_degree;             /* Atmospheric_Flow.cc:492 (3) */
+  double cheb_range;           /* Atmospheric_Flow.cc:491 (15) */
+  int cheb_eig_its;            /* Atmospheric_Flow.cc:493 (10) */
+  int gmres_restart;           /* deal.II SolverGMRES default (28) */
+  int precision;               /* 0: FP64 ; 1: FP32 multigrid inside FP64 Krylov */
+} atmo_params;
+
+typedef struct {
+  long cg_its, gmres_its, inner_cg_its, fp_its, mg_cycles, coarse_its, schur_matvecs, host_syncs;
+} atmo_stats;
+
+/* Defaults equal to the reference's hard-coded values. */
+void atmo_params_default(atmo_params* p);
+
+/* Library version string. */
+const char* atmo_version(void);
+/* Message of the last failed call on ctx (or of the last failed create when ctx is NULL). */
+const char* atmo_last_error(const atmo_ctx* ctx);
+
+/* EulerSolver constructor part: mesh, MG hierarchy, device buffers
+ * (Atmospheric_Flow.cc:217-272, setup_dofs :296-409). */
+atmo_status atmo_ctx_create(const atmo_mesh_desc* mesh, const atmo_params* params, atmo_ctx** out);
+atmo_status atmo_ctx_destroy(atmo_ctx* ctx);
+atmo_status atmo_set_params(atmo_ctx* ctx, const atmo_params* params);
+atmo_status atmo_ctx_info(atmo_ctx* ctx, int* ncells, long* scalar_dofs, int* nlevels);
+atmo_status atmo_sync(atmo_ctx* ctx);
+atmo_status atmo_get_stats(atmo_ctx* ctx, atmo_stats* st);
+atmo_status atmo_reset_stats(atmo_ctx* ctx);
+
+/* MatrixFree::initialize_dof_vector(v, block) (Atmospheric_Flow.cc:351-376). */
+atmo_status atmo_vec_alloc(atmo_ctx* ctx, int block, atmo_vec** out);
+atmo_status atmo_vec_free(atmo_ctx* ctx, atmo_vec* v);
+atmo_status atmo_vec_size(const atmo_vec* v, size_t* n);
+atmo_status atmo_vec_upload(atmo_ctx* ctx, atmo_vec* v, const double* host, size_t n);
+atmo_status atmo_vec_download(atmo_ctx* ctx, const atmo_vec* v, double* host, size_t n);
+/* Device address of the vector's storage (for zero-copy callers on ctx's stream). */
+atmo_status atmo_vec_device_ptr(atmo_vec* v, double** dev);
+
+/* EULEROperator::set_HYPERBOLIC_stage / set_NS_stage (euler_operator.h:283-303). */
+atmo_status atmo_set_stages(atmo_ctx* ctx, int hyperbolic_stage, int ns_stage);
+/* set_rho_for_fixed (:312-315): also refreshes the level densities
+ * (interpolate_to_mg, Atmospheric_Flow.cc:1127-1132). */
+atmo_status atmo_set_rho_for_fixed(atmo_ctx* ctx, const atmo_vec* rho);
+/* set_pres_fixed (:324-327). */
+atmo_status atmo_set_pres_fixed(atmo_ctx* ctx, const atmo_vec* p);
+
+/* vmult_rhs_rho_update / vmult_rhs_pressure / vmult_rhs_velocity_fixed
+ * (euler_operator.h:697-712, 1364-1381, 2308-2325); src follows the same
+ * positional conventions as the reference's callers (Atmospheric_Flow.cc:439-449,
+ * 555-568, 696-698, 796-799). */
+atmo_status atmo_rhs_rho(atmo_ctx* ctx, atmo_vec* dst, const atmo_vec* const* src, int nsrc);
+atmo_status atmo_rhs_pressure(atmo_ctx* ctx, atmo_vec* dst, const atmo_vec* const* src, int nsrc);
+atmo_status atmo_rhs_velocity(atmo_ctx* ctx, atmo_vec* dst, const atmo_vec* const* src, int nsrc);
+/* vmult_pressure (B, :2420-2435) and vmult_enthalpy (C, :2438-2453). */
+atmo_status atmo_vmult_pressure(atmo_ctx* ctx, atmo_vec* dst_u, const atmo_vec* src_p);
+atmo_status atmo_vmult_enthalpy(atmo_ctx* ctx, atmo_vec* dst_p, const atmo_vec* src_u);
+/* MatrixFreeOperators::Base::vmult = zero + apply_add per NS_stage (:2360-2417);
+ * NS_stage 2 is the Schur complement D - C A^{-1} B with its inner CG. */
+atmo_status atmo_vmult(atmo_ctx* ctx, atmo_vec* dst, const atmo_vec* src);
+/* compute_diagonal + get_matrix_diagonal_inverse (:2589-2626). */
+atmo_status atmo_inverse_diagonal(atmo_ctx* ctx, atmo_vec* dst);
+
+/* SolverCG + PreconditionMG on the mass-type block of ns_stage (1/4 M_rho,
+ * 3/5 A, 6 D) to tol_factor*|b| (Atmospheric_Flow.cc:452-536, 701-780, 801-870). */
+atmo_status atmo_solve_cg_mg(atmo_ctx* ctx, int ns_stage, atmo_vec* x, const atmo_vec* b, double tol_factor,
+                             int* iterations);
+/* SolverGMRES + PreconditionJacobi on the Schur complement (:656-672). */
+atmo_status atmo_solve_schur_gmres(atmo_ctx* ctx, atmo_vec* x, const atmo_vec* b, double tol_factor,
+                                   int* iterations);
+
+/* IMEX driver state (rho_old, u_old, pres_old) and time steps (EulerSolver::run body). */
+atmo_status atmo_state_upload(atmo_ctx* ctx, const double* rho, const double* u, const double* p);
+atmo_status atmo_state_download(atmo_ctx* ctx, double* rho, double* u, double* p);
+atmo_status atmo_imex_step(atmo_ctx* ctx, int nsteps);
+
+/* Hill-flow initial state at the FE_DGQ support points (replaces
+ * EquationData's analytic sphere state, equation_data.h:47-176, evaluated by
+ * VectorTools::interpolate at Atmospheric_Flow.cc:420-422). */
+atmo_status atmo_init_hill_state(const atmo_mesh_desc* mesh, double theta0, double p0, double N, double U, double g,
+                                 double R, double gamma_gas, double perturbation, uint64_t seed, double* rho,
+                                 double* u, double* p);
+
+/* Measurement helper: launches kernel `which` nrep times back-to-back on the
+ * context's stream on its current data and reports the mean duration from
+ * HIP events recorded on that stream. which: 0 A apply (k_mass, weighted),
+ * 1 fused inner-CG apply (k_jcg_apply), 2 M_rho apply, 3 B, 4 C,
+ * 5 stage-3 pressure residual. Also reports the kernel's algorithmic HBM
+ * bytes per launch. */
+atmo_status atmo_time_kernel(atmo_ctx* ctx, int which, int nrep, double* ms_per_launch, double* bytes_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ATMO_HIP_H */

@@ -120,6 +120,7 @@ class Oracle:
                              fp_lock)
 
     def set_stage(self, hyperbolic_stage, ns_stage=1):
+        self._hs = hyperbolic_stage
         lib().orc_set_stage(self._h, hyperbolic_stage, ns_stage)
 
     def set_rho_for_fixed(self, rho):

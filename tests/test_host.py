@@ -1,0 +1,109 @@
+"""Host-only tests (no GPU needed): C-ABI exports, parameter defaults, the
+host-side initial state against the oracle, argument errors."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import atmospheric_flows_amd as af
+from atmospheric_flows_amd import _native as N
+from atmospheric_flows_amd import prm
+from oracle.pyoracle import Oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    src = open(os.path.join(ROOT, "include", "atmo_hip.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:atmo_status|void|const char\*)\s+(atmo_\w+)\s*\(", src, re.M)))
+
+
+def test_every_declared_symbol_is_exported():
+    L = N.load()
+    syms = declared_symbols()
+    assert len(syms) >= 30
+    for s in syms:
+        assert hasattr(L, s), s
+    # the ctypes signature table covers every declaration
+    assert sorted(n for n, _, _ in N.SIGNATURES) == syms
+
+
+def test_param_defaults_match_reference_constants():
+    p = af.default_params()
+    assert p.gamma_gas == 1.4                      # equation_data.h:24
+    assert p.schur_inner_tol == 1e-12              # euler_operator.h:2391
+    assert p.schur_inner_max == 10000              # euler_operator.h:2391
+    assert p.a_solve_tol == 1e-12                  # Atmospheric_Flow.cc:571
+    assert p.fp_max_iter == 100 and p.fp_tol == 1e-10  # :1135, :1149
+    assert p.cheb_degree == 3 and p.cheb_range == 15.0 and p.cheb_eig_its == 10  # :491-493
+    assert p.gmres_restart == 28                   # SolverGMRES default: 30 temporary vectors
+    assert p.precision == 0
+
+
+@pytest.mark.parametrize("k,nx,nz", [(2, 40, 20), (3, 16, 8), (1, 6, 4)])
+def test_initial_state_matches_oracle(k, nx, nz):
+    h = af.witch_of_agnesi(nx)
+    mesh = N.MeshDesc(k, nx, nz, 1, 40000.0, 20000.0, h.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    ns = nx * nz * (k + 1) ** 2
+    rho, u, p = np.zeros(ns), np.zeros(2 * ns), np.zeros(ns)
+    dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))  # noqa: E731
+    N.check(N.load().atmo_init_hill_state(ctypes.byref(mesh), 300.0, 1e5, 0.01, 10.0, 9.81, 287.0, 1.4, 1e-4, 42,
+                                          dp(rho), dp(u), dp(p)))
+    o = Oracle(k, nx, nz, hx=h)
+    r2, u2, p2 = o.init_hill(seed=42)
+    assert np.abs(rho - r2).max() <= 1e-15 * np.abs(r2).max()
+    assert np.abs(p - p2).max() <= 1e-15 * np.abs(p2).max() * 4
+    assert np.array_equal(u, u2)
+    # background: surface pressure ~ p0 at z = h, density decreasing upward
+    assert 9e4 < p.max() <= 1e5 and rho.min() > 0
+
+
+def test_init_rejects_bad_arguments():
+    h = np.zeros(5)
+    mesh = N.MeshDesc(7, 4, 2, 1, 1.0, 1.0, h.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    z = np.zeros(4 * 2 * 64 * 2)
+    dp = z.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    st = N.load().atmo_init_hill_state(ctypes.byref(mesh), 300.0, 1e5, 0.01, 10.0, 9.81, 287.0, 1.4, 1e-4, 1, dp, dp,
+                                       dp)
+    assert st == N.ATMO_INVALID_ARG
+    assert b"degree" in N.load().atmo_last_error(None) or b"mesh" in N.load().atmo_last_error(None)
+    with pytest.raises(KeyError):
+        af.default_params(no_such_parameter=1)
+
+
+def test_prm_parser_reads_reference_file_format(tmp_path):
+    text = """subsection Physical data
+  # comment
+  set initial_time = 0.0
+  set final_time   = 500.0
+  set Mach         = 1.0
+  set Froude       = 0.31928
+end
+subsection Time step data
+  set dt = 5.0
+end
+subsection Space discretization
+  set n_of_refines = 3
+end
+subsection Data solve
+  set max_iterations = 30000 # maximal number of iterations
+  set eps            = 1e-10  # stopping criterion
+end
+set saving directory = Out
+set output_interval = 100
+set verbose = true
+"""
+    f = tmp_path / "parameter-file.prm"
+    f.write_text(text)
+    d = prm.read_data(str(f))
+    assert d.final_time == 500.0 and d.Mach == 1.0 and d.Froude == 0.31928 and d.dt == 5.0
+    assert d.n_global_refines == 3 and d.max_iterations == 30000 and d.eps == 1e-10
+    assert d.dir == "Out" and d.output_interval == 100 and d.verbose is True
+    with pytest.raises(FileNotFoundError):
+        prm.read_data(str(tmp_path / "missing.prm"))
+    bad = tmp_path / "bad.prm"
+    bad.write_text("subsection Time step data\n set dt = -1\nend\n")
+    with pytest.raises(ValueError):
+        prm.read_data(str(bad))
