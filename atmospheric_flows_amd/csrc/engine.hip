@@ -110,11 +110,9 @@ class EngineK : public Engine {
     oc_ = consts();
     build_levels(mesh);
     const long Ns = fine().Ns;
-    red_partials.alloc((size_t)RED_MAX_SLOTS * RED_MAX_BLOCKS);
-    red_counter.alloc(1);
-    red_counter.zero(s_);
-    rb.partials = red_partials.p;
-    rb.counter = red_counter.p;
+    red_partials.alloc(RedBuf::doubles());
+    red_partials.zero(s_);
+    rb.bind(red_partials.p);
     cgst.alloc(4);
     cgst.zero(s_);
     scal.alloc(256);
@@ -129,6 +127,8 @@ class EngineK : public Engine {
     pres_fix.zero(s_);
     for (auto* b : {&cg_g, &cg_d, &cg_h, &s_t1, &s_t2, &in_g, &in_d, &in_h}) b->alloc(2 * Ns);
     s_t3.alloc(Ns);
+    in_z.alloc(2 * Ns);
+    if (P.precision == 1) blk_inv.alloc((size_t)fine().ncell * (K + 1) * (K + 1) * (K + 1) * (K + 1));
     gm_V.alloc((size_t)(P.gmres_restart + 2) * Ns);
     gm_p.alloc(Ns);
     gm_coef.alloc(MAX_GMRES + 2);
@@ -171,7 +171,11 @@ class EngineK : public Engine {
     P = p;
     oc_ = consts();
     if (rebuild) gm_V.alloc((size_t)(P.gmres_restart + 2) * fine().Ns);
-    if (prec_change) refresh_levels_rho();
+    if (prec_change) {
+      if (P.precision == 1 && !blk_inv.p)
+        blk_inv.alloc((size_t)fine().ncell * (K + 1) * (K + 1) * (K + 1) * (K + 1));
+      refresh_levels_rho();
+    }
   }
 
   // ----------------------------------------------------------- operator API
@@ -251,10 +255,12 @@ class EngineK : public Engine {
     ATMO_HIP_CHECK(hipEventCreate(&e0));
     ATMO_HIP_CHECK(hipEventCreate(&e1));
     CGState* st = cgst.p + 3;
-    if (which == 1) {
-      hipLaunchKernelGGL(k_set_cg, dim3(1), dim3(1), 0, s_, st, 0.0, 1 << 30, 0.0);
-      vec_copy(2 * Ns, u_old.p, in_g.p, s_);
-      in_d.zero(s_);
+    if (which == 1 || which == 6) {
+      hipLaunchKernelGGL(k_set_cg, dim3(1), dim3(1), 0, s_, st, 0.0, 1 << 30, 1e-300);
+      vec_copy(2 * Ns, u_old.p, s_t1.p, s_);
+      vec_fill(2 * Ns, 0.0, s_t2.p, s_);
+      jcg_init(Ns, m, K, s_t1.p, fine().inv64[BLK_A].p, nullptr, in_g.p, in_d.p, nullptr, st, rb, s_);
+      jcg_apply<K>(1, m, rho_fix.p, fine().inv64[BLK_A].p, nullptr, in_g.p, in_d.p, in_h.p, st, rb, Ns, s_);
     }
     auto launch = [&]() {
       switch (which) {
@@ -263,8 +269,13 @@ class EngineK : public Engine {
           fields = 5;
           break;
         case 1:
-          jcg_apply<K>(m, rho_fix.p, fine().inv64[BLK_A].p, in_g.p, in_d.p, in_h.p, st, rb, Ns, s_);
+          jcg_apply<K>(1, m, rho_fix.p, fine().inv64[BLK_A].p, nullptr, in_g.p, in_d.p, in_h.p, st, rb, Ns, s_);
           fields = 11;
+          break;
+        case 6:
+          jcg_update(1, Ns, m, K, s_t2.p, in_g.p, in_d.p, in_h.p, fine().inv64[BLK_A].p, nullptr, nullptr, st, rb,
+                     s_);
+          fields = 14;
           break;
         case 2:
           launch_mass<K, double>(m, 1, false, rho_old.p, nullptr, gm_p.p, 1.0, nullptr, 0.0, Ns, s_);
@@ -338,7 +349,8 @@ class EngineK : public Engine {
   std::vector<std::unique_ptr<Level>> lv;
   RedBuf rb;
   DBuf<double> red_partials;
-  DBuf<unsigned> red_counter;
+  DBuf<float> blk_inv;  // FP32 inverse cell blocks of A (mixed form)
+  DBuf<double> in_z;
   DBuf<CGState> cgst;
   DBuf<double> scal;
   DBuf<int> coarse_its;
@@ -351,7 +363,7 @@ class EngineK : public Engine {
   DBuf<double> rho_old, rho2, rho3, rho_curr, u_old, u2, u3, u_curr, u_fixed, p_old, p2, p3, p_fixed, p_fixed_old,
       tmp1, rhs_u, rhs_p;
   EngineStats st_;
-  int last_inner = 8;
+  int last_inner[2] = {8, 2};
 
   OpConsts consts() const {
     OpConsts o;
@@ -464,6 +476,7 @@ class EngineK : public Engine {
       level_diag(*L, BLK_A);
     }
     for (int l = 1; l < (int)lv.size(); ++l) estimate_eig(l, BLK_A);
+    if (P.precision == 1) build_block_inverse<K>(F.m, rho_fix.p, blk_inv.p, s_);
   }
 
   // Chebyshev eigenvalue estimate: <= cheb_eig_its CG iterations with the
@@ -645,29 +658,28 @@ class EngineK : public Engine {
     return it;
   }
 
-  // inner A^{-1} of the Schur complement (euler_operator.h:2391-2397):
-  // Jacobi CG to 1e-12*|b| from a zero guess, fused kernels, device-side
-  // convergence flag, host polls once per batch of iterations.
+  // inner A^{-1} of the Schur complement (euler_operator.h:2391-2397): CG
+  // to 1e-12*|b| from a zero guess with fused kernels and a device-side
+  // convergence flag; the host polls once per batch of iterations. The plain
+  // form uses the reference's Jacobi preconditioner; the mixed form uses the
+  // FP32 inverse of every cell block of A (A is block diagonal, so this is
+  // an FP32 solve refined by the FP64 CG).
   int inner_solve(double* x, const double* b) {
-    const long n = 2 * fine().Ns;
-    if (P.precision == 1) {
-      vec_fill(n, 0.0, x, s_);
-      const int its = pcg_mg(BLK_A, x, b, P.schur_inner_tol, P.schur_inner_max);
-      st_.cg_its -= its;
-      st_.inner_cg_its += its;
-      return its;
-    }
+    const long Ns = fine().Ns;
+    const long n = 2 * Ns;
     CGState* st = cgst.p + 1;
     const double* PA = fine().inv64[BLK_A].p;
+    const float* Pinv = P.precision == 1 ? blk_inv.p : nullptr;
+    double* z = Pinv ? in_z.p : nullptr;
     hipLaunchKernelGGL(k_set_cg, dim3(1), dim3(1), 0, s_, st, 0.0, P.schur_inner_max, P.schur_inner_tol);
     vec_fill(n, 0.0, x, s_);
-    jcg_init(n, b, PA, in_g.p, in_d.p, st, rb, s_);
-    int batch = std::max(1, last_inner);
-    int its = 0;
+    jcg_init(Ns, fine().m, K, b, PA, Pinv, in_g.p, in_d.p, z, st, rb, s_);
+    int batch = std::max(1, last_inner[Pinv ? 1 : 0]);
+    int i = 1, its = 0;
     while (true) {
-      for (int i = 0; i < batch; ++i) {
-        jcg_apply<K>(fine().m, rho_fix.p, PA, in_g.p, in_d.p, in_h.p, st, rb, fine().Ns, s_);
-        jcg_update(n, x, in_g.p, in_d.p, in_h.p, PA, st, rb, s_);
+      for (int r = 0; r < batch; ++r, ++i) {
+        jcg_apply<K>(i, fine().m, rho_fix.p, PA, z, in_g.p, in_d.p, in_h.p, st, rb, Ns, s_);
+        jcg_update(i, Ns, fine().m, K, x, in_g.p, in_d.p, in_h.p, PA, Pinv, z, st, rb, s_);
       }
       read_state(1);
       its = hst[1].it;
@@ -675,7 +687,7 @@ class EngineK : public Engine {
       if (hst[1].done == -1) throw NoConvergence("Schur inner CG: no convergence");
       batch = 2;
     }
-    last_inner = its;
+    last_inner[Pinv ? 1 : 0] = its;
     st_.inner_cg_its += its;
     return its;
   }

@@ -1,12 +1,12 @@
 // Vector operations, deterministic reductions and device-resident Krylov
 // state (krylov.hip).
 //
-// Reductions are two-level and order-fixed: a grid of RED_BLOCKS-limited
-// blocks writes per-block partials, and the last block to finish (atomic
-// ticket) sums them in block order and runs the finalization (alpha, beta,
-// norms, convergence flag) on the device. Results therefore do not depend on
-// scheduling, and Krylov scalars never travel to the host inside a solve
-// except the convergence flag.
+// Reductions never use atomics: a producer kernel writes one partial per
+// block (at most RED_MAX_BLOCKS blocks) and the consumer sums the partials in
+// block order, either in a one-block finalize kernel (host-driven solvers)
+// or redundantly in every block of the next kernel (the fused inner CG), so
+// results do not depend on scheduling and no counter is contended across
+// the chip's XCDs.
 #pragma once
 #include "common.h"
 #include "ops.h"
@@ -16,12 +16,22 @@ namespace atmo {
 constexpr int RED_THREADS = 256;
 constexpr int RED_MAX_BLOCKS = 1024;
 constexpr int MAX_GMRES = 64;
-
-struct RedBuf {
-  double* partials = nullptr;  // [RED_MAX_SLOTS][RED_MAX_BLOCKS]
-  unsigned* counter = nullptr;
-};
 constexpr int RED_MAX_SLOTS = MAX_GMRES + 2;
+
+// Partial buffers: gen[RED_MAX_SLOTS][RED_MAX_BLOCKS] for host-driven
+// reductions, pa[RED_MAX_BLOCKS] and pb[2][2][RED_MAX_BLOCKS] for the fused
+// CG ping-pong.
+struct RedBuf {
+  double* gen = nullptr;
+  double* pa = nullptr;
+  double* pb = nullptr;
+  static size_t doubles() { return (size_t)(RED_MAX_SLOTS + 1 + 4) * RED_MAX_BLOCKS; }
+  void bind(double* base) {
+    gen = base;
+    pa = base + (size_t)RED_MAX_SLOTS * RED_MAX_BLOCKS;
+    pb = pa + RED_MAX_BLOCKS;
+  }
+};
 
 // State of one CG solve (deal.II SolverCG semantics), lives in device memory.
 struct CGState {
@@ -39,39 +49,42 @@ template <typename T>
 void vec_copy(long n, const T* x, T* y, hipStream_t s);
 template <typename TO, typename TI>
 void vec_cast(long n, const TI* x, TO* y, hipStream_t s);
-// y = x * d (elementwise), optional scale
 template <typename T>
-void vec_mul(long n, const T* x, const T* d, T* y, hipStream_t s);
+void vec_mul(long n, const T* x, const T* d, T* y, hipStream_t s);  // y = x * d
 
-// out = x . y (sqrt when take_sqrt), written to device scalar *out
+// *out = x . y (sqrt when take_sqrt)
 void vec_dot(long n, const double* x, const double* y, double* out, bool take_sqrt, RedBuf& rb, hipStream_t s);
-// out = max |x - y| (y may be null), written to *out
+// *out = max |x - y| (y may be null)
 void vec_linf(long n, const double* x, const double* y, double* out, RedBuf& rb, hipStream_t s);
 
 // --- generic preconditioned CG pieces (host drives matvec/preconditioner)
-enum DotMode { DOT_ALPHA = 0, DOT_GH_INIT = 1, DOT_BETA = 2, DOT_RES0 = 3 };
+enum DotMode { DOT_ALPHA = 0, DOT_GH_INIT = 1, DOT_BETA = 2 };
 void cg_dot(long n, const double* x, const double* y, CGState* st, int mode, RedBuf& rb, hipStream_t s);
 // x += alpha d ; g += alpha h ; res = |g| ; it++ ; convergence flag
 void cg_update(long n, double* x, double* g, const double* d, const double* h, CGState* st, RedBuf& rb,
                hipStream_t s);
 // d = -h + beta d
 void cg_dir(long n, double* d, const double* h, const CGState* st, hipStream_t s);
-// g = Ax - b (Ax in g) or g = -b (x == 0)
+// g = Ax - b (Ax already in g) or g = -b
 void cg_residual(long n, double* g, const double* b, bool have_ax, hipStream_t s);
-// check after init: res0 = |g|, tol = tol_factor*tol_base (if tol_from_res0), done
+// res0 = |g|, tol = tol_factor*res0 when tol_from_res0, it = 0, check
 void cg_init_check(long n, const double* g, CGState* st, bool tol_from_res0, RedBuf& rb, hipStream_t s);
 
-// --- fused Jacobi-preconditioned CG on the A block (inner Schur solve)
-// init: g = -b, d = 0, res0 = |b|, tol = tol_factor*res0, gh = g.P g, beta = 0
-void jcg_init(long n, const double* b, const double* P, double* g, double* d, CGState* st, RedBuf& rb,
-              hipStream_t s);
-// d = -P g + beta d ; h = A d ; alpha = gh / (d.h)
+// --- fused CG on the velocity block A (inner Schur solve), zero initial
+// guess, tolerance tol_factor*|b|. Preconditioner: Jacobi (P = inverse
+// diagonal, FP64) or the FP32 cellwise inverse blocks (Pinv). Iteration i
+// (1-based, passed by the host) is jcg_apply(i) then jcg_update(i).
+void jcg_init(long Ns, const MeshDev& m, int K, const double* b, const double* P, const float* Pinv, double* g,
+              double* d, double* z, CGState* st, RedBuf& rb, hipStream_t s);
 template <int K>
-void jcg_apply(const MeshDev& m, const double* rho, const double* P, const double* g, double* d, double* h,
-               CGState* st, RedBuf& rb, long Ns, hipStream_t s);
-// x += alpha d ; g += alpha h ; res, convergence ; beta = g.Pg / gh
-void jcg_update(long n, double* x, double* g, const double* d, const double* h, const double* P, CGState* st,
-                RedBuf& rb, hipStream_t s);
+void jcg_apply(int i, const MeshDev& m, const double* rho, const double* P, const double* z, const double* g,
+               double* d, double* h, CGState* st, RedBuf& rb, long Ns, hipStream_t s);
+void jcg_update(int i, long Ns, const MeshDev& m, int K, double* x, double* g, const double* d, const double* h,
+                const double* P, const float* Pinv, double* z, CGState* st, RedBuf& rb, hipStream_t s);
+
+// FP32 inverse of every cell block of A (rho-weighted mass), [cell][i][j]
+template <int K>
+void build_block_inverse(const MeshDev& m, const double* rho, float* Pinv, hipStream_t s);
 
 // --- GMRES pieces
 // out[i] = w . V_i for i < nv (V column-major with leading dimension ldv)
@@ -80,7 +93,5 @@ void gm_mdot(long n, const double* w, const double* V, long ldv, int nv, double*
 void gm_maxpy(long n, double* w, const double* V, long ldv, int nv, const double* coef, hipStream_t s);
 // x += sum_i y[i] V_i
 void gm_update(long n, double* x, const double* V, long ldv, int nv, const double* y, hipStream_t s);
-// y = x * scale_dev[0]^{-1}
-void vec_scale_inv(long n, const double* x, const double* scal, double* y, hipStream_t s);
 
 }  // namespace atmo

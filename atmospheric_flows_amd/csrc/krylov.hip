@@ -1,8 +1,8 @@
 // Vector operations, order-fixed reductions and Krylov building blocks.
 // CG follows deal.II SolverCG as used by the reference
 // (Atmospheric_Flow.cc:452-536, euler_operator.h:2391-2397): residual
-// g = A x - b, convergence on the unpreconditioned |g| <= tol, Jacobi
-// preconditioner h = P g, direction d = -h + beta d.
+// g = A x - b, convergence on the unpreconditioned |g| <= tol, preconditioned
+// residual z = P g, direction d = -z + beta d, beta = (g.z)_new / (g.z)_old.
 #include "krylov.h"
 #include "wave.h"
 
@@ -12,6 +12,13 @@ int red_blocks(long n) {
   long b = (n + RED_THREADS * 4 - 1) / (RED_THREADS * 4);
   if (b < 1) b = 1;
   if (b > RED_MAX_BLOCKS) b = RED_MAX_BLOCKS;
+  return (int)b;
+}
+
+static int ew_blocks(long n) {
+  long b = (n + 255) / 256;
+  if (b > 8192) b = 8192;
+  if (b < 1) b = 1;
   return (int)b;
 }
 
@@ -27,10 +34,12 @@ __device__ inline double wave_max(double v) {
   return v;
 }
 
-// Block-level sum (or max) of NS values; result valid in thread 0.
+// Block-level sum (or max) of NS values, result broadcast to all threads.
+// Must be called by every thread of the block.
 template <int NS, bool MAX>
 __device__ inline void block_reduce(double (&v)[NS]) {
   __shared__ double sh[NS][16];
+  __shared__ double res[NS];
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int nw = (blockDim.x + 63) >> 6;
 #pragma unroll
@@ -44,47 +53,106 @@ __device__ inline void block_reduce(double (&v)[NS]) {
     for (int k = 0; k < NS; ++k) {
       double a = sh[k][0];
       for (int i = 1; i < nw; ++i) a = MAX ? fmax(a, sh[k][i]) : a + sh[k][i];
-      v[k] = a;
+      res[k] = a;
     }
   }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NS; ++k) v[k] = res[k];
   __syncthreads();
 }
 
-// Publish the block partials; the last block (ticket) reduces all partials
-// in block order and calls fin(sum) in thread 0. Returns nothing; all
-// threads of all blocks must call it.
-template <int NS, bool MAX, class Fin>
-__device__ inline void grid_reduce(double (&v)[NS], double* partials, unsigned* counter, Fin fin) {
-  __shared__ int s_last;
+template <int NS, bool MAX>
+__device__ inline void write_partials(double (&v)[NS], double* part) {
   block_reduce<NS, MAX>(v);
-  const int G = gridDim.x;
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0)
 #pragma unroll
-    for (int k = 0; k < NS; ++k)
-      __hip_atomic_store(&partials[(long)k * RED_MAX_BLOCKS + blockIdx.x], v[k], __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    unsigned t = atomicAdd(counter, 1u);
-    s_last = (t == (unsigned)(G - 1));
+    for (int k = 0; k < NS; ++k) part[(long)k * RED_MAX_BLOCKS + blockIdx.x] = v[k];
+}
+
+// sum of G partials (one slot), identical in every block (fixed order)
+__device__ inline double sum_slot(const double* part, int G) {
+  double v[1] = {0.0};
+  for (int b = threadIdx.x; b < G; b += blockDim.x) v[0] += part[b];
+  block_reduce<1, false>(v);
+  return v[0];
+}
+
+__device__ inline void cg_check(CGState* st, double res) {
+  // SolverControl::check: success if res <= tol, failure if it >= max_its
+  st->res = res;
+  if (res <= st->tol) st->done = 1;
+  else if (st->it >= st->max_its || isnan(res)) st->done = -1;
+}
+
+enum FinMode {
+  FIN_DOT = 0, FIN_SQRT, FIN_MAX, FIN_CG_ALPHA, FIN_CG_GH, FIN_CG_BETA, FIN_CG_UPDATE, FIN_CG_INIT,
+  FIN_CG_INIT_TOL, FIN_JCG_INIT, FIN_MDOT
+};
+
+// One block: sums nslot partial slots of G blocks and finalizes.
+__global__ void k_fin(const double* __restrict__ part, int G, int nslot, int mode, double* out, CGState* st) {
+  __shared__ double s_sum[RED_MAX_SLOTS];
+  for (int k = 0; k < nslot; ++k) {
+    double v[1] = {mode == FIN_MAX ? 0.0 : 0.0};
+    for (int b = threadIdx.x; b < G; b += blockDim.x) {
+      const double p = part[(long)k * RED_MAX_BLOCKS + b];
+      v[0] = mode == FIN_MAX ? fmax(v[0], p) : v[0] + p;
+    }
+    if (mode == FIN_MAX) block_reduce<1, true>(v);
+    else block_reduce<1, false>(v);
+    if (threadIdx.x == 0) s_sum[k] = v[0];
   }
   __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  double s[NS];
-#pragma unroll
-  for (int k = 0; k < NS; ++k) s[k] = MAX ? 0.0 : 0.0;
-  for (int b = threadIdx.x; b < G; b += blockDim.x) {
-#pragma unroll
-    for (int k = 0; k < NS; ++k) {
-      double p = __hip_atomic_load(&partials[(long)k * RED_MAX_BLOCKS + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      s[k] = MAX ? fmax(s[k], p) : s[k] + p;
+  if (threadIdx.x != 0) return;
+  const double s0 = s_sum[0];
+  switch (mode) {
+    case FIN_DOT: *out = s0; break;
+    case FIN_SQRT: *out = sqrt(s0); break;
+    case FIN_MAX: *out = s0; break;
+    case FIN_CG_ALPHA:
+      if (!st->done) { st->dh = s0; st->alpha = st->gh / s0; }
+      break;
+    case FIN_CG_GH:
+      if (!st->done) st->gh = s0;
+      break;
+    case FIN_CG_BETA:
+      if (!st->done) { st->beta = s0 / st->gh; st->gh = s0; }
+      break;
+    case FIN_CG_UPDATE:
+      if (!st->done) { st->it += 1; cg_check(st, sqrt(s0)); }
+      break;
+    case FIN_CG_INIT:
+    case FIN_CG_INIT_TOL: {
+      const double r = sqrt(s0);
+      st->res0 = r;
+      if (mode == FIN_CG_INIT_TOL) st->tol = st->tol_factor * r;
+      st->it = 0;
+      st->done = 0;
+      st->beta = 0;
+      cg_check(st, r);
+      break;
     }
+    case FIN_JCG_INIT: {
+      const double r = sqrt(s0);
+      st->res0 = r;
+      st->tol = st->tol_factor * r;
+      st->it = 0;
+      st->done = 0;
+      st->beta = 0;
+      st->gh = s_sum[1];
+      cg_check(st, r);
+      break;
+    }
+    case FIN_MDOT:
+      for (int k = 0; k < nslot; ++k) out[k] = s_sum[k];
+      break;
   }
-  block_reduce<NS, MAX>(s);
-  if (threadIdx.x == 0) {
-    fin(s);
-    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
+}
+
+static void fin(const double* part, int G, int nslot, int mode, double* out, CGState* st, hipStream_t s) {
+  hipLaunchKernelGGL(k_fin, dim3(1), dim3(RED_THREADS), 0, s, part, G, nslot, mode, out, st);
+  ATMO_HIP_CHECK(hipGetLastError());
 }
 
 // ------------------------------------------------------------ vector ops
@@ -106,13 +174,6 @@ template <typename T>
 __global__ void k_mul(long n, const T* __restrict__ x, const T* __restrict__ d, T* __restrict__ y) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
     y[i] = x[i] * d[i];
-}
-
-static int ew_blocks(long n) {
-  long b = (n + 255) / 256;
-  if (b > 8192) b = 8192;
-  if (b < 1) b = 1;
-  return (int)b;
 }
 
 template <typename T>
@@ -156,68 +217,47 @@ template void vec_cast<double, float>(long, const float*, double*, hipStream_t);
 template void vec_mul<double>(long, const double*, const double*, double*, hipStream_t);
 template void vec_mul<float>(long, const float*, const float*, float*, hipStream_t);
 
-__global__ void k_dot(long n, const double* __restrict__ x, const double* __restrict__ y, double* out, int take_sqrt,
-                      double* partials, unsigned* counter) {
+__global__ void k_dot_part(long n, const double* __restrict__ x, const double* __restrict__ y, double* part,
+                           const CGState* st) {
+  if (st && st->done) return;
   double v[1] = {0.0};
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
     v[0] += x[i] * y[i];
-  grid_reduce<1, false>(v, partials, counter, [&](double (&s)[1]) { *out = take_sqrt ? sqrt(s[0]) : s[0]; });
+  write_partials<1, false>(v, part);
 }
 
-__global__ void k_linf(long n, const double* __restrict__ x, const double* __restrict__ y, double* out,
-                       double* partials, unsigned* counter) {
+__global__ void k_linf_part(long n, const double* __restrict__ x, const double* __restrict__ y, double* part) {
   double v[1] = {0.0};
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
     v[0] = fmax(v[0], fabs(y ? x[i] - y[i] : x[i]));
-  grid_reduce<1, true>(v, partials, counter, [&](double (&s)[1]) { *out = s[0]; });
+  write_partials<1, true>(v, part);
 }
 
 void vec_dot(long n, const double* x, const double* y, double* out, bool take_sqrt, RedBuf& rb, hipStream_t s) {
-  hipLaunchKernelGGL(k_dot, dim3(red_blocks(n)), dim3(RED_THREADS), 0, s, n, x, y, out, take_sqrt ? 1 : 0,
-                     rb.partials, rb.counter);
+  const int G = red_blocks(n);
+  hipLaunchKernelGGL(k_dot_part, dim3(G), dim3(RED_THREADS), 0, s, n, x, y, rb.gen, (const CGState*)nullptr);
   ATMO_HIP_CHECK(hipGetLastError());
+  fin(rb.gen, G, 1, take_sqrt ? FIN_SQRT : FIN_DOT, out, nullptr, s);
 }
 
 void vec_linf(long n, const double* x, const double* y, double* out, RedBuf& rb, hipStream_t s) {
-  hipLaunchKernelGGL(k_linf, dim3(red_blocks(n)), dim3(RED_THREADS), 0, s, n, x, y, out, rb.partials, rb.counter);
+  const int G = red_blocks(n);
+  hipLaunchKernelGGL(k_linf_part, dim3(G), dim3(RED_THREADS), 0, s, n, x, y, rb.gen);
   ATMO_HIP_CHECK(hipGetLastError());
+  fin(rb.gen, G, 1, FIN_MAX, out, nullptr, s);
 }
 
 // ------------------------------------------------------------ generic CG
-__device__ inline void cg_check(CGState* st, double res) {
-  // SolverControl::check: success if res <= tol, failure if it >= max_its
-  st->res = res;
-  if (res <= st->tol) st->done = 1;
-  else if (st->it >= st->max_its || isnan(res)) st->done = -1;
-}
-
-__global__ void k_cg_dot(long n, const double* __restrict__ x, const double* __restrict__ y, CGState* st, int mode,
-                         double* partials, unsigned* counter) {
-  if (st->done) return;
-  double v[1] = {0.0};
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-    v[0] += x[i] * y[i];
-  grid_reduce<1, false>(v, partials, counter, [&](double (&s)[1]) {
-    if (mode == DOT_ALPHA) {
-      st->dh = s[0];
-      st->alpha = st->gh / s[0];
-    } else if (mode == DOT_GH_INIT) {
-      st->gh = s[0];
-    } else if (mode == DOT_BETA) {
-      st->beta = s[0] / st->gh;
-      st->gh = s[0];
-    }
-  });
-}
-
 void cg_dot(long n, const double* x, const double* y, CGState* st, int mode, RedBuf& rb, hipStream_t s) {
-  hipLaunchKernelGGL(k_cg_dot, dim3(red_blocks(n)), dim3(RED_THREADS), 0, s, n, x, y, st, mode, rb.partials,
-                     rb.counter);
+  const int G = red_blocks(n);
+  hipLaunchKernelGGL(k_dot_part, dim3(G), dim3(RED_THREADS), 0, s, n, x, y, rb.gen, (const CGState*)st);
   ATMO_HIP_CHECK(hipGetLastError());
+  const int fm = mode == DOT_ALPHA ? FIN_CG_ALPHA : (mode == DOT_GH_INIT ? FIN_CG_GH : FIN_CG_BETA);
+  fin(rb.gen, G, 1, fm, nullptr, st, s);
 }
 
 __global__ void k_cg_update(long n, double* __restrict__ x, double* __restrict__ g, const double* __restrict__ d,
-                            const double* __restrict__ h, CGState* st, double* partials, unsigned* counter) {
+                            const double* __restrict__ h, CGState* st, double* part) {
   if (st->done) return;
   const double a = st->alpha;
   double v[1] = {0.0};
@@ -227,17 +267,15 @@ __global__ void k_cg_update(long n, double* __restrict__ x, double* __restrict__
     g[i] = gi;
     v[0] += gi * gi;
   }
-  grid_reduce<1, false>(v, partials, counter, [&](double (&s)[1]) {
-    st->it += 1;
-    cg_check(st, sqrt(s[0]));
-  });
+  write_partials<1, false>(v, part);
 }
 
 void cg_update(long n, double* x, double* g, const double* d, const double* h, CGState* st, RedBuf& rb,
                hipStream_t s) {
-  hipLaunchKernelGGL(k_cg_update, dim3(red_blocks(n)), dim3(RED_THREADS), 0, s, n, x, g, d, h, st, rb.partials,
-                     rb.counter);
+  const int G = red_blocks(n);
+  hipLaunchKernelGGL(k_cg_update, dim3(G), dim3(RED_THREADS), 0, s, n, x, g, d, h, st, rb.gen);
   ATMO_HIP_CHECK(hipGetLastError());
+  fin(rb.gen, G, 1, FIN_CG_UPDATE, nullptr, st, s);
 }
 
 __global__ void k_cg_dir(long n, double* __restrict__ d, const double* __restrict__ h, const CGState* st) {
@@ -262,152 +300,321 @@ void cg_residual(long n, double* g, const double* b, bool have_ax, hipStream_t s
   ATMO_HIP_CHECK(hipGetLastError());
 }
 
-__global__ void k_cg_init_check(long n, const double* __restrict__ g, CGState* st, int tol_from_res0,
-                                double* partials, unsigned* counter) {
-  double v[1] = {0.0};
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-    v[0] += g[i] * g[i];
-  grid_reduce<1, false>(v, partials, counter, [&](double (&s)[1]) {
-    const double r = sqrt(s[0]);
-    st->res0 = r;
-    if (tol_from_res0) st->tol = st->tol_factor * r;
-    st->it = 0;
-    st->done = 0;
-    st->beta = 0;
-    cg_check(st, r);
-  });
-}
-
 void cg_init_check(long n, const double* g, CGState* st, bool tol_from_res0, RedBuf& rb, hipStream_t s) {
-  hipLaunchKernelGGL(k_cg_init_check, dim3(red_blocks(n)), dim3(RED_THREADS), 0, s, n, g, st,
-                     tol_from_res0 ? 1 : 0, rb.partials, rb.counter);
+  const int G = red_blocks(n);
+  hipLaunchKernelGGL(k_dot_part, dim3(G), dim3(RED_THREADS), 0, s, n, g, g, rb.gen, (const CGState*)nullptr);
   ATMO_HIP_CHECK(hipGetLastError());
+  fin(rb.gen, G, 1, tol_from_res0 ? FIN_CG_INIT_TOL : FIN_CG_INIT, nullptr, st, s);
 }
 
-// ------------------------------------------------------ fused Jacobi CG
-__global__ void k_jcg_init(long n, const double* __restrict__ b, const double* __restrict__ P, double* __restrict__ g,
-                           double* __restrict__ d, CGState* st, double* partials, unsigned* counter) {
-  double v[2] = {0.0, 0.0};
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    const double gi = -b[i];
-    g[i] = gi;
-    d[i] = 0.0;
-    v[0] += gi * gi;
-    v[1] += gi * P[i] * gi;
+// --------------------------------------------------- fused inner CG (A)
+// Update kernel: one thread per (cell, dof), CB cells per block batch.
+// INIT: g = -b (x = 0), d = 0. Otherwise x += alpha d, g += alpha h with
+// alpha = gh / dh re-reduced from the partials of jcg_apply (pa) and of the
+// previous update (pb[(i-1)&1]). Preconditioned residual z = P g (Jacobi) or
+// z = Pinv_cell g_cell (FP32 cell blocks). Partials (g.g, g.z) -> pb[i&1].
+template <int K, bool BLK, bool INIT>
+__global__ __launch_bounds__(256) void k_jcg_upd(int i, long Ns, int ncell, double* __restrict__ x,
+                                                 double* __restrict__ g, double* __restrict__ d,
+                                                 const double* __restrict__ h, const double* __restrict__ b,
+                                                 const double* __restrict__ P, const float* __restrict__ Pinv,
+                                                 double* __restrict__ z, CGState* st, double* pa, int Ga, double* pb,
+                                                 int Gb) {
+  constexpr int ND2 = (K + 1) * (K + 1);
+  constexpr int CB = 256 / ND2;
+  __shared__ double s_g[CB * ND2];
+  if (!INIT && st->done) return;
+  double alpha = 0.0;
+  if (!INIT) {
+    const double dh = sum_slot(pa, Ga);
+    const double gh = sum_slot(pb + (long)((i - 1) & 1) * 2 * RED_MAX_BLOCKS + RED_MAX_BLOCKS, Gb);
+    alpha = gh / dh;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->alpha = alpha;
+      st->dh = dh;
+      st->it = i;
+    }
   }
-  grid_reduce<2, false>(v, partials, counter, [&](double (&s)[2]) {
-    const double r = sqrt(s[0]);
-    st->res0 = r;
-    st->tol = st->tol_factor * r;
-    st->it = 0;
-    st->done = 0;
-    st->beta = 0;
-    st->gh = s[1];
-    cg_check(st, r);
-  });
-}
-
-void jcg_init(long n, const double* b, const double* P, double* g, double* d, CGState* st, RedBuf& rb,
-              hipStream_t s) {
-  hipLaunchKernelGGL(k_jcg_init, dim3(red_blocks(n)), dim3(RED_THREADS), 0, s, n, b, P, g, d, st, rb.partials,
-                     rb.counter);
-  ATMO_HIP_CHECK(hipGetLastError());
-}
-
-// d = -P g + beta d (cell-local) ; h = A d ; partial d.h ; alpha = gh / dh
-template <int K>
-__global__ __launch_bounds__(64) void k_jcg_apply(MeshDev m, const double* __restrict__ rho,
-                                                  const double* __restrict__ P, const double* __restrict__ g,
-                                                  double* __restrict__ d, double* __restrict__ h, CGState* st,
-                                                  long Ns, double* partials, unsigned* counter) {
-  using C = Cfg<K>;
-  constexpr int ND2 = C::ND2, CPW = C::CPW;
-  if (st->done) return;
-  __shared__ double s_u[2][CPW * ND2];
-  __shared__ double s_r[CPW * ND2];
-  __shared__ double s_red[2][CPW][C::ND][C::NQ];
-  const double beta = st->beta;
-  const long base = (long)blockIdx.x * CPW * ND2;
-  const long lim = (long)m.ncell * ND2;
-  for (int t = threadIdx.x; t < CPW * ND2; t += 64) {
-    const long gi = base + t;
+  const int cb = threadIdx.x / ND2, li = threadIdx.x - cb * ND2;
+  const int nbatch = (ncell + CB - 1) / CB;
+  double v[2] = {0.0, 0.0};
+  for (int bt = blockIdx.x; bt < nbatch; bt += gridDim.x) {
+    const int cell = bt * CB + cb;
+    const bool act = cb < CB && cell < ncell;
 #pragma unroll
     for (int comp = 0; comp < 2; ++comp) {
-      double dv = 0;
-      if (gi < lim) {
-        const long k = comp * Ns + gi;
-        dv = -P[k] * g[k] + beta * d[k];
-        d[k] = dv;
+      const long k = comp * Ns + (long)cell * ND2 + li;
+      double gv = 0.0;
+      if (act) {
+        if (INIT) {
+          gv = -b[k];
+          d[k] = 0.0;
+        } else {
+          x[k] += alpha * d[k];
+          gv = g[k] + alpha * h[k];
+        }
+        g[k] = gv;
       }
-      s_u[comp][t] = dv;
-    }
-    s_r[t] = gi < lim ? rho[gi] : 0.0;
-  }
-  __syncthreads();
-  wave_mass_core<K, double, 2, true>(m, blockIdx.x * CPW, &s_u[0][0], s_r, 1.0, &s_red[0][0][0][0]);
-  __syncthreads();
-  double v[1] = {0.0};
-  for (int t = threadIdx.x; t < CPW * ND2; t += 64) {
-    const long gi = base + t;
-    if (gi >= lim) break;
+      double zv;
+      if (BLK) {
+        if (act) s_g[cb * ND2 + li] = gv;
+        __syncthreads();
+        zv = 0.0;
+        if (act) {
+          const float* row = Pinv + ((long)cell * ND2 + li) * ND2;
 #pragma unroll
-    for (int comp = 0; comp < 2; ++comp) {
-      const double hv = wave_mass_out<K, double>(&s_red[comp][0][0][0], t);
-      h[comp * Ns + gi] = hv;
-      v[0] += s_u[comp][t] * hv;
+          for (int j = 0; j < ND2; ++j) zv += (double)row[j] * s_g[cb * ND2 + j];
+          z[k] = zv;
+        }
+        __syncthreads();
+      } else {
+        zv = act ? P[k] * gv : 0.0;
+      }
+      v[0] += gv * gv;
+      v[1] += gv * zv;
     }
   }
-  grid_reduce<1, false>(v, partials, counter, [&](double (&s)[1]) {
-    st->dh = s[0];
-    st->alpha = st->gh / s[0];
-  });
+  block_reduce<2, false>(v);
+  if (threadIdx.x == 0) {
+    double* dst = pb + (long)(i & 1) * 2 * RED_MAX_BLOCKS;
+    dst[blockIdx.x] = v[0];
+    dst[RED_MAX_BLOCKS + blockIdx.x] = v[1];
+  }
+}
+
+static int upd_blocks(int ncell, int K) {
+  const int CB = 256 / ((K + 1) * (K + 1));
+  long b = (ncell + CB - 1) / CB;
+  if (b > RED_MAX_BLOCKS) b = RED_MAX_BLOCKS;
+  return (int)std::max(1L, b);
 }
 
 template <int K>
-void jcg_apply(const MeshDev& m, const double* rho, const double* P, const double* g, double* d, double* h,
-               CGState* st, RedBuf& rb, long Ns, hipStream_t s) {
-  const int grid = ceil_div(m.ncell, Cfg<K>::CPW);
-  if (grid > RED_MAX_BLOCKS * 64) throw std::invalid_argument("jcg_apply: grid too large");
-  // The fused kernel reduces over its own grid; the partial buffer holds up
-  // to RED_MAX_BLOCKS*64 blocks for slot 0 only.
-  hipLaunchKernelGGL((k_jcg_apply<K>), dim3(grid), dim3(64), 0, s, m, rho, P, g, d, h, st, Ns, rb.partials,
-                     rb.counter);
+static void launch_upd(bool blk, bool init, int i, long Ns, const MeshDev& m, double* x, double* g, double* d,
+                       const double* h, const double* b, const double* P, const float* Pinv, double* z, CGState* st,
+                       RedBuf& rb, hipStream_t s) {
+  const int G = upd_blocks(m.ncell, K);
+  const int Ga = std::min(RED_MAX_BLOCKS, std::max(1, ceil_div(ceil_div(m.ncell, Cfg<K>::CPW), 4)));
+#define ATMO_UPD(B, I)                                                                                            \
+  hipLaunchKernelGGL((k_jcg_upd<K, B, I>), dim3(G), dim3(256), 0, s, i, Ns, m.ncell, x, g, d, h, b, P, Pinv, z, st, \
+                     rb.pa, Ga, rb.pb, G)
+  if (blk) {
+    if (init) ATMO_UPD(true, true);
+    else ATMO_UPD(true, false);
+  } else {
+    if (init) ATMO_UPD(false, true);
+    else ATMO_UPD(false, false);
+  }
+#undef ATMO_UPD
   ATMO_HIP_CHECK(hipGetLastError());
 }
 
-__global__ void k_jcg_update(long n, double* __restrict__ x, double* __restrict__ g, const double* __restrict__ d,
-                             const double* __restrict__ h, const double* __restrict__ P, CGState* st,
-                             double* partials, unsigned* counter) {
-  if (st->done) return;
-  const double a = st->alpha;
-  double v[2] = {0.0, 0.0};
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    x[i] += a * d[i];
-    const double gi = g[i] + a * h[i];
-    g[i] = gi;
-    v[0] += gi * gi;
-    v[1] += gi * P[i] * gi;
-  }
-  grid_reduce<2, false>(v, partials, counter, [&](double (&s)[2]) {
-    st->it += 1;
-    cg_check(st, sqrt(s[0]));
-    st->beta = s[1] / st->gh;
-    st->gh = s[1];
-  });
+void jcg_init(long Ns, const MeshDev& m, int K, const double* b, const double* P, const float* Pinv, double* g,
+              double* d, double* z, CGState* st, RedBuf& rb, hipStream_t s) {
+  const bool blk = Pinv != nullptr;
+  if (K == 1) launch_upd<1>(blk, true, 0, Ns, m, nullptr, g, d, nullptr, b, P, Pinv, z, st, rb, s);
+  else if (K == 2) launch_upd<2>(blk, true, 0, Ns, m, nullptr, g, d, nullptr, b, P, Pinv, z, st, rb, s);
+  else launch_upd<3>(blk, true, 0, Ns, m, nullptr, g, d, nullptr, b, P, Pinv, z, st, rb, s);
+  const int G = upd_blocks(m.ncell, K);
+  fin(rb.pb, G, 2, FIN_JCG_INIT, nullptr, st, s);
 }
 
-void jcg_update(long n, double* x, double* g, const double* d, const double* h, const double* P, CGState* st,
-                RedBuf& rb, hipStream_t s) {
-  hipLaunchKernelGGL(k_jcg_update, dim3(red_blocks(n)), dim3(RED_THREADS), 0, s, n, x, g, d, h, P, st, rb.partials,
-                     rb.counter);
+void jcg_update(int i, long Ns, const MeshDev& m, int K, double* x, double* g, const double* d, const double* h,
+                const double* P, const float* Pinv, double* z, CGState* st, RedBuf& rb, hipStream_t s) {
+  const bool blk = Pinv != nullptr;
+  double* dd = const_cast<double*>(d);
+  if (K == 1) launch_upd<1>(blk, false, i, Ns, m, x, g, dd, h, nullptr, P, Pinv, z, st, rb, s);
+  else if (K == 2) launch_upd<2>(blk, false, i, Ns, m, x, g, dd, h, nullptr, P, Pinv, z, st, rb, s);
+  else launch_upd<3>(blk, false, i, Ns, m, x, g, dd, h, nullptr, P, Pinv, z, st, rb, s);
+}
+
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Apply kernel of iteration i: (re-reducing the previous update's partials)
+// convergence test on |g_{i-1}|, beta = gz_{i-1}/gz_{i-2}; then per cell
+// d = -z + beta d (z = P g for Jacobi), h = A d, partial d.h -> pa.
+// Four independent waves per block, each owning CPW cells at a time.
+template <int K, bool USEZ>
+__global__ __launch_bounds__(256) void k_jcg_apply(int i, MeshDev m, const double* __restrict__ rho,
+                                                   const double* __restrict__ P, const double* __restrict__ z,
+                                                   const double* __restrict__ g, double* __restrict__ d,
+                                                   double* __restrict__ h, CGState* st, long Ns, double* pa,
+                                                   const double* pb, int Gb) {
+  using C = Cfg<K>;
+  constexpr int ND2 = C::ND2, CPW = C::CPW, NQ = C::NQ, ND = C::ND;
+  __shared__ double s_u[4][2][CPW * ND2];
+  __shared__ double s_r[4][CPW * ND2];
+  __shared__ double s_red[4][2][CPW * ND * NQ];
+  __shared__ int s_done;
+  if (st->done) return;
+  double beta = 0.0;
+  if (i >= 2) {
+    const double* newer = pb + (long)((i - 1) & 1) * 2 * RED_MAX_BLOCKS;
+    const double* older = pb + (long)(i & 1) * 2 * RED_MAX_BLOCKS;
+    const double gg = sum_slot(newer, Gb);
+    const double gz = sum_slot(newer + RED_MAX_BLOCKS, Gb);
+    const double gz_old = sum_slot(older + RED_MAX_BLOCKS, Gb);
+    const double res = sqrt(gg);
+    if (threadIdx.x == 0) {
+      int dn = 0;
+      if (res <= st->tol) dn = 1;
+      else if (i - 1 >= st->max_its || isnan(res)) dn = -1;
+      s_done = dn;
+      if (blockIdx.x == 0) {
+        st->res = res;
+        st->beta = gz / gz_old;
+        st->gh = gz;
+        if (dn) st->done = dn;
+      }
+    }
+    __syncthreads();
+    if (s_done) return;
+    beta = gz / gz_old;
+  }
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int ngroups = (m.ncell + CPW - 1) / CPW;
+  const long lim = (long)m.ncell * ND2;
+  double acc = 0.0;
+  for (int grp = blockIdx.x * 4 + w; grp < ngroups; grp += gridDim.x * 4) {
+    const long base = (long)grp * CPW * ND2;
+    for (int t = lane; t < CPW * ND2; t += 64) {
+      const long gi = base + t;
+#pragma unroll
+      for (int comp = 0; comp < 2; ++comp) {
+        double dv = 0.0;
+        if (gi < lim) {
+          const long k = comp * Ns + gi;
+          const double zz = USEZ ? z[k] : P[k] * g[k];
+          dv = -zz + beta * d[k];
+          d[k] = dv;
+        }
+        s_u[w][comp][t] = dv;
+      }
+      s_r[w][t] = gi < lim ? rho[gi] : 0.0;
+    }
+    wave_sync();
+    wave_mass_core<K, double, 2, true>(m, grp * CPW, &s_u[w][0][0], s_r[w], 1.0, &s_red[w][0][0]);
+    wave_sync();
+    for (int t = lane; t < CPW * ND2; t += 64) {
+      const long gi = base + t;
+      if (gi >= lim) break;
+#pragma unroll
+      for (int comp = 0; comp < 2; ++comp) {
+        const double hv = wave_mass_out<K, double>(&s_red[w][comp][0], t);
+        h[comp * Ns + gi] = hv;
+        acc += s_u[w][comp][t] * hv;
+      }
+    }
+    wave_sync();
+  }
+  double v[1] = {acc};
+  block_reduce<1, false>(v);
+  if (threadIdx.x == 0) pa[blockIdx.x] = v[0];
+}
+
+template <int K>
+void jcg_apply(int i, const MeshDev& m, const double* rho, const double* P, const double* z, const double* g,
+               double* d, double* h, CGState* st, RedBuf& rb, long Ns, hipStream_t s) {
+  const int ngroups = ceil_div(m.ncell, Cfg<K>::CPW);
+  const int Ga = std::min(RED_MAX_BLOCKS, std::max(1, ceil_div(ngroups, 4)));
+  const int Gb = upd_blocks(m.ncell, K);
+  if (z)
+    hipLaunchKernelGGL((k_jcg_apply<K, true>), dim3(Ga), dim3(256), 0, s, i, m, rho, P, z, g, d, h, st, Ns, rb.pa,
+                       rb.pb, Gb);
+  else
+    hipLaunchKernelGGL((k_jcg_apply<K, false>), dim3(Ga), dim3(256), 0, s, i, m, rho, P, z, g, d, h, st, Ns, rb.pa,
+                       rb.pb, Gb);
+  ATMO_HIP_CHECK(hipGetLastError());
+}
+
+// ------------------------------------------------ cell block inverses
+// One wave per cell: assemble the (k+1)^2 x (k+1)^2 block of A (rho-weighted
+// mass at the 2k+1 Gauss points) in LDS, invert it in place by Gauss-Jordan
+// (SPD, no pivoting) in FP64, store FP32.
+template <int K>
+__global__ __launch_bounds__(256) void k_block_inv(MeshDev m, const double* __restrict__ rho, float* __restrict__ Pinv) {
+  using C = Cfg<K>;
+  constexpr int ND = C::ND, ND2 = C::ND2, NQ = C::NQ, NQ2 = NQ * NQ;
+  constexpr auto& tb = basis::kT<K>;
+  __shared__ double s_A[4][ND2 * ND2];
+  __shared__ double s_w[4][NQ2];
+  __shared__ double s_S[NQ * ND];
+  __shared__ double s_row[4][ND2], s_col[4][ND2];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int t = threadIdx.x; t < NQ * ND; t += blockDim.x) {
+    double v = 0;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+      for (int i = 0; i < ND; ++i)
+        if (q * ND + i == t) v = tb.S[q][i];
+    s_S[t] = v;
+  }
+  __syncthreads();
+  const int cell = blockIdx.x * 4 + w;
+  if (cell >= m.ncell) return;
+  const int ci = cell % m.nx, cj = cell / m.nx;
+  const CellVerts vv = cell_verts(m, ci, cj);
+  // weights JxW * rho at the NQ^2 points (lane = q)
+  for (int q = lane; q < NQ2; q += 64) {
+    const int c = q % NQ, r = q / NQ;
+    const ColGeo<double> g = col_geo<K, double>(vv, c);
+    double eta = 0, wr = 0;
+#pragma unroll
+    for (int rr = 0; rr < NQ; ++rr)
+      if (rr == r) {
+        eta = tb.gq[rr];
+        wr = tb.gw[rr];
+      }
+    const double xs = g.a0 + eta * g.a1, zs = g.b0 + eta * g.b1;
+    const double jxw = (xs * g.ze - g.xe * zs) * g.wc * wr;
+    double rv = 0;
+    for (int j = 0; j < ND; ++j)
+      for (int i = 0; i < ND; ++i) rv += s_S[c * ND + i] * s_S[r * ND + j] * rho[(long)cell * ND2 + j * ND + i];
+    s_w[w][q] = jxw * rv;
+  }
+  wave_sync();
+  for (int e = lane; e < ND2 * ND2; e += 64) {
+    const int a = e / ND2, b = e - a * ND2;
+    const int ja = a / ND, ia = a - ja * ND, jb = b / ND, ib = b - jb * ND;
+    double s = 0;
+    for (int r = 0; r < NQ; ++r) {
+      const double fy = s_S[r * ND + ja] * s_S[r * ND + jb];
+      for (int c = 0; c < NQ; ++c) s += fy * s_S[c * ND + ia] * s_S[c * ND + ib] * s_w[w][r * NQ + c];
+    }
+    s_A[w][e] = s;
+  }
+  wave_sync();
+  for (int k = 0; k < ND2; ++k) {
+    const double piv = s_A[w][k * ND2 + k];
+    for (int j = lane; j < ND2; j += 64) {
+      s_row[w][j] = (j == k ? 1.0 : s_A[w][k * ND2 + j]) / piv;
+      s_col[w][j] = s_A[w][j * ND2 + k];
+    }
+    wave_sync();
+    for (int e = lane; e < ND2 * ND2; e += 64) {
+      const int r = e / ND2, j = e - r * ND2;
+      if (r == k) s_A[w][e] = s_row[w][j];
+      else s_A[w][e] = (j == k ? 0.0 : s_A[w][e]) - s_col[w][r] * s_row[w][j];
+    }
+    wave_sync();
+  }
+  for (int e = lane; e < ND2 * ND2; e += 64) Pinv[(long)cell * ND2 * ND2 + e] = (float)s_A[w][e];
+}
+
+template <int K>
+void build_block_inverse(const MeshDev& m, const double* rho, float* Pinv, hipStream_t s) {
+  hipLaunchKernelGGL((k_block_inv<K>), dim3(ceil_div(m.ncell, 4)), dim3(256), 0, s, m, rho, Pinv);
   ATMO_HIP_CHECK(hipGetLastError());
 }
 
 // ------------------------------------------------------------ GMRES
-__global__ void k_mdot(long n, const double* __restrict__ w, const double* __restrict__ V, long ldv, int nv,
-                       double* out, double* partials, unsigned* counter) {
-  // one pass over w per vector group of 8 to bound registers
-  __shared__ int s_last;
+__global__ void k_mdot_part(long n, const double* __restrict__ w, const double* __restrict__ V, long ldv, int nv,
+                            double* part) {
   double acc[8];
   for (int g0 = 0; g0 < nv; g0 += 8) {
     const int ng = (nv - g0) < 8 ? (nv - g0) : 8;
@@ -421,41 +628,16 @@ __global__ void k_mdot(long n, const double* __restrict__ w, const double* __res
     }
     block_reduce<8, false>(acc);
     if (threadIdx.x == 0)
-      for (int k = 0; k < ng; ++k)
-        __hip_atomic_store(&partials[(long)(g0 + k) * RED_MAX_BLOCKS + blockIdx.x], acc[k], __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
+      for (int k = 0; k < ng; ++k) part[(long)(g0 + k) * RED_MAX_BLOCKS + blockIdx.x] = acc[k];
   }
-  if (threadIdx.x == 0) {
-    __threadfence();
-    unsigned t = atomicAdd(counter, 1u);
-    s_last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  for (int g0 = 0; g0 < nv; g0 += 8) {
-    const int ng = (nv - g0) < 8 ? (nv - g0) : 8;
-    double s[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) s[k] = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
-#pragma unroll
-      for (int k = 0; k < 8; ++k)
-        if (k < ng)
-          s[k] += __hip_atomic_load(&partials[(long)(g0 + k) * RED_MAX_BLOCKS + b], __ATOMIC_RELAXED,
-                                    __HIP_MEMORY_SCOPE_AGENT);
-    block_reduce<8, false>(s);
-    if (threadIdx.x == 0)
-      for (int k = 0; k < ng; ++k) out[g0 + k] = s[k];
-  }
-  if (threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 void gm_mdot(long n, const double* w, const double* V, long ldv, int nv, double* out, RedBuf& rb, hipStream_t s) {
   if (nv > RED_MAX_SLOTS) throw std::invalid_argument("gm_mdot: too many vectors");
-  hipLaunchKernelGGL(k_mdot, dim3(red_blocks(n)), dim3(RED_THREADS), 0, s, n, w, V, ldv, nv, out, rb.partials,
-                     rb.counter);
+  const int G = red_blocks(n);
+  hipLaunchKernelGGL(k_mdot_part, dim3(G), dim3(RED_THREADS), 0, s, n, w, V, ldv, nv, rb.gen);
   ATMO_HIP_CHECK(hipGetLastError());
+  fin(rb.gen, G, nv, FIN_MDOT, out, nullptr, s);
 }
 
 __global__ void k_maxpy(long n, double* __restrict__ w, const double* __restrict__ V, long ldv, int nv,
@@ -477,23 +659,13 @@ void gm_update(long n, double* x, const double* V, long ldv, int nv, const doubl
   ATMO_HIP_CHECK(hipGetLastError());
 }
 
-__global__ void k_scale_inv(long n, const double* __restrict__ x, const double* __restrict__ sc,
-                            double* __restrict__ y) {
-  const double f = 1.0 / sc[0];
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-    y[i] = x[i] * f;
-}
+#define ATMO_KRY_INST(K)                                                                                          \
+  template void jcg_apply<K>(int, const MeshDev&, const double*, const double*, const double*, const double*,      \
+                             double*, double*, CGState*, RedBuf&, long, hipStream_t);                             \
+  template void build_block_inverse<K>(const MeshDev&, const double*, float*, hipStream_t);
 
-void vec_scale_inv(long n, const double* x, const double* scal, double* y, hipStream_t s) {
-  hipLaunchKernelGGL(k_scale_inv, dim3(ew_blocks(n)), dim3(256), 0, s, n, x, scal, y);
-  ATMO_HIP_CHECK(hipGetLastError());
-}
-
-template void jcg_apply<1>(const MeshDev&, const double*, const double*, const double*, double*, double*, CGState*,
-                           RedBuf&, long, hipStream_t);
-template void jcg_apply<2>(const MeshDev&, const double*, const double*, const double*, double*, double*, CGState*,
-                           RedBuf&, long, hipStream_t);
-template void jcg_apply<3>(const MeshDev&, const double*, const double*, const double*, double*, double*, CGState*,
-                           RedBuf&, long, hipStream_t);
+ATMO_KRY_INST(1)
+ATMO_KRY_INST(2)
+ATMO_KRY_INST(3)
 
 }  // namespace atmo

@@ -57,7 +57,7 @@ def run_gpu(args):
         print(f"[bench] {name}: {args.steps} steps in {el:.3f} s  stats={st}", file=sys.stderr, flush=True)
         if name == "fp64":
             kern = {}
-            for kid, kname in ((1, "k_jcg_apply"), (0, "k_mass_A"), (2, "k_mass_M"), (3, "k_cellface_B"),
+            for kid, kname in ((1, "k_jcg_apply"), (6, "k_jcg_upd"), (0, "k_mass_A"), (2, "k_mass_M"), (3, "k_cellface_B"),
                                (4, "k_cellface_C"), (5, "k_cellface_rhs_p3")):
                 ms, b = ctx.time_kernel(kid, nrep=50)
                 kern[kname] = dict(ms=ms, bytes=b, gbs=b / (ms * 1e-3) / 1e9)

@@ -25,6 +25,7 @@ def make_pair(k, nx, nz, periodic=True, hill=True, dt=5.0, **kw):
     h = af.witch_of_agnesi(nx) if hill else np.zeros(nx + 1)
     common = dict(Ma=1.0, Fr=FR, dt=dt)
     dev = af.DeviceContext(k, nx, nz, h=h, periodic=periodic, **common, **kw)
+    kw.pop("precision", None)
     orc = Oracle(k, nx, nz, hx=h, periodic=periodic, mg_min_cells=kw.get("mg_min_cells", 8))
     orc.set_params(Ma=1.0, Fr=FR, dt=dt, eps=kw.get("eps", 1e-10), schur_inner_tol=kw.get("schur_inner_tol", 1e-12),
                    a_solve_tol=kw.get("a_solve_tol", 1e-12), fp_lock=kw.get("fp_lock", 0))
@@ -160,3 +161,43 @@ def test_imex_steps_parity(k, nx, nz, dt):
         assert e < 1e-11, name
     # mass: sum of M rho over all dofs is the integral of rho
     assert np.all(np.isfinite(r_d))
+
+
+@pytest.mark.parametrize("k,nx,nz,dt", [(3, 16, 8, 4.0)])
+def test_mixed_precision_matches_fp64(k, nx, nz, dt):
+    """FP32 multigrid and FP32 cell-block inverses inside FP64 Krylov: same
+    steps as the FP64 path to <= 1e-8 relative L2 at the reference tolerances."""
+    h = af.witch_of_agnesi(nx)
+    out = {}
+    for prec in (0, 1):
+        dev = af.DeviceContext(k, nx, nz, h=h, Ma=1.0, Fr=FR, dt=dt, eps=1e-10, precision=prec)
+        rho, u, p = dev.init_hill()
+        dev.upload_state(rho, u, p)
+        dev.imex_step(2)
+        out[prec] = dev.download_state()
+        print(prec, dev.stats())
+    for i, name in enumerate(("rho", "u", "p")):
+        e = rel(out[1][i], out[0][i])
+        print(f"{name}: mixed vs fp64 rel L2 {e:.3e}")
+        assert e < 1e-8, name
+
+
+def test_mixed_inner_solve_parity():
+    """Schur vmult with the FP32 block-inverse preconditioned inner CG
+    against the oracle (Jacobi inner CG), both at a tight inner tolerance."""
+    k, nx, nz = 3, 8, 8
+    dev, orc = make_pair(k, nx, nz, schur_inner_tol=1e-14, precision=1)
+    op = af.EULEROperator(dev)
+    (rho, u, p), (rho2, u2, p2) = fields(dev, 11, 2)
+    op.set_HYPERBOLIC_stage(1)
+    orc.set_stage(1, 2)
+    op.set_rho_for_fixed(dev.vector(af.BLOCK_DENSITY, rho2))
+    orc.set_rho_for_fixed(rho2)
+    op.set_pres_fixed(dev.vector(af.BLOCK_PRESSURE, p2))
+    orc.set_pres_fixed(p2)
+    op.set_NS_stage(2)
+    out = dev.vector(af.BLOCK_PRESSURE)
+    op.vmult(out, dev.vector(af.BLOCK_PRESSURE, p))
+    assert rel(out.download(), orc.vmult(p, 2)) < 1e-11
+    st = dev.stats()
+    assert st["inner_cg_its"] <= 4, st
