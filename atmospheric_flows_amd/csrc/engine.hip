@@ -277,6 +277,10 @@ class EngineK : public Engine {
                      s_);
           fields = 14;
           break;
+        case 7:  // plain device copy of 2*Ns doubles (counter calibration)
+          ATMO_HIP_CHECK(hipMemcpyAsync(s_t1.p, u_old.p, 2 * Ns * sizeof(double), hipMemcpyDeviceToDevice, s_));
+          fields = 4;
+          break;
         case 2:
           launch_mass<K, double>(m, 1, false, rho_old.p, nullptr, gm_p.p, 1.0, nullptr, 0.0, Ns, s_);
           fields = 2;

@@ -57,8 +57,7 @@ def run_gpu(args):
         print(f"[bench] {name}: {args.steps} steps in {el:.3f} s  stats={st}", file=sys.stderr, flush=True)
         if name == "fp64":
             kern = {}
-            for kid, kname in ((1, "k_jcg_apply"), (6, "k_jcg_upd"), (0, "k_mass_A"), (2, "k_mass_M"), (3, "k_cellface_B"),
-                               (4, "k_cellface_C"), (5, "k_cellface_rhs_p3")):
+            for kid, kname in KERNELS:
                 ms, b = ctx.time_kernel(kid, nrep=50)
                 kern[kname] = dict(ms=ms, bytes=b, gbs=b / (ms * 1e-3) / 1e9)
             res["kernels"] = kern
@@ -71,6 +70,30 @@ def run_gpu(args):
     res["dofs"] = dof_count(nx, nz, k)
     res["nx"], res["nz"] = nx, nz
     return res
+
+
+KERNELS = ((1, "k_jcg_apply"), (6, "k_jcg_upd"), (0, "k_mass_A"), (2, "k_mass_M"), (3, "k_cellface_B"),
+           (4, "k_cellface_C"), (5, "k_cellface_rhs_p3"), (7, "copy_2field"))
+
+
+def kernels_only(args):
+    """Times each hot kernel on the 512x256 Q3 data (no time loop): the
+    entry point for hardware-counter runs, which serialize every dispatch."""
+    import atmospheric_flows_amd as af
+    W = WORKLOAD
+    nx, nz, k = (args.nx or W["nx"]), (args.nz or W["nz"]), W["degree"]
+    ctx = af.DeviceContext(k, nx, nz, h=af.witch_of_agnesi(nx), Ma=1.0, Fr=1.0 / np.sqrt(9.81), dt=W["dt"])
+    rho, u, p = ctx.init_hill(seed=20240601)
+    ctx.upload_state(rho, u, p)
+    op = af.EULEROperator(ctx)
+    op.set_HYPERBOLIC_stage(1)
+    op.set_rho_for_fixed(ctx.vector(af.BLOCK_DENSITY, rho))
+    op.set_pres_fixed(ctx.vector(af.BLOCK_PRESSURE, p))
+    out = {}
+    for kid, name in KERNELS:
+        ms, b = ctx.time_kernel(kid, nrep=args.reps)
+        out[name] = dict(ms=ms, bytes=b, gbs=b / (ms * 1e-3) / 1e9)
+    print(json.dumps({"kernels": out, "nx": nx, "nz": nz}))
 
 
 def cpu_baseline(threads):
@@ -99,12 +122,17 @@ def main():
     ap.add_argument("--nx", type=int, default=0)
     ap.add_argument("--nz", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernels-only", action="store_true")
+    ap.add_argument("--reps", type=int, default=50)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
                                    "(multi-GPU slab decomposition is a design, not built)"}))
         sys.exit(2)
+    if args.kernels_only:
+        kernels_only(args)
+        return
     r = run_gpu(args)
     dofs = r["dofs"]
     mixed = dofs * args.steps / r["mixed"]["seconds"]

@@ -158,7 +158,8 @@ atmo_status atmo_init_hill_state(const atmo_mesh_desc* mesh, double theta0, doub
  * context's stream on its current data and reports the mean duration from
  * HIP events recorded on that stream. which: 0 A apply (k_mass, weighted),
  * 1 fused inner-CG apply (k_jcg_apply), 2 M_rho apply, 3 B, 4 C,
- * 5 stage-3 pressure residual, 6 fused inner-CG update (k_jcg_upd). Also reports the kernel's algorithmic HBM
+ * 5 stage-3 pressure residual, 6 fused inner-CG update (k_jcg_upd),
+ * 7 plain device-to-device copy of one velocity field (calibration). Also reports the kernel's algorithmic HBM
  * bytes per launch. */
 atmo_status atmo_time_kernel(atmo_ctx* ctx, int which, int nrep, double* ms_per_launch, double* bytes_per_launch);
 
