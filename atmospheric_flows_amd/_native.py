@@ -40,6 +40,13 @@ class Stats(ctypes.Structure):
                                                "coarse_its", "schur_matvecs", "host_syncs")]
 
 
+class StepDiag(ctypes.Structure):
+    _fields_ = [("min_density", ctypes.c_double * 3), ("max_density", ctypes.c_double * 3),
+                ("max_celerity", ctypes.c_double), ("max_velocity", ctypes.c_double),
+                ("min_cell_diameter", ctypes.c_double), ("cfl_c", ctypes.c_double), ("cfl_u", ctypes.c_double),
+                ("nonfinite", ctypes.c_long)]
+
+
 # (name, restype, argtypes) of every exported symbol
 SIGNATURES = [
     ("atmo_params_default", None, [ctypes.POINTER(Params)]),
@@ -79,6 +86,7 @@ SIGNATURES = [
     ("atmo_state_upload", ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp]),
     ("atmo_state_download", ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp]),
     ("atmo_imex_step", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("atmo_step_diagnostics", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(StepDiag)]),
     ("atmo_init_hill_state", ctypes.c_int, [ctypes.POINTER(MeshDesc)] + [ctypes.c_double] * 8 +
      [ctypes.c_uint64, _dp, _dp, _dp]),
     ("atmo_time_kernel", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp]),

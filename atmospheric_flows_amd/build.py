@@ -12,7 +12,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libatmo_hip.so")
-SOURCES = ["ops.hip", "krylov.hip", "mg.hip", "engine.hip", "capi.cpp"]
+DRIVER = os.path.join(HERE, "atmo_run")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+SOURCES = ["ops.hip", "krylov.hip", "mg.hip", "diag.hip", "engine.hip", "capi.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "-std=c++17", "-O3", "-fPIC", "--offload-arch=gfx950",
@@ -46,7 +48,21 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
     os.replace(tmp, LIB)
+    build_driver()
     return LIB
+
+
+def build_driver():
+    """The C++ command-line driver (driver/atmo_run.cpp) linked against the library."""
+    cxx = os.environ.get("CXX", "g++")
+    tmp = DRIVER + ".tmp"
+    cmd = [cxx, "-std=c++17", "-O2", "-Wall", "-I" + INCLUDE, os.path.join(HERE, "driver", "atmo_run.cpp"),
+           "-L" + HERE, "-latmo_hip", "-Wl,-rpath,$ORIGIN", "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("driver build failed:\n" + r.stdout + r.stderr)
+    os.replace(tmp, DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":

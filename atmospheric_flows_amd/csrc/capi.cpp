@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/atmo_hip.h"
+#include "diag.h"
 #include "engine.h"
 #include "krylov.h"
 
@@ -464,6 +465,31 @@ atmo_status atmo_imex_step(atmo_ctx* ctx, int nsteps) {
     if (nsteps < 0) throw std::invalid_argument("nsteps must be >= 0");
     for (int n = 0; n < nsteps; ++n) ctx->eng->step();
     ATMO_HIP_CHECK(hipStreamSynchronize(ctx->eng->stream()));
+    return ATMO_OK;
+  });
+}
+
+atmo_status atmo_step_diagnostics(atmo_ctx* ctx, int of_state, atmo_step_diag* out) {
+  return guarded(ctx, [&]() {
+    if (!ctx || !out) throw std::invalid_argument("null argument");
+    double rec[DG_SLOTS];
+    ctx->eng->step_diagnostics(rec, of_state != 0);
+    for (int s = 0; s < 3; ++s) {
+      out->min_density[s] = rec[DG_MIN_RHO + s];
+      out->max_density[s] = rec[DG_MAX_RHO + s];
+    }
+    out->max_celerity = rec[DG_MAX_CEL];
+    out->max_velocity = rec[DG_MAX_VEL];
+    double bad = 0.0;
+    for (int s = 0; s < 4; ++s) bad += rec[DG_NONFINITE + s];
+    out->nonfinite = (long)bad;
+    const MeshDesc& m = ctx->mesh;
+    out->min_cell_diameter = min_cell_diameter(m.nx, m.nz, m.Lx, m.H, m.h.data());
+    // Courant numbers as printed by EulerSolver::run (Atmospheric_Flow.cc:1225-1235), dim = 2
+    const EngineParams& p = ctx->eng->params();
+    const double f = p.dt * ctx->degree * std::sqrt(2.0) / out->min_cell_diameter;
+    out->cfl_c = f * out->max_celerity / p.Ma;
+    out->cfl_u = f * out->max_velocity;
     return ATMO_OK;
   });
 }

@@ -79,6 +79,10 @@ class Engine {
   virtual void store_state(double* rho, double* u, double* p) = 0;
   virtual void step() = 0;
   virtual double* state_ptr(int which) = 0;  // 0 rho, 1 u, 2 p (device)
+  // Diagnostics record (DiagSlot layout, DG_SLOTS doubles) of the last step,
+  // or, with of_state, first recomputed on the current state (density
+  // slots of stage 3 and the flow slots). Synchronizes.
+  virtual void step_diagnostics(double* rec, bool of_state) = 0;
 
   // ---- measurement: mean duration of one kernel on the engine's data
   virtual void time_kernel(int which, int nrep, double* ms_per_launch, double* bytes_per_launch) = 0;

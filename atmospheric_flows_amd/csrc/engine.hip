@@ -8,6 +8,7 @@
 #include <functional>
 #include <type_traits>
 
+#include "diag.h"
 #include "engine.h"
 #include "krylov.h"
 #include "mg.h"
@@ -118,6 +119,9 @@ class EngineK : public Engine {
     scal.alloc(256);
     scal.zero(s_);
     coarse_its.alloc(1);
+    diag_part.alloc(DIAG_PARTIALS);
+    diag_rec.alloc(DG_SLOTS);
+    diag_rec.zero(s_);
     ATMO_HIP_CHECK(hipHostMalloc((void**)&hst, sizeof(CGState) * 4, hipHostMallocDefault));
     ATMO_HIP_CHECK(hipHostMalloc((void**)&hscal, sizeof(double) * 256, hipHostMallocDefault));
     ATMO_HIP_CHECK(hipHostMalloc((void**)&hint, sizeof(int) * 4, hipHostMallocDefault));
@@ -322,6 +326,7 @@ class EngineK : public Engine {
     const long Ns = fine().Ns;
     // stage 1
     update_density(1);
+    launch_diag_density<K>(fine().m, rho2.p, 0, diag_part.p, diag_rec.p, s_);
     set_rho_for_fixed(rho2.p);
     vec_copy(Ns, p_old.p, p_fixed_old.p, s_);
     vec_copy(2 * Ns, u_old.p, u_fixed.p, s_);
@@ -330,6 +335,7 @@ class EngineK : public Engine {
     vec_copy(2 * Ns, u_fixed.p, u2.p, s_);
     // stage 2
     update_density(2);
+    launch_diag_density<K>(fine().m, rho3.p, 1, diag_part.p, diag_rec.p, s_);
     set_rho_for_fixed(rho3.p);
     vec_copy(Ns, p2.p, p_fixed_old.p, s_);
     vec_copy(2 * Ns, u2.p, u_fixed.p, s_);
@@ -338,11 +344,24 @@ class EngineK : public Engine {
     vec_copy(2 * Ns, u_fixed.p, u3.p, s_);
     // stage 3
     update_density(3);
+    launch_diag_density<K>(fine().m, rho_curr.p, 2, diag_part.p, diag_rec.p, s_);
     set_rho_for_fixed(rho_curr.p);
     update_velocity(3);
     update_pressure();
     vec_copy(Ns, rho_curr.p, rho_old.p, s_);
     vec_copy(2 * Ns, u_curr.p, u_old.p, s_);
+    launch_diag_flow<K>(fine().m, rho_old.p, u_old.p, p_old.p, P.gamma_gas, Ns, diag_part.p, diag_rec.p, s_);
+  }
+
+  void step_diagnostics(double* rec, bool of_state) override {
+    if (of_state) {
+      launch_diag_density<K>(fine().m, rho_old.p, 2, diag_part.p, diag_rec.p, s_);
+      launch_diag_flow<K>(fine().m, rho_old.p, u_old.p, p_old.p, P.gamma_gas, fine().Ns, diag_part.p, diag_rec.p,
+                          s_);
+    }
+    ATMO_HIP_CHECK(hipMemcpyAsync(hscal + 128, diag_rec.p, sizeof(double) * DG_SLOTS, hipMemcpyDeviceToHost, s_));
+    sync();
+    std::memcpy(rec, hscal + 128, sizeof(double) * DG_SLOTS);
   }
 
  private:
@@ -353,6 +372,7 @@ class EngineK : public Engine {
   std::vector<std::unique_ptr<Level>> lv;
   RedBuf rb;
   DBuf<double> red_partials;
+  DBuf<double> diag_part, diag_rec;  // per-step diagnostics: block partials and the step record
   DBuf<float> blk_inv;  // FP32 inverse cell blocks of A (mixed form)
   DBuf<double> in_z;
   DBuf<CGState> cgst;

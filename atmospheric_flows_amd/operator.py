@@ -120,6 +120,17 @@ class DeviceContext:
     def imex_step(self, nsteps=1):
         N.check(N.load().atmo_imex_step(self.handle, nsteps), self.handle)
 
+    def step_diagnostics(self, of_state=False):
+        """Diagnostics of the last step (min/max density per stage, max
+        celerity and velocity, Courant numbers, non-finite count); with
+        of_state, recomputed on the current state first."""
+        d = N.StepDiag()
+        N.check(N.load().atmo_step_diagnostics(self.handle, 1 if of_state else 0, ctypes.byref(d)), self.handle)
+        out = {n: getattr(d, n) for n, _ in N.StepDiag._fields_}
+        out["min_density"] = list(d.min_density)
+        out["max_density"] = list(d.max_density)
+        return out
+
     def stats(self):
         s = N.Stats()
         N.load().atmo_get_stats(self.handle, ctypes.byref(s))

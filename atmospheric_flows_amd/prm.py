@@ -33,6 +33,9 @@ _ENTRIES = {
     ("Device", "hill_center"): ("hill_center", float, 20000.0, lambda v: True),
     ("Device", "precision"): ("precision", str, "fp64", lambda v: v in ("fp64", "mixed")),
     ("Device", "gravity"): ("gravity", float, 9.81, lambda v: v > 0),
+    ("Device", "wind"): ("wind", float, 10.0, lambda v: True),
+    ("Device", "brunt_vaisala"): ("brunt_vaisala", float, 0.01, lambda v: v > 0),
+    ("Device", "seed"): ("seed", int, 20240601, lambda v: v >= 0),
 }
 
 
@@ -59,6 +62,9 @@ class DataStorage:
     hill_center: float = 20000.0
     precision: str = "fp64"
     gravity: float = 9.81
+    wind: float = 10.0
+    brunt_vaisala: float = 0.01
+    seed: int = 20240601
 
 
 def _convert(typ, text):
@@ -110,7 +116,7 @@ def read_data(filename):
                 raise ValueError(f"{filename}:{ln}: cannot parse '{line}'")
     if stack:
         raise ValueError(f"{filename}: unterminated subsection '{stack[-1]}'")
-    if d.dt <= 0.0 or d.dt > 0.5 * (d.final_time - d.initial_time):
-        # EulerSolver constructor check (Atmospheric_Flow.cc:256)
-        raise ValueError(f"The time step {d.dt} is out of range (0, {0.5 * (d.final_time - d.initial_time)}]")
+    if d.dt <= 0.0 or d.dt > 0.5 * d.final_time:
+        # EulerSolver constructor check (Atmospheric_Flow.cc:256, T = final_time at :220)
+        raise ValueError(f"The time step {d.dt} is out of range (0, {0.5 * d.final_time}]")
     return d

@@ -147,6 +147,28 @@ atmo_status atmo_state_upload(atmo_ctx* ctx, const double* rho, const double* u,
 atmo_status atmo_state_download(atmo_ctx* ctx, double* rho, double* u, double* p);
 atmo_status atmo_imex_step(atmo_ctx* ctx, int nsteps);
 
+/* Per-step diagnostics of EulerSolver::run, computed on the device inside
+ * every step without host synchronization: get_min_density / get_max_density
+ * after each density update (Atmospheric_Flow.cc:1121-1122, 1164-1165,
+ * 1205-1206; :1020-1061), compute_max_celerity and get_max_velocity of the new
+ * state (:1066-1087, :992-1015) at the QGauss(k+1) points, and the Courant
+ * numbers built from them with GridTools::minimal_cell_diameter (:1225-1235). */
+typedef struct {
+  double min_density[3];  /* min rho at the points after the density update of stage 1..3 */
+  double max_density[3];  /* linfty_norm of the same densities */
+  double max_celerity;    /* max sqrt(gamma p / rho); the reference prints it times 1/Ma */
+  double max_velocity;    /* max |u| */
+  double min_cell_diameter;
+  double cfl_c;           /* dt * max_celerity / Ma * k * sqrt(2) / min_cell_diameter */
+  double cfl_u;           /* dt * max_velocity * k * sqrt(2) / min_cell_diameter */
+  long nonfinite;         /* non-finite values met by these reductions (0 on a healthy state) */
+} atmo_step_diag;
+
+/* Reads the record of the last atmo_imex_step (synchronizes the context's
+ * stream). of_state != 0 first recomputes the stage-3 density and the flow
+ * entries on the current state (e.g. the initial state before any step). */
+atmo_status atmo_step_diagnostics(atmo_ctx* ctx, int of_state, atmo_step_diag* out);
+
 /* Hill-flow initial state at the FE_DGQ support points (replaces
  * EquationData's analytic sphere state, equation_data.h:47-176, evaluated by
  * VectorTools::interpolate at Atmospheric_Flow.cc:420-422). */

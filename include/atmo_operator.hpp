@@ -1,0 +1,261 @@
+// C++ mirror of the reference's operator/solver interface over the C ABI of
+// libatmo_hip.so (header-only).
+//
+//   atmo::Context      EulerSolver's set-up: mesh, MatrixFree storage, MG
+//                      hierarchy (Atmospheric_Flow.cc:217-409) and the IMEX
+//                      loop body (EulerSolver::run, :1100-1257).
+//   atmo::Vector       MatrixFree::initialize_dof_vector(v, block) (:351-376):
+//                      a device-resident vector of one block.
+//   atmo::EULEROperator  EULEROperator's public interface with the same member
+//                      names and argument meaning (euler_operator.h:24-67):
+//                      setters, vmult_rhs_*, vmult_pressure (B), vmult_enthalpy
+//                      (C), vmult per NS_stage, compute_diagonal.
+//
+// Errors: every failed call throws atmo::Error carrying the status and the
+// library's message (deal.II's SolverControl::NoConvergence maps to
+// ATMO_NO_CONVERGENCE, its Assert on stage/argument checks to
+// ATMO_INVALID_ARG).
+#ifndef ATMO_OPERATOR_HPP
+#define ATMO_OPERATOR_HPP
+
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "atmo_hip.h"
+
+namespace atmo {
+
+class Error : public std::runtime_error {
+ public:
+  Error(atmo_status s, const std::string& msg) : std::runtime_error(msg), status(s) {}
+  atmo_status status;
+};
+
+inline void check(atmo_status s, const atmo_ctx* ctx) {
+  if (s != ATMO_OK) throw Error(s, std::string("libatmo_hip: ") + atmo_last_error(ctx));
+}
+
+inline atmo_params default_params() {
+  atmo_params p;
+  atmo_params_default(&p);
+  return p;
+}
+
+// Block of a vector, in the reference's DoF-handler order (Atmospheric_Flow.cc:334-336).
+enum Block { velocity = ATMO_BLOCK_VELOCITY, pressure = ATMO_BLOCK_PRESSURE, density = ATMO_BLOCK_DENSITY };
+
+class Context;
+
+class Vector {
+ public:
+  Vector(Context& ctx, int block);
+  Vector(Context& ctx, int block, const std::vector<double>& values) : Vector(ctx, block) { upload(values); }
+  Vector(const Vector&) = delete;
+  Vector& operator=(const Vector&) = delete;
+  Vector(Vector&& o) noexcept : ctx_(o.ctx_), v_(o.v_), block_(o.block_) { o.v_ = nullptr; }
+  ~Vector();
+
+  size_t size() const {
+    size_t n = 0;
+    atmo_vec_size(v_, &n);
+    return n;
+  }
+  int block() const { return block_; }
+  void upload(const std::vector<double>& values);
+  std::vector<double> download() const;
+  double* device_ptr() {
+    double* p = nullptr;
+    atmo_vec_device_ptr(v_, &p);
+    return p;
+  }
+  atmo_vec* handle() const { return v_; }
+
+ private:
+  atmo_ctx* ctx_;
+  atmo_vec* v_ = nullptr;
+  int block_;
+};
+
+class Context {
+ public:
+  Context(const atmo_mesh_desc& mesh, const atmo_params& params) : params_(params), mesh_(mesh) {
+    h_.assign(mesh.h, mesh.h + mesh.nx + 1);
+    mesh_.h = h_.data();
+    check(atmo_ctx_create(&mesh_, &params_, &ctx_), nullptr);
+    check(atmo_ctx_info(ctx_, &ncells_, &ns_, &nlevels_), ctx_);
+  }
+  Context(const Context&) = delete;
+  Context& operator=(const Context&) = delete;
+  ~Context() {
+    if (ctx_) atmo_ctx_destroy(ctx_);
+  }
+
+  atmo_ctx* handle() const { return ctx_; }
+  const atmo_params& params() const { return params_; }
+  void set_params(const atmo_params& p) {
+    check(atmo_set_params(ctx_, &p), ctx_);
+    params_ = p;
+  }
+  int n_cells() const { return ncells_; }
+  long scalar_dofs() const { return ns_; }
+  int n_levels() const { return nlevels_; }
+  void sync() { check(atmo_sync(ctx_), ctx_); }
+  atmo_stats stats() const {
+    atmo_stats s{};
+    check(atmo_get_stats(ctx_, &s), ctx_);
+    return s;
+  }
+  void reset_stats() { check(atmo_reset_stats(ctx_), ctx_); }
+
+  // Hill-flow initial state at the support points (host arrays).
+  void initial_state(std::vector<double>& rho, std::vector<double>& u, std::vector<double>& p, double theta0 = 300.0,
+                     double p0 = 1e5, double N = 0.01, double U = 10.0, double g = 9.81, double R = 287.0,
+                     double gamma_gas = 1.4, double perturbation = 1e-4, uint64_t seed = 1234) const {
+    rho.assign(ns_, 0.0);
+    u.assign(2 * ns_, 0.0);
+    p.assign(ns_, 0.0);
+    check(atmo_init_hill_state(&mesh_, theta0, p0, N, U, g, R, gamma_gas, perturbation, seed, rho.data(), u.data(),
+                               p.data()),
+          nullptr);
+  }
+  void upload_state(const std::vector<double>& rho, const std::vector<double>& u, const std::vector<double>& p) {
+    if ((long)rho.size() != ns_ || (long)u.size() != 2 * ns_ || (long)p.size() != ns_)
+      throw Error(ATMO_INVALID_ARG, "upload_state: sizes do not match the mesh");
+    check(atmo_state_upload(ctx_, rho.data(), u.data(), p.data()), ctx_);
+  }
+  void download_state(std::vector<double>& rho, std::vector<double>& u, std::vector<double>& p) const {
+    rho.assign(ns_, 0.0);
+    u.assign(2 * ns_, 0.0);
+    p.assign(ns_, 0.0);
+    check(atmo_state_download(ctx_, rho.data(), u.data(), p.data()), ctx_);
+  }
+  // nsteps bodies of EulerSolver::run's time loop (Atmospheric_Flow.cc:1115-1222).
+  void imex_step(int nsteps = 1) { check(atmo_imex_step(ctx_, nsteps), ctx_); }
+
+ private:
+  atmo_params params_;
+  atmo_mesh_desc mesh_;
+  std::vector<double> h_;
+  atmo_ctx* ctx_ = nullptr;
+  int ncells_ = 0, nlevels_ = 0;
+  long ns_ = 0;
+};
+
+inline Vector::Vector(Context& ctx, int block) : ctx_(ctx.handle()), block_(block) {
+  check(atmo_vec_alloc(ctx_, block, &v_), ctx_);
+}
+inline Vector::~Vector() {
+  if (v_) atmo_vec_free(ctx_, v_);
+}
+inline void Vector::upload(const std::vector<double>& values) {
+  check(atmo_vec_upload(ctx_, v_, values.data(), values.size()), ctx_);
+}
+inline std::vector<double> Vector::download() const {
+  std::vector<double> out(size());
+  check(atmo_vec_download(ctx_, v_, out.data(), out.size()), ctx_);
+  return out;
+}
+
+// euler_operator.h:24-67. Stage numbering, source-list order and the
+// operator each NS_stage selects are the reference's; vectors live on the GPU.
+class EULEROperator {
+ public:
+  explicit EULEROperator(Context& ctx) : ctx_(ctx) {}
+
+  void set_dt(double dt) { set_param(&atmo_params::dt, dt); }
+  void set_Mach(double Ma) { set_param(&atmo_params::Ma, Ma); }
+  void set_Froude(double Fr) { set_param(&atmo_params::Fr, Fr); }
+  void set_HYPERBOLIC_stage(unsigned int stage) {
+    if (stage < 1 || stage > 3) throw Error(ATMO_INVALID_ARG, "Stage not implemented");
+    HYPERBOLIC_stage = stage;
+    apply_stages();
+  }
+  void set_NS_stage(unsigned int stage) {
+    if (stage < 1 || stage > 6) throw Error(ATMO_INVALID_ARG, "Stage not implemented");
+    NS_stage = stage;
+    apply_stages();
+  }
+  void set_rho_for_fixed(const Vector& src) { check(atmo_set_rho_for_fixed(h(), src.handle()), h()); }
+  void set_pres_fixed(const Vector& src) { check(atmo_set_pres_fixed(h(), src.handle()), h()); }
+  // The reference stores u_fixed (euler_operator.h:336-339) but no form reads
+  // the member: the fixed-point velocity enters through the rhs source lists.
+  void set_u_fixed(const Vector&) {}
+
+  void vmult_rhs_rho_update(Vector& dst, const std::vector<const Vector*>& src) const {
+    apply_stages();
+    auto s = handles(src);
+    check(atmo_rhs_rho(h(), dst.handle(), s.data(), (int)s.size()), h());
+  }
+  void vmult_rhs_pressure(Vector& dst, const std::vector<const Vector*>& src) const {
+    apply_stages();
+    auto s = handles(src);
+    check(atmo_rhs_pressure(h(), dst.handle(), s.data(), (int)s.size()), h());
+  }
+  void vmult_rhs_velocity_fixed(Vector& dst, const std::vector<const Vector*>& src) const {
+    apply_stages();
+    auto s = handles(src);
+    check(atmo_rhs_velocity(h(), dst.handle(), s.data(), (int)s.size()), h());
+  }
+  void vmult_pressure(Vector& dst, const Vector& src) const {
+    apply_stages();
+    check(atmo_vmult_pressure(h(), dst.handle(), src.handle()), h());
+  }
+  void vmult_enthalpy(Vector& dst, const Vector& src) const {
+    apply_stages();
+    check(atmo_vmult_enthalpy(h(), dst.handle(), src.handle()), h());
+  }
+  void vmult(Vector& dst, const Vector& src) const {
+    apply_stages();
+    check(atmo_vmult(h(), dst.handle(), src.handle()), h());
+  }
+  void compute_diagonal() {
+    apply_stages();
+    const int blk = (NS_stage == 3 || NS_stage == 5) ? velocity : (NS_stage == 1 || NS_stage == 4) ? density : pressure;
+    inverse_diagonal_.reset(new Vector(ctx_, blk));
+    check(atmo_inverse_diagonal(h(), inverse_diagonal_->handle()), h());
+  }
+  const Vector& get_matrix_diagonal_inverse() const {
+    if (!inverse_diagonal_) throw Error(ATMO_INVALID_ARG, "compute_diagonal() has not been called");
+    return *inverse_diagonal_;
+  }
+
+  // The stage solves' Krylov calls (Atmospheric_Flow.cc:516-535, 656-672).
+  int solve_cg_mg(unsigned int ns_stage, Vector& x, const Vector& b, double tol_factor) const {
+    int its = 0;
+    check(atmo_solve_cg_mg(h(), (int)ns_stage, x.handle(), b.handle(), tol_factor, &its), h());
+    return its;
+  }
+  int solve_schur_gmres(Vector& x, const Vector& b, double tol_factor) const {
+    apply_stages();
+    int its = 0;
+    check(atmo_solve_schur_gmres(h(), x.handle(), b.handle(), tol_factor, &its), h());
+    return its;
+  }
+
+  unsigned int HYPERBOLIC_stage = 1;
+  unsigned int NS_stage = 1;
+
+ private:
+  atmo_ctx* h() const { return ctx_.handle(); }
+  void apply_stages() const { check(atmo_set_stages(h(), (int)HYPERBOLIC_stage, (int)NS_stage), h()); }
+  void set_param(double atmo_params::*field, double v) {
+    atmo_params p = ctx_.params();
+    p.*field = v;
+    ctx_.set_params(p);
+  }
+  static std::vector<const atmo_vec*> handles(const std::vector<const Vector*>& src) {
+    std::vector<const atmo_vec*> out;
+    out.reserve(src.size());
+    for (const Vector* v : src) out.push_back(v->handle());
+    return out;
+  }
+
+  Context& ctx_;
+  std::unique_ptr<Vector> inverse_diagonal_;
+};
+
+}  // namespace atmo
+
+#endif  // ATMO_OPERATOR_HPP

@@ -31,6 +31,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <limits>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -1927,6 +1928,66 @@ int orc_run(void* hp, double* rho, double* u, double* p, int nsteps, long* stats
     std::fprintf(stderr, "orc_run: %s\n", e.what());
     return 1;
   }
+}
+
+// Per-step diagnostics of EulerSolver::run on one state, evaluated at the
+// QGauss<dim>(k+1) points the reference's FEValues use
+// (Atmospheric_Flow.cc:230-232): out[0] = min rho at the points
+// (get_min_density, :1020-1049), out[1] = max |rho| over the dofs
+// (get_max_density = linfty_norm, :1052-1061), out[2] = max sqrt(gamma p / rho)
+// (compute_max_celerity, :1066-1087), out[3] = max |u| (get_max_velocity,
+// :992-1015), out[4] = number of non-finite values met.
+void orc_diagnostics(void* hp, const double* rho, const double* u, const double* p, double gamma_gas, double* out) {
+  Solver* s = ((OrcHandle*)hp)->s;
+  const Tables& T = s->fine->T;
+  const int nd = T.nd, Ns = s->fine->Ns, nc = s->fine->ncell;
+  Vec xq, wq;
+  gauss_legendre(nd, xq, wq);
+  std::vector<double> L(nd * nd);
+  for (int q = 0; q < nd; ++q)
+    for (int i = 0; i < nd; ++i) L[q * nd + i] = lagrange(T.gll, i, xq[q]);
+  double mn = std::numeric_limits<double>::max(), mx = 0.0, cel = 0.0, vel = 0.0;
+  long nonfinite = 0;
+  auto at = [&](const double* f, int c, int qx, int qz) {
+    double v = 0.0;
+    for (int j = 0; j < nd; ++j)
+      for (int i = 0; i < nd; ++i) v += L[qz * nd + j] * L[qx * nd + i] * f[c * nd * nd + j * nd + i];
+    return v;
+  };
+  for (int c = 0; c < nc; ++c) {
+    for (int qz = 0; qz < nd; ++qz)
+      for (int qx = 0; qx < nd; ++qx) {
+        const double r = at(rho, c, qx, qz), pp = at(p, c, qx, qz);
+        const double u0 = at(u, c, qx, qz), u1 = at(u + Ns, c, qx, qz);
+        const double cc = std::sqrt(gamma_gas * pp / r), vv = std::sqrt(u0 * u0 + u1 * u1);
+        if (std::isfinite(r)) mn = std::min(mn, r); else ++nonfinite;
+        if (std::isfinite(cc)) cel = std::max(cel, cc); else ++nonfinite;
+        if (std::isfinite(vv)) vel = std::max(vel, vv); else ++nonfinite;
+      }
+    for (int i = 0; i < nd * nd; ++i) {
+      const double r = rho[c * nd * nd + i];
+      if (std::isfinite(r)) mx = std::max(mx, std::fabs(r)); else ++nonfinite;
+    }
+  }
+  out[0] = mn;
+  out[1] = mx;
+  out[2] = cel;
+  out[3] = vel;
+  out[4] = (double)nonfinite;
+}
+
+// GridTools::minimal_cell_diameter: min over cells of the longer diagonal
+// (the diameter of a quadrilateral with straight edges).
+double orc_min_cell_diameter(void* hp) {
+  const Mesh& m = ((OrcHandle*)hp)->s->fine->m;
+  double d = std::numeric_limits<double>::max();
+  for (int c = 0; c < m.ncell(); ++c) {
+    double x[4], z[4];
+    m.verts(c, x, z);
+    const double d1 = std::hypot(x[3] - x[0], z[3] - z[0]), d2 = std::hypot(x[2] - x[1], z[2] - z[1]);
+    d = std::min(d, std::max(d1, d2));
+  }
+  return d;
 }
 
 }  // extern "C"

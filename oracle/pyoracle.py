@@ -69,6 +69,9 @@ def lib():
         L.orc_init_hill.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 8 + [ctypes.c_uint64, _dp, _dp, _dp]
         L.orc_run.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_int, ctypes.POINTER(ctypes.c_long)]
         L.orc_run.restype = ctypes.c_int
+        L.orc_diagnostics.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_double, _dp]
+        L.orc_min_cell_diameter.argtypes = [ctypes.c_void_p]
+        L.orc_min_cell_diameter.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -202,3 +205,14 @@ class Oracle:
             raise RuntimeError("oracle IMEX run failed (no convergence)")
         keys = ("cg_its", "gmres_its", "inner_cg_its", "fp_its", "mg_cycles", "coarse_its", "schur_matvecs")
         return rho, u, p, dict(zip(keys, list(st)))
+
+    def diagnostics(self, rho, u, p, gamma=1.4):
+        """(min rho at QGauss(k+1) points, max |rho| over dofs, max celerity,
+        max |u|, number of non-finite values) of one state."""
+        out = np.zeros(5)
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (rho, u, p)]
+        lib().orc_diagnostics(self._h, ptr(a[0]), ptr(a[1]), ptr(a[2]), gamma, ptr(out))
+        return out
+
+    def min_cell_diameter(self):
+        return lib().orc_min_cell_diameter(self._h)

@@ -107,3 +107,81 @@ set verbose = true
     bad.write_text("subsection Time step data\n set dt = -1\nend\n")
     with pytest.raises(ValueError):
         prm.read_data(str(bad))
+
+
+DRIVER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "atmospheric_flows_amd", "atmo_run")
+
+
+def _driver_params(path):
+    import subprocess
+    r = subprocess.run([DRIVER, "--print-params", str(path)], capture_output=True, text=True, timeout=60)
+    return r.returncode, dict(l.split(" = ", 1) for l in r.stdout.splitlines() if " = " in l), r.stderr
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="C++ driver not built")
+def test_cpp_and_python_prm_readers_agree(tmp_path):
+    """The C++ driver's reader and prm.py accept and reject the same files
+    and read the same values (parsing only, no device use)."""
+    good = tmp_path / "good.prm"
+    good.write_text("""subsection Physical data
+  set final_time = 600.0
+  set Mach = 1.0
+  set Froude = 0.3192754284  # 1/sqrt(9.81)
+end
+subsection Time step data
+  set dt = 2.5
+end
+subsection Data solve
+  set max_iterations = 500
+  set eps = 1e-10
+end
+subsection Device
+  set degree = 2
+  set nx = 40
+  set nz = 20
+  set precision = mixed
+  set wind = 12.5
+  set seed = 7
+end
+set verbose = False
+set output_interval = 10
+set saving directory = Run A
+""")
+    rc, vals, err = _driver_params(good)
+    assert rc == 0, err
+    d = prm.read_data(str(good))
+    for k, v in vals.items():
+        ref = getattr(d, k)
+        if isinstance(ref, bool):
+            assert int(v) == int(ref), k
+        elif isinstance(ref, (int, float)):
+            assert float(v) == pytest.approx(float(ref), rel=1e-15), k
+        else:
+            assert v == ref, k
+    assert set(vals) == set(prm.DataStorage.__dataclass_fields__)
+    bad_files = {
+        "dt_range.prm": "subsection Physical data\n set final_time = 10\nend\nsubsection Time step data\n set dt = 6\nend\n",
+        "pattern.prm": "subsection Data solve\n set max_iterations = 0\nend\n",
+        "unknown.prm": "subsection Data solve\n set tolerance = 1\nend\n",
+        "unterminated.prm": "subsection Data solve\n set eps = 1\n",
+        "precision.prm": "subsection Device\n set precision = fp16\nend\n",
+    }
+    for name, text in bad_files.items():
+        f = tmp_path / name
+        f.write_text(text)
+        rc, _, _ = _driver_params(f)
+        assert rc == 1, name
+        with pytest.raises(ValueError):
+            prm.read_data(str(f))
+
+
+def test_cpp_operator_header_compiles(tmp_path):
+    """include/atmo_operator.hpp is self-contained C++17 over include/atmo_hip.h."""
+    import subprocess
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    src = tmp_path / "t.cpp"
+    src.write_text('#include "atmo_operator.hpp"\nint main() { atmo_params p = atmo::default_params(); '
+                   'return p.cheb_degree == 3 ? 0 : 1; }\n')
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-I" + inc, str(src)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
