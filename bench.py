@@ -11,9 +11,14 @@ reported beside it. `roofline` is the dominant kernel of the plain path
 `cpu_baseline` is the CPU FP64 oracle on a bounded sample (see `sample`).
 """
 import argparse
+import csv
+import glob
 import json
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -78,7 +83,8 @@ KERNELS = ((1, "k_jcg_apply"), (6, "k_jcg_upd"), (0, "k_mass_A"), (2, "k_mass_M"
 
 def kernels_only(args):
     """Times each hot kernel on the 512x256 Q3 data (no time loop): the
-    entry point for hardware-counter runs, which serialize every dispatch."""
+    entry point for hardware-counter runs, which serialize every dispatch.
+    --only restricts the kernel ids (comma-separated)."""
     import atmospheric_flows_amd as af
     W = WORKLOAD
     nx, nz, k = (args.nx or W["nx"]), (args.nz or W["nz"]), W["degree"]
@@ -90,10 +96,61 @@ def kernels_only(args):
     op.set_rho_for_fixed(ctx.vector(af.BLOCK_DENSITY, rho))
     op.set_pres_fixed(ctx.vector(af.BLOCK_PRESSURE, p))
     out = {}
+    only = {int(x) for x in args.only.split(",")} if args.only else None
     for kid, name in KERNELS:
+        if only is not None and kid not in only:
+            continue
         ms, b = ctx.time_kernel(kid, nrep=args.reps)
         out[name] = dict(ms=ms, bytes=b, gbs=b / (ms * 1e-3) / 1e9)
     print(json.dumps({"kernels": out, "nx": nx, "nz": nz}))
+
+
+def _pmc_pass(counter, reps, timeout):
+    """One rocprofv3 counter pass over `--kernels-only` for the dominant
+    kernel (id 1) and the device copy (id 7). Returns {kernel name: [values
+    of its last `reps` dispatches, KiB]}."""
+    exe = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    out = tempfile.mkdtemp(prefix="atmo_pmc_", dir="/tmp")
+    cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", out, "-o", "run", "--", sys.executable,
+           os.path.abspath(__file__), "--kernels-only", "--reps", str(reps), "--only", "1,7"]
+    env = dict(os.environ, TMPDIR="/tmp")
+    print(f"[bench] counter pass {counter}", file=sys.stderr, flush=True)
+    try:
+        subprocess.run(cmd, env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=timeout)
+        files = glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True)
+        if not files:
+            return None
+        rows = {}
+        with open(files[0]) as f:
+            for r in csv.DictReader(f):
+                if r["Counter_Name"] == counter:
+                    rows.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+        return {k: v[-reps:] for k, v in rows.items()}
+    finally:
+        shutil.rmtree(out, ignore_errors=True)
+
+
+def measure_traffic(dom_kernel_prefix, copy_bytes, reps=3, timeout=240):
+    """HBM bytes per launch of the dominant kernel from FETCH_SIZE and
+    WRITE_SIZE, each scaled by what the counter reads for a device copy of
+    copy_bytes read + copy_bytes written in the same pass."""
+    res = {}
+    for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+        rows = _pmc_pass(ctr, reps, timeout)
+        if not rows:
+            return None
+        dom = [v for k, v in rows.items() if dom_kernel_prefix in k]
+        cp = rows.get("__amd_rocclr_copyBuffer")
+        if not dom or not cp:
+            return None
+        kib = sum(dom[0]) / len(dom[0])
+        cal = copy_bytes / (sum(cp) / len(cp))  # HBM bytes per counted KiB
+        res[ctr] = (kib, cal)
+    read = res["FETCH_SIZE"][0] * res["FETCH_SIZE"][1]
+    write = res["WRITE_SIZE"][0] * res["WRITE_SIZE"][1]
+    return dict(bytes=read + write, read_bytes=read, write_bytes=write,
+                fetch_kib=res["FETCH_SIZE"][0], write_kib=res["WRITE_SIZE"][0],
+                fetch_scale=res["FETCH_SIZE"][1] / 1024.0, write_scale=res["WRITE_SIZE"][1] / 1024.0)
 
 
 def cpu_baseline(threads):
@@ -117,13 +174,15 @@ def cpu_baseline(threads):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nx", type=int, default=0)
     ap.add_argument("--nz", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernels-only", action="store_true")
     ap.add_argument("--reps", type=int, default=50)
+    ap.add_argument("--only", default="")
+    ap.add_argument("--no-traffic", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     args = ap.parse_args()
     if args.gpus != 1:
@@ -133,13 +192,20 @@ def main():
     if args.kernels_only:
         kernels_only(args)
         return
+    traffic = None
+    if not args.no_traffic:
+        # counter passes run in child processes before this process touches the GPU
+        nx, nz = (args.nx or WORKLOAD["nx"]), (args.nz or WORKLOAD["nz"])
+        ns = nx * nz * (WORKLOAD["degree"] + 1) ** 2
+        traffic = measure_traffic("k_jcg_apply<", 2 * ns * 8.0)
     r = run_gpu(args)
     dofs = r["dofs"]
     mixed = dofs * args.steps / r["mixed"]["seconds"]
     fp64 = dofs * args.steps / r["fp64"]["seconds"]
     dom = r["kernels"]["k_jcg_apply"]
     roof = dict(bound="hbm", achieved=dom["gbs"], peak=HBM_PEAK_GBS, unit="GB/s", frac=dom["gbs"] / HBM_PEAK_GBS,
-                traffic=None, kernel="k_jcg_apply", bytes_per_launch=dom["bytes"], ms_per_launch=dom["ms"])
+                traffic=traffic["bytes"] if traffic else None, kernel="k_jcg_apply", bytes_per_launch=dom["bytes"],
+                ms_per_launch=dom["ms"], traffic_detail=traffic)
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_threads)
     out = {
         "metric": "DoF-steps/s mixed vs FP64 GPU at <=1e-8 rel err; p50/p99 ensemble step latency",
