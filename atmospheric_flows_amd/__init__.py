@@ -2,3 +2,4 @@
 from ._native import AtmoError, NoConvergence, load  # noqa: F401
 from .operator import (BLOCK_DENSITY, BLOCK_PRESSURE, BLOCK_VELOCITY, DeviceContext, DeviceVector,  # noqa: F401
                        EULEROperator, default_params, witch_of_agnesi)
+from .ensemble import Ensemble, Member, device_bytes, latency_summary  # noqa: F401,E402

@@ -47,6 +47,19 @@ class StepDiag(ctypes.Structure):
                 ("nonfinite", ctypes.c_long)]
 
 
+class MemberDesc(ctypes.Structure):
+    _fields_ = [("mesh", MeshDesc), ("params", Params), ("theta0", ctypes.c_double), ("p0", ctypes.c_double),
+                ("N", ctypes.c_double), ("U", ctypes.c_double), ("g", ctypes.c_double), ("R", ctypes.c_double),
+                ("gamma_gas", ctypes.c_double), ("perturbation", ctypes.c_double), ("seed", ctypes.c_uint64),
+                ("nsteps", ctypes.c_int), ("arrival_s", ctypes.c_double)]
+
+
+class MemberResult(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("arrival_s", ctypes.c_double), ("admit_s", ctypes.c_double),
+                ("start_s", ctypes.c_double), ("finish_s", ctypes.c_double), ("steps_done", ctypes.c_int),
+                ("device_bytes", ctypes.c_size_t), ("stats", Stats), ("error", ctypes.c_char * 256)]
+
+
 # (name, restype, argtypes) of every exported symbol
 SIGNATURES = [
     ("atmo_params_default", None, [ctypes.POINTER(Params)]),
@@ -90,6 +103,17 @@ SIGNATURES = [
     ("atmo_init_hill_state", ctypes.c_int, [ctypes.POINTER(MeshDesc)] + [ctypes.c_double] * 8 +
      [ctypes.c_uint64, _dp, _dp, _dp]),
     ("atmo_time_kernel", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp]),
+    ("atmo_device_bytes", ctypes.c_int, [ctypes.POINTER(MeshDesc), ctypes.POINTER(Params),
+                                         ctypes.POINTER(ctypes.c_size_t)]),
+    ("atmo_ctx_device_bytes", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)]),
+    ("atmo_ensemble_create", ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                            ctypes.POINTER(ctypes.c_void_p)]),
+    ("atmo_ensemble_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("atmo_ensemble_submit", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(MemberDesc), ctypes.POINTER(ctypes.c_int)]),
+    ("atmo_ensemble_run", ctypes.c_int, [ctypes.c_void_p, _dp]),
+    ("atmo_ensemble_result", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(MemberResult), _dp,
+                                            ctypes.c_int]),
+    ("atmo_ensemble_state", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, ctypes.c_size_t]),
 ]
 
 _lib = None

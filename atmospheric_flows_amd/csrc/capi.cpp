@@ -1,6 +1,7 @@
 // C ABI of libatmo_hip.so (include/atmo_hip.h): argument checking, exception
 // to status mapping and library-owned device vectors.
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -10,6 +11,7 @@
 #include "../../include/atmo_hip.h"
 #include "diag.h"
 #include "engine.h"
+#include "ensemble.h"
 #include "krylov.h"
 
 using namespace atmo;
@@ -19,6 +21,10 @@ struct atmo_ctx {
   MeshDesc mesh;
   int degree = 0;
   std::string err;
+};
+
+struct atmo_ensemble {
+  std::unique_ptr<Ensemble> ens;
 };
 
 struct atmo_vec {
@@ -143,24 +149,32 @@ const char* atmo_last_error(const atmo_ctx* ctx) {
   return g_err.c_str();
 }
 
+// Validated copy of a mesh description.
+static MeshDesc to_mesh(const atmo_mesh_desc* mesh) {
+  if (mesh->degree < 1 || mesh->degree > 3) throw std::invalid_argument("degree must be 1, 2 or 3");
+  if (mesh->nx < 1 || mesh->nz < 1 || !mesh->h) throw std::invalid_argument("mesh: nx, nz >= 1 and h required");
+  if (mesh->periodic && mesh->nx < 2) throw std::invalid_argument("mesh: periodic mesh needs nx >= 2");
+  if (!(mesh->Lx > 0) || !(mesh->H > 0)) throw std::invalid_argument("mesh: Lx, H must be > 0");
+  for (int i = 0; i <= mesh->nx; ++i)
+    if (!std::isfinite(mesh->h[i]) || mesh->h[i] >= mesh->H) throw std::invalid_argument("mesh: bad terrain height");
+  MeshDesc m;
+  m.nx = mesh->nx;
+  m.nz = mesh->nz;
+  m.periodic = mesh->periodic ? 1 : 0;
+  m.Lx = mesh->Lx;
+  m.H = mesh->H;
+  m.h.assign(mesh->h, mesh->h + mesh->nx + 1);
+  return m;
+}
+
 atmo_status atmo_ctx_create(const atmo_mesh_desc* mesh, const atmo_params* params, atmo_ctx** out) {
   return guarded(nullptr, [&]() {
     if (!mesh || !params || !out) throw std::invalid_argument("null argument");
     *out = nullptr;
-    if (mesh->degree < 1 || mesh->degree > 3) throw std::invalid_argument("degree must be 1, 2 or 3");
-    if (mesh->nx < 1 || mesh->nz < 1 || !mesh->h) throw std::invalid_argument("mesh: nx, nz >= 1 and h required");
-    if (mesh->periodic && mesh->nx < 2) throw std::invalid_argument("mesh: periodic mesh needs nx >= 2");
-    if (!(mesh->Lx > 0) || !(mesh->H > 0)) throw std::invalid_argument("mesh: Lx, H must be > 0");
-    for (int i = 0; i <= mesh->nx; ++i)
-      if (!std::isfinite(mesh->h[i]) || mesh->h[i] >= mesh->H) throw std::invalid_argument("mesh: bad terrain height");
+    MeshDesc m = to_mesh(mesh);
     check_params(*params);
     auto c = std::make_unique<atmo_ctx>();
-    c->mesh.nx = mesh->nx;
-    c->mesh.nz = mesh->nz;
-    c->mesh.periodic = mesh->periodic ? 1 : 0;
-    c->mesh.Lx = mesh->Lx;
-    c->mesh.H = mesh->H;
-    c->mesh.h.assign(mesh->h, mesh->h + mesh->nx + 1);
+    c->mesh = std::move(m);
     c->degree = mesh->degree;
     c->eng = Engine::create(mesh->degree, c->mesh, to_engine(*params));
     *out = c.release();
@@ -518,6 +532,107 @@ atmo_status atmo_time_kernel(atmo_ctx* ctx, int which, int nrep, double* ms_per_
   return guarded(ctx, [&]() {
     if (!ctx || !ms_per_launch || !bytes_per_launch || nrep < 1) throw std::invalid_argument("bad argument");
     ctx->eng->time_kernel(which, nrep, ms_per_launch, bytes_per_launch);
+    return ATMO_OK;
+  });
+}
+
+
+// ------------------------------------------------------------ footprint and ensemble
+atmo_status atmo_device_bytes(const atmo_mesh_desc* mesh, const atmo_params* params, size_t* bytes) {
+  return guarded(nullptr, [&]() {
+    if (!mesh || !params || !bytes) throw std::invalid_argument("null argument");
+    check_params(*params);
+    *bytes = engine_device_bytes(mesh->degree, to_mesh(mesh), to_engine(*params));
+    return ATMO_OK;
+  });
+}
+
+atmo_status atmo_ctx_device_bytes(atmo_ctx* ctx, size_t* bytes) {
+  return guarded(ctx, [&]() {
+    if (!ctx || !bytes) throw std::invalid_argument("null argument");
+    *bytes = ctx->eng->device_bytes();
+    return ATMO_OK;
+  });
+}
+
+atmo_status atmo_ensemble_create(int slots, double hbm_budget_bytes, int keep_states, atmo_ensemble** out) {
+  return guarded(nullptr, [&]() {
+    if (!out) throw std::invalid_argument("null argument");
+    auto e = std::make_unique<atmo_ensemble>();
+    e->ens = std::make_unique<Ensemble>(slots, hbm_budget_bytes, keep_states != 0);
+    *out = e.release();
+    return ATMO_OK;
+  });
+}
+
+atmo_status atmo_ensemble_destroy(atmo_ensemble* e) {
+  if (!e) return ATMO_INVALID_ARG;
+  delete e;
+  return ATMO_OK;
+}
+
+atmo_status atmo_ensemble_submit(atmo_ensemble* e, const atmo_member_desc* m, int* id) {
+  return guarded(nullptr, [&]() {
+    if (!e || !m || !id) throw std::invalid_argument("null argument");
+    check_params(m->params);
+    MemberSpec s;
+    s.degree = m->mesh.degree;
+    s.mesh = to_mesh(&m->mesh);
+    s.params = to_engine(m->params);
+    s.theta0 = m->theta0;
+    s.p0 = m->p0;
+    s.Nbv = m->N;
+    s.U = m->U;
+    s.g = m->g;
+    s.Rgas = m->R;
+    s.gamma_gas = m->gamma_gas;
+    s.pert = m->perturbation;
+    s.seed = m->seed;
+    s.nsteps = m->nsteps;
+    s.arrival_s = m->arrival_s;
+    *id = e->ens->submit(s);
+    return ATMO_OK;
+  });
+}
+
+atmo_status atmo_ensemble_run(atmo_ensemble* e, double* wall_s) {
+  return guarded(nullptr, [&]() {
+    if (!e) throw std::invalid_argument("null argument");
+    e->ens->run();
+    if (wall_s) *wall_s = e->ens->wall_s();
+    return ATMO_OK;
+  });
+}
+
+atmo_status atmo_ensemble_result(atmo_ensemble* e, int id, atmo_member_result* out, double* step_ms, int cap) {
+  return guarded(nullptr, [&]() {
+    if (!e || !out) throw std::invalid_argument("null argument");
+    const MemberResult& r = e->ens->result(id);
+    out->status = r.status;
+    out->arrival_s = r.arrival_s;
+    out->admit_s = r.admit_s;
+    out->start_s = r.start_s;
+    out->finish_s = r.finish_s;
+    out->steps_done = (int)r.step_ms.size();
+    out->device_bytes = r.device_bytes;
+    out->stats = atmo_stats{r.stats.cg_its, r.stats.gmres_its, r.stats.inner_cg_its, r.stats.fp_its,
+                            r.stats.mg_cycles, r.stats.coarse_its, r.stats.schur_matvecs, r.stats.host_syncs};
+    std::snprintf(out->error, sizeof(out->error), "%s", r.error.c_str());
+    if (step_ms)
+      for (int i = 0; i < cap && i < (int)r.step_ms.size(); ++i) step_ms[i] = r.step_ms[i];
+    return ATMO_OK;
+  });
+}
+
+atmo_status atmo_ensemble_state(atmo_ensemble* e, int id, double* rho, double* u, double* p, size_t ns) {
+  return guarded(nullptr, [&]() {
+    if (!e || !rho || !u || !p) throw std::invalid_argument("null argument");
+    const MemberResult& r = e->ens->result(id);
+    if (r.rho.empty()) throw std::invalid_argument("ensemble: state not kept (keep_states = 0) or member failed");
+    if (r.rho.size() != ns) throw std::invalid_argument("ensemble: state size mismatch");
+    std::memcpy(rho, r.rho.data(), ns * sizeof(double));
+    std::memcpy(u, r.u.data(), 2 * ns * sizeof(double));
+    std::memcpy(p, r.p.data(), ns * sizeof(double));
     return ATMO_OK;
   });
 }

@@ -59,6 +59,7 @@ class Engine {
   virtual void set_params(const EngineParams& p) = 0;
   virtual const EngineParams& params() const = 0;
   virtual EngineStats& stats() = 0;
+  virtual size_t device_bytes() const = 0;  // device memory requested by this engine
 
   // ---- EULEROperator interface on device pointers
   virtual void set_stages(int hyperbolic_stage, int ns_stage) = 0;
@@ -87,6 +88,10 @@ class Engine {
   // ---- measurement: mean duration of one kernel on the engine's data
   virtual void time_kernel(int which, int nrep, double* ms_per_launch, double* bytes_per_launch) = 0;
 };
+
+// Device bytes an engine of this mesh and parameters requests (equals
+// device_bytes() of such an engine; used for admission before creating it).
+size_t engine_device_bytes(int k, const MeshDesc& mesh, const EngineParams& p);
 
 // Host-side helpers shared by the C ABI and the driver.
 void hill_initial_state(int k, const MeshDesc& m, double theta0, double p0, double Nbv, double U, double g,

@@ -19,6 +19,10 @@ struct NoConvergence : std::runtime_error {
   explicit NoConvergence(const std::string& s) : std::runtime_error(s) {}
 };
 
+// Device bytes requested by the DBufs of the calling thread (constructors
+// measure their own footprint as the difference around their allocations).
+thread_local size_t tl_dev_bytes = 0;
+
 template <typename T>
 struct DBuf {
   T* p = nullptr;
@@ -29,14 +33,16 @@ struct DBuf {
   ~DBuf() { release(); }
   void alloc(size_t n_) {
     release();
+    if (n_) ATMO_HIP_CHECK(hipMalloc(&p, n_ * sizeof(T)));
     n = n_;
-    if (n) ATMO_HIP_CHECK(hipMalloc(&p, n * sizeof(T)));
+    tl_dev_bytes += n * sizeof(T);
   }
   void zero(hipStream_t s) {
     if (n) ATMO_HIP_CHECK(hipMemsetAsync(p, 0, n * sizeof(T), s));
   }
   void release() {
     if (p) (void)hipFree(p);
+    tl_dev_bytes -= n * sizeof(T);
     p = nullptr;
     n = 0;
   }
@@ -107,6 +113,7 @@ template <int K>
 class EngineK : public Engine {
  public:
   EngineK(const MeshDesc& mesh, const EngineParams& p) : P(p) {
+    const size_t bytes0 = tl_dev_bytes;
     ATMO_HIP_CHECK(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
     oc_ = consts();
     build_levels(mesh);
@@ -151,6 +158,7 @@ class EngineK : public Engine {
       estimate_eig(l, BLK_D);
     }
     sync();
+    bytes_ = tl_dev_bytes - bytes0;
   }
 
   ~EngineK() override {
@@ -169,7 +177,16 @@ class EngineK : public Engine {
   const EngineParams& params() const override { return P; }
   EngineStats& stats() override { return st_; }
 
+  size_t device_bytes() const override { return bytes_; }
+
   void set_params(const EngineParams& p) override {
+    const size_t bytes0 = tl_dev_bytes;
+    struct Account {
+      size_t& total;
+      const size_t& now;
+      size_t start;
+      ~Account() { total += now - start; }
+    } account{bytes_, tl_dev_bytes, bytes0};
     const bool prec_change = p.precision != P.precision;
     const bool rebuild = p.gmres_restart != P.gmres_restart;
     P = p;
@@ -388,6 +405,7 @@ class EngineK : public Engine {
       tmp1, rhs_u, rhs_p;
   EngineStats st_;
   int last_inner[2] = {8, 2};
+  size_t bytes_ = 0;  // device bytes requested by this engine's buffers
 
   OpConsts consts() const {
     OpConsts o;
@@ -924,6 +942,38 @@ void EngineK<K>::level_apply(Level& L, int blk, const T* src, T* dst, T cf, cons
   const T* rho = nullptr;
   if (blk == BLK_A) rho = std::is_same<T, double>::value ? (const T*)L.rho64.p : (const T*)L.rho32.p;
   launch_mass<K, T>(L.m, ncomp(blk), blk == BLK_A, src, rho, dst, (T)(cf * coeff(blk)), add, add_scale, L.Ns, s_);
+}
+
+// Mirrors the allocations of EngineK (constructor, build_levels): device
+// bytes an engine of this mesh and parameters requests.
+size_t engine_device_bytes(int k, const MeshDesc& mesh, const EngineParams& p) {
+  if (k < 1 || k > 3 || mesh.nx < 1 || mesh.nz < 1) throw std::invalid_argument("engine_device_bytes: bad mesh");
+  const size_t nd2 = (size_t)(k + 1) * (k + 1), D = sizeof(double), F = sizeof(float);
+  size_t b = 0;
+  int nx = mesh.nx, nz = mesh.nz, l = 0;
+  std::vector<std::pair<int, int>> lev{{nx, nz}};
+  while (nx % 2 == 0 && nz % 2 == 0 && nx >= 4 && (nx * nz) / 4 >= p.mg_min_cells) {
+    nx /= 2;
+    nz /= 2;
+    lev.push_back({nx, nz});
+  }
+  for (auto it = lev.rbegin(); it != lev.rend(); ++it, ++l) {
+    const size_t Ns = (size_t)it->first * it->second * nd2;
+    b += (it->first + 1) * D;             // h
+    b += Ns * D + Ns * F;                 // rho64, rho32
+    b += 4 * Ns * D + 4 * Ns * F;         // inv64[3], inv32[3]
+    b += 8 * Ns * D + 8 * Ns * F;         // x,b,t,d in both precisions
+    if (l == 0) b += 6 * Ns * D;          // coarse CG work
+    b += 3 * Ns * D;                      // eigenvalue start vectors
+  }
+  const size_t Ns = (size_t)mesh.nx * mesh.nz * nd2;
+  b += RedBuf::doubles() * D + 4 * sizeof(CGState) + 256 * D + sizeof(int) + DIAG_PARTIALS * D + DG_SLOTS * D;
+  b += 2 * Ns * D;                        // rho_fix, pres_fix
+  b += 8 * 2 * Ns * D + Ns * D + 2 * Ns * D;  // CG / Schur / inner work
+  if (p.precision == 1) b += (size_t)mesh.nx * mesh.nz * nd2 * nd2 * F;  // FP32 cell-block inverses
+  b += (size_t)(p.gmres_restart + 2) * Ns * D + Ns * D + (MAX_GMRES + 2) * D;
+  b += 10 * Ns * D + 7 * 2 * Ns * D;      // IMEX state and stage fields
+  return b;
 }
 
 std::unique_ptr<Engine> Engine::create(int k, const MeshDesc& mesh, const EngineParams& p) {

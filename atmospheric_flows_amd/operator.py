@@ -136,6 +136,11 @@ class DeviceContext:
         N.load().atmo_get_stats(self.handle, ctypes.byref(s))
         return {n: getattr(s, n) for n, _ in N.Stats._fields_}
 
+    def device_bytes(self):
+        b = ctypes.c_size_t()
+        N.check(N.load().atmo_ctx_device_bytes(self.handle, ctypes.byref(b)), self.handle)
+        return b.value
+
     def reset_stats(self):
         N.load().atmo_reset_stats(self.handle)
 

@@ -153,6 +153,40 @@ def measure_traffic(dom_kernel_prefix, copy_bytes, reps=3, timeout=240):
                 fetch_scale=res["FETCH_SIZE"][1] / 1024.0, write_scale=res["WRITE_SIZE"][1] / 1024.0)
 
 
+def run_ensemble(args):
+    """Ensemble of independent members on one GPU: 64 seed-drawn members,
+    mesh in {64x32, 128x64, 256x128}, degree in {2, 3}, hill height
+    U[100, 600] m, wind U[5, 15] m/s, N U[0.008, 0.012] 1/s, mixed
+    precision, dt = the headline's acoustic Courant number scaled to the
+    mesh, capped so the advective Courant number dt*U*k*sqrt(2)/dx stays
+    <= 0.35; --ens-steps steps each. Reports per member-step and
+    per-request latency percentiles and the aggregate throughput."""
+    import atmospheric_flows_amd as af
+    rng = np.random.default_rng(20240601)
+    ens = af.Ensemble(slots=args.ens_slots)
+    meshes = ((64, 32), (128, 64), (256, 128))
+    dofs = 0
+    for i in range(args.ens_members):
+        nx, nz = meshes[rng.integers(3)]
+        k = int(rng.integers(2, 4))
+        h0, U, Nbv = float(rng.uniform(100, 600)), float(rng.uniform(5, 15)), float(rng.uniform(0.008, 0.012))
+        dx = 40000.0 / nx
+        dt = min(512.0 / nx, 0.35 * dx / (U * k * np.sqrt(2.0)))
+        m = af.Member(k, nx, nz, nsteps=args.ens_steps, h0=h0, U=U, N_bv=Nbv, seed=1000 + i,
+                      params=dict(dt=dt, eps=1e-10, precision=1))
+        ens.submit(m)
+        dofs += 4 * nx * nz * (k + 1) ** 2 * args.ens_steps
+    print(f"[bench] ensemble: {args.ens_members} members x {args.ens_steps} steps, {args.ens_slots} slots",
+          file=sys.stderr, flush=True)
+    wall = ens.run()
+    res = [ens.result(i) for i in range(args.ens_members)]
+    failed = [r["error"] for r in res if r["status"] != 0]
+    out = dict(members=args.ens_members, steps_per_member=args.ens_steps, slots=args.ens_slots, wall_s=wall,
+               dof_steps_per_s=dofs / wall, failed=len(failed), errors=sorted(set(failed))[:3])
+    out.update(af.latency_summary(res))
+    return out
+
+
 def cpu_baseline(threads):
     """Oracle (C++ FP64, OpenMP) on a bounded sample of the same workload:
     one IMEX step on 128x64 Q3 at dt = 4 s, the same acoustic CFL (~13) and
@@ -183,6 +217,10 @@ def main():
     ap.add_argument("--reps", type=int, default=50)
     ap.add_argument("--only", default="")
     ap.add_argument("--no-traffic", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ens-members", type=int, default=64)
+    ap.add_argument("--ens-steps", type=int, default=5)
+    ap.add_argument("--ens-slots", type=int, default=8)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     args = ap.parse_args()
     if args.gpus != 1:
@@ -206,6 +244,7 @@ def main():
     roof = dict(bound="hbm", achieved=dom["gbs"], peak=HBM_PEAK_GBS, unit="GB/s", frac=dom["gbs"] / HBM_PEAK_GBS,
                 traffic=traffic["bytes"] if traffic else None, kernel="k_jcg_apply", bytes_per_launch=dom["bytes"],
                 ms_per_launch=dom["ms"], traffic_detail=traffic)
+    ensemble = None if args.no_ensemble else run_ensemble(args)
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_threads)
     out = {
         "metric": "DoF-steps/s mixed vs FP64 GPU at <=1e-8 rel err; p50/p99 ensemble step latency",
@@ -218,7 +257,7 @@ def main():
         "fp64_value": fp64, "speedup_mixed_vs_fp64": mixed / fp64,
         "rel_err_mixed_vs_fp64": r["rel_err_mixed_vs_fp64"],
         "solver_stats": {"mixed": r["mixed"]["stats"], "fp64": r["fp64"]["stats"]},
-        "kernels": r["kernels"], "roofline": roof, "cpu_baseline": cpu,
+        "kernels": r["kernels"], "roofline": roof, "cpu_baseline": cpu, "ensemble": ensemble,
     }
     print(json.dumps(out))
 

@@ -185,6 +185,53 @@ atmo_status atmo_init_hill_state(const atmo_mesh_desc* mesh, double theta0, doub
  * bytes per launch. */
 atmo_status atmo_time_kernel(atmo_ctx* ctx, int which, int nrep, double* ms_per_launch, double* bytes_per_launch);
 
+/* ------------------------------------------------------------ device footprint
+ * No counterpart in the reference (it sizes nothing up front): device bytes
+ * a context of this mesh and parameters allocates, for admission decisions
+ * before creating it; and what an existing context allocated. */
+atmo_status atmo_device_bytes(const atmo_mesh_desc* mesh, const atmo_params* params, size_t* bytes);
+atmo_status atmo_ctx_device_bytes(atmo_ctx* ctx, size_t* bytes);
+
+/* ------------------------------------------------------------ ensemble / serving
+ * No counterpart in the reference, which runs one simulation per process:
+ * many independent members (own mesh, parameters, hill initial state, step
+ * count, arrival time) run concurrently on one GPU. `slots` host threads
+ * each drive one member's context on its own HIP stream; requests are
+ * admitted in arrival order when a slot is free and their device footprint
+ * (atmo_device_bytes) fits the remaining HBM budget; an arrived request that
+ * does not fit is passed by later ones that do. Step and request latencies
+ * are measured on the host clock. A member's results do not depend on what
+ * else runs: every kernel and reduction is deterministic and per-stream. */
+typedef struct atmo_ensemble atmo_ensemble;
+
+typedef struct {
+  atmo_mesh_desc mesh;    /* h is copied at submit */
+  atmo_params params;
+  double theta0, p0, N, U, g, R, gamma_gas, perturbation; /* hill initial state (atmo_init_hill_state) */
+  uint64_t seed;
+  int nsteps;
+  double arrival_s;       /* seconds after atmo_ensemble_run starts */
+} atmo_member_desc;
+
+typedef struct {
+  int status;             /* atmo_status of the member's run */
+  double arrival_s, admit_s, start_s, finish_s; /* seconds after the run started */
+  int steps_done;
+  size_t device_bytes;
+  atmo_stats stats;
+  char error[256];
+} atmo_member_result;
+
+atmo_status atmo_ensemble_create(int slots, double hbm_budget_bytes, int keep_states, atmo_ensemble** out);
+atmo_status atmo_ensemble_destroy(atmo_ensemble* e);
+atmo_status atmo_ensemble_submit(atmo_ensemble* e, const atmo_member_desc* m, int* id);
+/* Runs every submitted member; returns when all are done (wall_s may be NULL). */
+atmo_status atmo_ensemble_run(atmo_ensemble* e, double* wall_s);
+/* Result of member id; step_ms (may be NULL) receives up to cap per-step wall times. */
+atmo_status atmo_ensemble_result(atmo_ensemble* e, int id, atmo_member_result* out, double* step_ms, int cap);
+/* Final state of member id (keep_states != 0); ns = scalar dofs per field. */
+atmo_status atmo_ensemble_state(atmo_ensemble* e, int id, double* rho, double* u, double* p, size_t ns);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,70 @@
+"""Ensemble / serving layer on the GPU: footprint accounting, a member's
+result independent of what runs beside it, HBM-budget admission and
+arrival-time ordering."""
+import numpy as np
+import pytest
+
+import atmospheric_flows_amd as af
+
+pytestmark = pytest.mark.gpu
+
+FR = 1.0 / np.sqrt(9.81)
+
+
+@pytest.mark.parametrize("k,nx,nz,prec", [(2, 16, 8, 0), (3, 32, 16, 1)])
+def test_footprint_matches_allocations(k, nx, nz, prec):
+    h = af.witch_of_agnesi(nx)
+    ctx = af.DeviceContext(k, nx, nz, h=h, Ma=1.0, Fr=FR, dt=4.0, precision=prec)
+    assert ctx.device_bytes() == af.device_bytes(k, nx, nz, h=h, precision=prec)
+
+
+def run_alone(m):
+    ctx = af.DeviceContext(m.degree, m.nx, m.nz, h=af.witch_of_agnesi(m.nx, h0=m.h0), Ma=1.0, Fr=FR, **m.params)
+    rho, u, p = ctx.init_hill(N_bv=m.N_bv, U=m.U, seed=m.seed)
+    ctx.upload_state(rho, u, p)
+    ctx.imex_step(m.nsteps)
+    return ctx.download_state()
+
+
+def test_member_result_does_not_depend_on_the_mix():
+    x = af.Member(2, 32, 16, nsteps=3, h0=250.0, U=8.0, N_bv=0.011, seed=5, params=dict(dt=8.0, precision=1))
+    others = [af.Member(3 if i % 2 else 2, 16 * (1 + i % 3), 8 * (1 + i % 3), nsteps=2 + i % 3, h0=100.0 + 50 * i,
+                        seed=100 + i, params=dict(dt=8.0, precision=i % 2)) for i in range(6)]
+    ens = af.Ensemble(slots=4, keep_states=True)
+    ids = [ens.submit(m) for m in others[:3]] + [ens.submit(x)] + [ens.submit(m) for m in others[3:]]
+    ens.run()
+    for i in ids:
+        r = ens.result(i)
+        assert r["status"] == 0, r["error"]
+        assert r["steps_done"] == ens.members[i].nsteps
+    mixed = ens.state(ids[3])
+    alone = run_alone(x)
+    for a, b in zip(mixed, alone):
+        assert np.array_equal(a, b)
+    solo = af.Ensemble(slots=1, keep_states=True)
+    solo.submit(x)
+    solo.run()
+    for a, b in zip(solo.state(0), alone):
+        assert np.array_equal(a, b)
+
+
+def test_admission_respects_the_hbm_budget_and_arrivals():
+    big = af.Member(3, 64, 32, nsteps=2, seed=1, params=dict(dt=8.0))
+    small = af.Member(2, 16, 8, nsteps=2, seed=2, params=dict(dt=8.0))
+    nb = af.device_bytes(3, 64, 32, h=af.witch_of_agnesi(64))
+    ns = af.device_bytes(2, 16, 8, h=af.witch_of_agnesi(16))
+    ens = af.Ensemble(slots=4, hbm_budget_bytes=nb + ns + 1)
+    bigs = [ens.submit(af.Member(**{**big.__dict__, "seed": 10 + i})) for i in range(3)]
+    late = ens.submit(af.Member(**{**small.__dict__, "arrival_s": 0.4}))
+    too_big = ens.submit(af.Member(3, 128, 64, nsteps=1, params=dict(dt=4.0)))
+    ens.run()
+    rb = [ens.result(i) for i in bigs]
+    for r in rb:
+        assert r["status"] == 0, r["error"]
+    # at most one big member holds HBM at any time
+    iv = sorted((r["admit_s"], r["finish_s"]) for r in rb)
+    for (a0, a1), (b0, b1) in zip(iv, iv[1:]):
+        assert b0 >= a1 - 1e-6
+    rl = ens.result(late)
+    assert rl["status"] == 0 and rl["admit_s"] >= 0.4 - 1e-6
+    assert ens.result(too_big)["status"] == af._native.ATMO_OOM
