@@ -50,6 +50,8 @@ struct Tables {
   double gw[NQ];
   double S[NQ][ND];
   double D[NQ][ND];
+  double St[ND][NQ];  // transposes: St[i][q] = S[q][i]
+  double Dt[ND][NQ];
 };
 
 template <int K>
@@ -105,6 +107,8 @@ constexpr Tables<K> make_tables() {
         s += pr;
       }
       t.D[q][i] = s;
+      t.St[i][q] = t.S[q][i];
+      t.Dt[i][q] = s;
     }
   }
   return t;

@@ -1,0 +1,303 @@
+// Cell + face operators on the lane-per-quadrature-column layout: the
+// linear operators B and C of the Schur complement and the device
+// functions every k_cellface-style kernel is built from.
+//
+// Each cell evaluates the flux of each of its four faces with itself as the
+// "+" side and its outward normal. All the reference's face fluxes are
+// antisymmetric under exchanging the sides (n -> -n), so this equals the
+// reference's face-loop contribution to the cell without any atomics; every
+// face flux is computed twice (once per side).
+#pragma once
+#include "common.h"
+#include "wave.h"
+
+namespace atmo {
+
+template <int NF>
+struct FieldPtrs {
+  const double* p[NF];
+};
+
+// B: euler_operator.h:1510-1617 (cell: submit_divergence, face: {p}[v.n],
+// wall: p_D = p)
+struct PhysB {
+  static constexpr int NF = 1, NO = 2;
+  static constexpr bool BDRY = true;
+  // sparsity of the cell integrand: value term, and which outputs have an x / z gradient term
+  static constexpr bool HAS_V = false;
+  static constexpr unsigned GX = 0b01, GZ = 0b10;
+  double cf;  // coeff*dt/Ma^2
+  __device__ void cell(const double (&v)[NF], double (&V)[NO], double (&Gx)[NO], double (&Gz)[NO]) const {
+    V[0] = V[1] = 0;
+    Gx[0] = -cf * v[0];
+    Gz[0] = 0;
+    Gx[1] = 0;
+    Gz[1] = -cf * v[0];
+  }
+  __device__ void face(const double (&a)[NF], const double (&b)[NF], double n0, double n1, double (&F)[NO]) const {
+    const double avg = 0.5 * (a[0] + b[0]);
+    F[0] = cf * avg * n0;
+    F[1] = cf * avg * n1;
+  }
+  __device__ void bdry(const double (&a)[NF], double n0, double n1, double (&F)[NO]) const {
+    const double pD = a[0];
+    F[0] = cf * 0.5 * (a[0] + pD) * n0;
+    F[1] = cf * 0.5 * (a[0] + pD) * n1;
+  }
+};
+
+// C: euler_operator.h:1411-1507 (nonlinear in u through lambda)
+struct PhysC {
+  static constexpr int NF = 3, NO = 1;  // pfix, ux, uz
+  static constexpr bool BDRY = false;
+  static constexpr bool HAS_V = false;
+  static constexpr unsigned GX = 0b1, GZ = 0b1;
+  double cdt, gm;
+  __device__ void cell(const double (&v)[NF], double (&V)[NO], double (&Gx)[NO], double (&Gz)[NO]) const {
+    const double s = -cdt * gm / (gm - 1.0) * v[0];
+    V[0] = 0;
+    Gx[0] = s * v[1];
+    Gz[0] = s * v[2];
+  }
+  __device__ void face(const double (&a)[NF], const double (&b)[NF], double n0, double n1, double (&F)[NO]) const {
+    const double ax = 0.5 * gm / (gm - 1.0) * (a[0] * a[1] + b[0] * b[1]);
+    const double az = 0.5 * gm / (gm - 1.0) * (a[0] * a[2] + b[0] * b[2]);
+    const double lam = fmax(fabs(a[1] * n0 + a[2] * n1), fabs(b[1] * n0 + b[2] * n1));
+    const double jump = 1.0 / (gm - 1.0) * (a[0] - b[0]);
+    F[0] = cdt * (ax * n0 + az * n1 + 0.5 * lam * jump);
+  }
+  __device__ void bdry(const double (&)[NF], double, double, double (&F)[NO]) const { F[0] = 0; }
+};
+
+// D p - C u in one pass (the Schur complement without its inner solve,
+// euler_operator.h:2374-2408): fields (pfix, ux, uz, p); D = M/(gamma-1)
+// (:1384-1408) enters as a cell value term, C with the opposite sign.
+struct PhysDminusC {
+  static constexpr int NF = 4, NO = 1;
+  static constexpr bool BDRY = false;
+  static constexpr bool HAS_V = true;
+  static constexpr unsigned GX = 0b1, GZ = 0b1;
+  PhysC c;
+  double cD;  // 1/(gamma-1)
+  __device__ void cell(const double (&v)[NF], double (&V)[NO], double (&Gx)[NO], double (&Gz)[NO]) const {
+    const double s = c.cdt * c.gm / (c.gm - 1.0) * v[0];
+    V[0] = cD * v[3];
+    Gx[0] = s * v[1];
+    Gz[0] = s * v[2];
+  }
+  __device__ void face(const double (&a)[NF], const double (&b)[NF], double n0, double n1, double (&F)[NO]) const {
+    const double ac[3] = {a[0], a[1], a[2]}, bc[3] = {b[0], b[1], b[2]};
+    double Fc[1];
+    c.face(ac, bc, n0, n1, Fc);
+    F[0] = -Fc[0];
+  }
+  __device__ void bdry(const double (&)[NF], double, double, double (&F)[NO]) const { F[0] = 0; }
+};
+
+// LDS of one 64-lane block of a cell+face kernel.
+template <int K, class Ph>
+struct CFSmem {
+  static constexpr int ND = K + 1, NQ = 2 * K + 1, ND2 = ND * ND, CPW = 64 / NQ;
+  double in[Ph::NF][CPW * ND2];
+  double T[Ph::NO][2][CPW][ND][NQ];  // column partials of the value/eta and xi test functions
+  double F[Ph::NO][2][CPW][NQ];      // left/right face fluxes times the face weight
+};
+
+// Column phase of a cell+face operator for the block's CPW cells: stages
+// the input fields in LDS, evaluates cell and face terms per quadrature
+// column (neighbour traces are read from memory; they are L2/MALL hits)
+// and leaves the column partials in sh.T / sh.F. Ends with a block
+// barrier.
+template <int K, class Ph>
+__device__ inline void cellface_columns(const MeshDev& m, const Ph& ph, const FieldPtrs<Ph::NF>& in,
+                                        CFSmem<K, Ph>& sh) {
+  using C = Cfg<K>;
+  constexpr int ND = C::ND, NQ = C::NQ, ND2 = C::ND2;
+  constexpr int NF = Ph::NF, NO = Ph::NO;
+  constexpr auto& tb = basis::kT<K>;
+  LaneMap<K> L(m.ncell);
+#pragma unroll
+  for (int f = 0; f < NF; ++f) stage_cells<K>(sh.in[f], in.p[f], m.ncell);
+  __syncthreads();
+  if (L.active) {
+    const int ci = L.cell % m.nx, cj = L.cell / m.nx;
+    const CellVerts v = cell_verts(m, ci, cj);
+    const ColGeo<double> g = col_geo<K, double>(v, m.dx, L.c);
+    double Sc[ND], Dc[ND];
+    lane_rows<K, double>(L.c, Sc, Dc);
+    const int off = L.cw * ND2;
+    double tv[NF][ND];
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int j = 0; j < ND; ++j) {
+        double s = 0;
+#pragma unroll
+        for (int i = 0; i < ND; ++i) s += Sc[i] * sh.in[f][off + j * ND + i];
+        tv[f][j] = s;
+      }
+    double Tv[NO][ND], Td[NO][ND];
+#pragma unroll
+    for (int o = 0; o < NO; ++o)
+#pragma unroll
+      for (int j = 0; j < ND; ++j) Tv[o][j] = Td[o][j] = 0;
+    const double wdx = g.wc * g.dx, wze = g.wc * g.ze, jc = wdx * g.ze;
+#pragma unroll
+    for (int r = 0; r < NQ; ++r) {
+      double val[NF];
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        double s = 0;
+#pragma unroll
+        for (int j = 0; j < ND; ++j) s += tb.S[r][j] * tv[f][j];
+        val[f] = s;
+      }
+      double V[NO], Gx[NO], Gz[NO];
+      ph.cell(val, V, Gx, Gz);
+      const double wr = tb.gw[r];
+      const double zs = g.b0 + tb.gq[r] * g.b1;
+#pragma unroll
+      for (int o = 0; o < NO; ++o) {
+        const bool gx = (Ph::GX >> o) & 1, gz = (Ph::GZ >> o) & 1;
+        // reference-space integrands times the weights: G_xi = ze*Gx, G_eta = dx*Gz - z_xi*Gx
+        double Geta = 0.0;
+        if (gx) Geta = -(g.wc * zs) * Gx[o];
+        if (gz) Geta += wdx * Gz[o];
+        Geta *= wr;
+        const double Gxi = gx ? wr * wze * Gx[o] : 0.0;
+        const double Vq = Ph::HAS_V ? V[o] * (jc * wr) : 0.0;
+#pragma unroll
+        for (int j = 0; j < ND; ++j) {
+          if (Ph::HAS_V) Tv[o][j] += tb.S[r][j] * Vq;
+          if (gx || gz) Tv[o][j] += tb.D[r][j] * Geta;
+          if (gx) Td[o][j] += tb.S[r][j] * Gxi;
+        }
+      }
+    }
+    // bottom (j-1) and top (j+1) faces: this lane is face point c
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      const int f = 2 + side;
+      const FaceGeo fg = face_geo(v, m.dx, f);
+      const double wl = fg.len * g.wc;
+      double own[NF], nb[NF], F[NO];
+#pragma unroll
+      for (int q = 0; q < NF; ++q) own[q] = tv[q][side ? K : 0];
+      const int nj = side ? cj + 1 : cj - 1;
+      if (nj >= 0 && nj < m.nz) {
+        const long nbase = (long)(nj * m.nx + ci) * ND2 + (side ? 0 : K * ND);
+#pragma unroll
+        for (int q = 0; q < NF; ++q) {
+          double s = 0;
+#pragma unroll
+          for (int i = 0; i < ND; ++i) s += Sc[i] * in.p[q][nbase + i];
+          nb[q] = s;
+        }
+        ph.face(own, nb, fg.n0, fg.n1, F);
+      } else if (Ph::BDRY) {
+        ph.bdry(own, fg.n0, fg.n1, F);
+      } else {
+#pragma unroll
+        for (int o = 0; o < NO; ++o) F[o] = 0;
+      }
+#pragma unroll
+      for (int o = 0; o < NO; ++o) Tv[o][side ? K : 0] += F[o] * wl;
+    }
+    // left (i-1) and right (i+1) faces: this lane is face point s = c
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      const int f = side;
+      const FaceGeo fg = face_geo(v, m.dx, f);
+      const double wl = fg.len * g.wc;
+      double own[NF], nb[NF], F[NO];
+      const int col = side ? K : 0;
+#pragma unroll
+      for (int q = 0; q < NF; ++q) {
+        double s = 0;
+#pragma unroll
+        for (int j = 0; j < ND; ++j) s += Sc[j] * sh.in[q][off + j * ND + col];
+        own[q] = s;
+      }
+      int ni = side ? ci + 1 : ci - 1;
+      bool inner = true;
+      if (ni < 0 || ni >= m.nx) {
+        if (m.periodic) ni = (ni + m.nx) % m.nx;
+        else inner = false;
+      }
+      if (inner) {
+        const long nbase = (long)(cj * m.nx + ni) * ND2 + (side ? 0 : K);
+#pragma unroll
+        for (int q = 0; q < NF; ++q) {
+          double s = 0;
+#pragma unroll
+          for (int j = 0; j < ND; ++j) s += Sc[j] * in.p[q][nbase + j * ND];
+          nb[q] = s;
+        }
+        ph.face(own, nb, fg.n0, fg.n1, F);
+      } else if (Ph::BDRY) {
+        ph.bdry(own, fg.n0, fg.n1, F);
+      } else {
+#pragma unroll
+        for (int o = 0; o < NO; ++o) F[o] = 0;
+      }
+#pragma unroll
+      for (int o = 0; o < NO; ++o) sh.F[o][side][L.cw][L.c] = F[o] * wl;
+    }
+#pragma unroll
+    for (int o = 0; o < NO; ++o)
+#pragma unroll
+      for (int j = 0; j < ND; ++j) {
+        sh.T[o][0][L.cw][j][L.c] = Tv[o][j];
+        if ((Ph::GX >> o) & 1) sh.T[o][1][L.cw][j][L.c] = Td[o][j];
+      }
+  }
+  __syncthreads();
+}
+
+// Table rows an output thread needs for block-local dof t (ii, jj = its x
+// and z node). When ND2 divides 64 (k = 1, 3) they do not change across a
+// thread's passes t, t+64, ..., so callers load them once.
+template <int K>
+struct OutRows {
+  static constexpr int NQ = 2 * K + 1;
+  double Si[NQ], Di[NQ], Sj[NQ];
+};
+template <int K>
+__device__ inline OutRows<K> out_rows(int t) {
+  constexpr int ND = K + 1, NQ = 2 * K + 1, ND2 = ND * ND;
+  const int o2 = t % ND2, jj = o2 / ND, ii = o2 - jj * ND;
+  OutRows<K> R;
+#pragma unroll
+  for (int c = 0; c < NQ; ++c) {
+    R.Si[c] = cT<K>.St[ii][c];
+    R.Di[c] = cT<K>.Dt[ii][c];
+    R.Sj[c] = cT<K>.St[jj][c];
+  }
+  return R;
+}
+template <int K>
+constexpr bool rows_invariant() {
+  return 64 % ((K + 1) * (K + 1)) == 0;
+}
+
+// Output phase: component o of block-local dof t = cw*ND2 + jj*ND + ii.
+template <int K, class Ph>
+__device__ inline double cellface_out(const CFSmem<K, Ph>& sh, int o, int t, const OutRows<K>& R) {
+  constexpr int ND = K + 1, NQ = 2 * K + 1, ND2 = ND * ND;
+  const int cw = t / ND2, o2 = t - cw * ND2, jj = o2 / ND, ii = o2 - jj * ND;
+  const bool gx = (Ph::GX >> o) & 1;
+  double acc = 0;
+#pragma unroll
+  for (int c = 0; c < NQ; ++c) {
+    acc += R.Si[c] * sh.T[o][0][cw][jj][c];
+    if (gx) acc += R.Di[c] * sh.T[o][1][cw][jj][c];
+  }
+  if (ii == 0 || ii == K) {
+    const int side = ii == 0 ? 0 : 1;
+#pragma unroll
+    for (int c = 0; c < NQ; ++c) acc += R.Sj[c] * sh.F[o][side][cw][c];
+  }
+  return acc;
+}
+
+}  // namespace atmo

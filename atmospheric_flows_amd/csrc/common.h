@@ -46,15 +46,20 @@ struct Cfg {
 };
 
 // Terrain-following structured mesh of one MG level: vertex (i,j) at
-// x = x0 + i*dx, z = h[i] + (H - h[i]) * j / nz. h has nx+1 entries (for a
-// periodic mesh h[nx] == h[0]).
+// x = i*dx, z = h[i] + (H - h[i]) * j * inv_nz. h has nx+1 entries (for a
+// periodic mesh h[nx] == h[0]). Cell columns are vertical: every cell's
+// left and right edges have constant x, so x_eta = 0 and x_xi = dx exactly.
 struct MeshDev {
   const double* h;
   int nx, nz;
-  double dx, H;
+  double dx, H, inv_nz;
   int periodic;
   int ncell;
 };
+
+// The 1D tables in constant memory, for lane-dependent rows and columns.
+template <int K>
+__constant__ basis::Tables<K> cT = basis::make_tables<K>();
 
 // Per-cell vertex data (x00,x10,x01,x11 and z likewise).
 struct CellVerts {
@@ -64,7 +69,7 @@ struct CellVerts {
 __device__ inline CellVerts cell_verts(const MeshDev& m, int i, int j) {
   CellVerts v;
   const double hl = m.h[i], hr = m.h[i + 1];
-  const double fj = (double)j / (double)m.nz, fj1 = (double)(j + 1) / (double)m.nz;
+  const double fj = j * m.inv_nz, fj1 = (j + 1) * m.inv_nz;
   v.x[0] = i * m.dx;
   v.x[1] = (i + 1) * m.dx;
   v.x[2] = v.x[0];
@@ -76,34 +81,26 @@ __device__ inline CellVerts cell_verts(const MeshDev& m, int i, int j) {
   return v;
 }
 
-// Bilinear geometry of the lane's quadrature column xi_c. x_eta, z_eta are
-// constant along the column; x_xi, z_xi are linear in eta.
+// Bilinear geometry of the lane's quadrature column xi_c on a vertical cell
+// column: x_xi = dx, x_eta = 0, z_eta(xi_c) = ze (constant along the
+// column) and z_xi(eta) = b0 + eta*b1. Then det J = dx*ze, and for a
+// physical gradient (Gx, Gz) the reference-space integrands are
+// G_xi = ze*Gx and G_eta = dx*Gz - z_xi*Gx (cofactor form).
 template <typename T>
 struct ColGeo {
-  T xe, ze;      // x_eta(xi_c), z_eta(xi_c)
-  T a0, a1;      // x_xi(eta) = a0 + eta*a1
-  T b0, b1;      // z_xi(eta) = b0 + eta*b1
-  T wc;          // Gauss weight of the column
+  T dx, ze, b0, b1;
+  T wc;  // Gauss weight of the column
 };
 
 template <int K, typename T>
-__device__ inline ColGeo<T> col_geo(const CellVerts& v, int c) {
-  constexpr auto& tb = basis::kT<K>;
-  double xi = 0, w = 0;
-#pragma unroll
-  for (int q = 0; q < Cfg<K>::NQ; ++q)
-    if (q == c) {
-      xi = tb.gq[q];
-      w = tb.gw[q];
-    }
+__device__ inline ColGeo<T> col_geo(const CellVerts& v, double dx, int c) {
+  const double xi = cT<K>.gq[c];
   ColGeo<T> g;
-  g.xe = (T)((1 - xi) * (v.x[2] - v.x[0]) + xi * (v.x[3] - v.x[1]));
+  g.dx = (T)dx;
   g.ze = (T)((1 - xi) * (v.z[2] - v.z[0]) + xi * (v.z[3] - v.z[1]));
-  g.a0 = (T)(v.x[1] - v.x[0]);
-  g.a1 = (T)((v.x[3] - v.x[2]) - (v.x[1] - v.x[0]));
   g.b0 = (T)(v.z[1] - v.z[0]);
   g.b1 = (T)((v.z[3] - v.z[2]) - (v.z[1] - v.z[0]));
-  g.wc = (T)w;
+  g.wc = (T)cT<K>.gw[c];
   return g;
 }
 
@@ -112,18 +109,21 @@ struct FaceGeo {
   double n0, n1, len;
 };
 
-// f: 0 left (xi=0), 1 right (xi=1), 2 bottom (eta=0), 3 top (eta=1)
-__device__ inline FaceGeo face_geo(const CellVerts& v, int f) {
-  double tx, tz, s;
-  if (f == 0) { tx = v.x[2] - v.x[0]; tz = v.z[2] - v.z[0]; s = -1; }
-  else if (f == 1) { tx = v.x[3] - v.x[1]; tz = v.z[3] - v.z[1]; s = 1; }
-  else if (f == 2) { tx = v.x[1] - v.x[0]; tz = v.z[1] - v.z[0]; s = 1; }
-  else { tx = v.x[3] - v.x[2]; tz = v.z[3] - v.z[2]; s = -1; }
+// f: 0 left (xi=0), 1 right (xi=1), 2 bottom (eta=0), 3 top (eta=1). The
+// left/right edges are vertical, so their normals are exactly (-1,0)/(1,0).
+__device__ inline FaceGeo face_geo(const CellVerts& v, double dx, int f) {
   FaceGeo g;
-  g.len = sqrt(tx * tx + tz * tz);
-  // left/top: (-tz, tx)/L ; right/bottom: (tz, -tx)/L
-  g.n0 = s * tz / g.len;
-  g.n1 = -s * tx / g.len;
+  if (f < 2) {
+    g.len = f == 0 ? v.z[2] - v.z[0] : v.z[3] - v.z[1];
+    g.n0 = f == 0 ? -1.0 : 1.0;
+    g.n1 = 0.0;
+  } else {
+    const double tz = f == 2 ? v.z[1] - v.z[0] : v.z[3] - v.z[2];
+    g.len = sqrt(dx * dx + tz * tz);
+    const double s = (f == 2 ? 1.0 : -1.0) / g.len;  // bottom: (tz, -dx)/L ; top: (-tz, dx)/L
+    g.n0 = s * tz;
+    g.n1 = -s * dx;
+  }
   return g;
 }
 
@@ -153,24 +153,15 @@ __device__ inline void stage_cells(T* __restrict__ lds, const S* __restrict__ sr
   }
 }
 
-// Per-lane copies of the lane-dependent table rows (select chains, no memory).
+// Per-lane copies of the lane-dependent table rows Sc[i] = S[c][i],
+// Dc[i] = D[c][i] (constant-memory loads).
 template <int K, typename T>
 __device__ inline void lane_rows(int c, T (&Sc)[Cfg<K>::ND], T (&Dc)[Cfg<K>::ND]) {
-  constexpr auto& tb = basis::kT<K>;
 #pragma unroll
   for (int i = 0; i < Cfg<K>::ND; ++i) {
-    Sc[i] = 0;
-    Dc[i] = 0;
+    Sc[i] = (T)cT<K>.S[c][i];
+    Dc[i] = (T)cT<K>.D[c][i];
   }
-#pragma unroll
-  for (int q = 0; q < Cfg<K>::NQ; ++q)
-    if (q == c) {
-#pragma unroll
-      for (int i = 0; i < Cfg<K>::ND; ++i) {
-        Sc[i] = (T)tb.S[q][i];
-        Dc[i] = (T)tb.D[q][i];
-      }
-    }
 }
 
 inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }

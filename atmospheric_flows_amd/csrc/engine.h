@@ -29,7 +29,7 @@ struct EngineParams {
   double cheb_range = 15.0;        // :491
   int cheb_eig_its = 10;           // :493
   int gmres_restart = 28;          // deal.II SolverGMRES default (30 temporary vectors)
-  int precision = 0;               // 0: FP64 ; 1: mixed (FP32 MG levels, MG-preconditioned inner solve)
+  int precision = 0;               // 0: FP64 ; 1: mixed (FP32 MG levels, cellwise-direct A^{-1})
 };
 
 struct MeshDesc {

@@ -12,6 +12,7 @@
 #include "engine.h"
 #include "krylov.h"
 #include "mg.h"
+#include "schur.h"
 
 namespace atmo {
 
@@ -138,8 +139,7 @@ class EngineK : public Engine {
     pres_fix.zero(s_);
     for (auto* b : {&cg_g, &cg_d, &cg_h, &s_t1, &s_t2, &in_g, &in_d, &in_h}) b->alloc(2 * Ns);
     s_t3.alloc(Ns);
-    in_z.alloc(2 * Ns);
-    if (P.precision == 1) blk_inv.alloc((size_t)fine().ncell * (K + 1) * (K + 1) * (K + 1) * (K + 1));
+    if (P.precision == 1) ainv.alloc((size_t)fine().ncell * (K + 1) * (K + 1) * (K + 1) * (K + 1));
     gm_V.alloc((size_t)(P.gmres_restart + 2) * Ns);
     gm_p.alloc(Ns);
     gm_coef.alloc(MAX_GMRES + 2);
@@ -193,8 +193,7 @@ class EngineK : public Engine {
     oc_ = consts();
     if (rebuild) gm_V.alloc((size_t)(P.gmres_restart + 2) * fine().Ns);
     if (prec_change) {
-      if (P.precision == 1 && !blk_inv.p)
-        blk_inv.alloc((size_t)fine().ncell * (K + 1) * (K + 1) * (K + 1) * (K + 1));
+      if (P.precision == 1 && !ainv.p) ainv.alloc((size_t)fine().ncell * (K + 1) * (K + 1) * (K + 1) * (K + 1));
       refresh_levels_rho();
     }
   }
@@ -280,8 +279,8 @@ class EngineK : public Engine {
       hipLaunchKernelGGL(k_set_cg, dim3(1), dim3(1), 0, s_, st, 0.0, 1 << 30, 1e-300);
       vec_copy(2 * Ns, u_old.p, s_t1.p, s_);
       vec_fill(2 * Ns, 0.0, s_t2.p, s_);
-      jcg_init(Ns, m, K, s_t1.p, fine().inv64[BLK_A].p, nullptr, in_g.p, in_d.p, nullptr, st, rb, s_);
-      jcg_apply<K>(1, m, rho_fix.p, fine().inv64[BLK_A].p, nullptr, in_g.p, in_d.p, in_h.p, st, rb, Ns, s_);
+      jcg_init(Ns, m, K, s_t1.p, fine().inv64[BLK_A].p, in_g.p, in_d.p, st, rb, s_);
+      jcg_apply<K>(1, m, rho_fix.p, fine().inv64[BLK_A].p, in_g.p, in_d.p, in_h.p, st, rb, Ns, s_);
     }
     auto launch = [&]() {
       switch (which) {
@@ -290,12 +289,11 @@ class EngineK : public Engine {
           fields = 5;
           break;
         case 1:
-          jcg_apply<K>(1, m, rho_fix.p, fine().inv64[BLK_A].p, nullptr, in_g.p, in_d.p, in_h.p, st, rb, Ns, s_);
+          jcg_apply<K>(1, m, rho_fix.p, fine().inv64[BLK_A].p, in_g.p, in_d.p, in_h.p, st, rb, Ns, s_);
           fields = 11;
           break;
         case 6:
-          jcg_update(1, Ns, m, K, s_t2.p, in_g.p, in_d.p, in_h.p, fine().inv64[BLK_A].p, nullptr, nullptr, st, rb,
-                     s_);
+          jcg_update(1, Ns, m, K, s_t2.p, in_g.p, in_d.p, in_h.p, fine().inv64[BLK_A].p, st, rb, s_);
           fields = 14;
           break;
         case 7:  // plain device copy of 2*Ns doubles (counter calibration)
@@ -321,6 +319,16 @@ class EngineK : public Engine {
           fields = 16;
           break;
         }
+        case 8:  // mixed Schur: B and the cellwise A^{-1} in one pass
+          if (!ainv.p) throw std::invalid_argument("time_kernel 8 needs the mixed form (precision 1)");
+          launch_b_ainv<K>(m, oc_, bc_coeff(), p_old.p, ainv.p, in_h.p, Ns, s_);
+          fields = 3 + (K + 1) * (K + 1);
+          break;
+        case 9:  // D p - C u, scaled (GMRES's preconditioned Schur product)
+          launch_d_minus_c<K>(m, oc_, bc_coeff(), pres_fix.p, u_old.p, p_old.p, fine().inv64[BLK_D].p, gm_p.p, Ns,
+                              s_);
+          fields = 6;
+          break;
         default:
           throw std::invalid_argument("time_kernel: unknown kernel id");
       }
@@ -390,8 +398,7 @@ class EngineK : public Engine {
   RedBuf rb;
   DBuf<double> red_partials;
   DBuf<double> diag_part, diag_rec;  // per-step diagnostics: block partials and the step record
-  DBuf<float> blk_inv;  // FP32 inverse cell blocks of A (mixed form)
-  DBuf<double> in_z;
+  DBuf<double> ainv;  // FP64 inverse cell blocks of A (mixed form)
   DBuf<CGState> cgst;
   DBuf<double> scal;
   DBuf<int> coarse_its;
@@ -404,7 +411,7 @@ class EngineK : public Engine {
   DBuf<double> rho_old, rho2, rho3, rho_curr, u_old, u2, u3, u_curr, u_fixed, p_old, p2, p3, p_fixed, p_fixed_old,
       tmp1, rhs_u, rhs_p;
   EngineStats st_;
-  int last_inner[2] = {8, 2};
+  int last_inner = 8;  // iterations of the last inner CG (first batch of the next)
   size_t bytes_ = 0;  // device bytes requested by this engine's buffers
 
   OpConsts consts() const {
@@ -463,6 +470,7 @@ class EngineK : public Engine {
       L->m.nz = d.nz;
       L->m.dx = mesh.Lx / d.nx;
       L->m.H = mesh.H;
+      L->m.inv_nz = 1.0 / d.nz;
       L->m.periodic = mesh.periodic;
       L->m.ncell = L->ncell;
       const long n2 = 2 * L->Ns;
@@ -518,7 +526,7 @@ class EngineK : public Engine {
       level_diag(*L, BLK_A);
     }
     for (int l = 1; l < (int)lv.size(); ++l) estimate_eig(l, BLK_A);
-    if (P.precision == 1) build_block_inverse<K>(F.m, rho_fix.p, blk_inv.p, s_);
+    if (P.precision == 1) build_block_inverse<K>(F.m, rho_fix.p, ainv.p, s_);
   }
 
   // Chebyshev eigenvalue estimate: <= cheb_eig_its CG iterations with the
@@ -701,27 +709,23 @@ class EngineK : public Engine {
   }
 
   // inner A^{-1} of the Schur complement (euler_operator.h:2391-2397): CG
-  // to 1e-12*|b| from a zero guess with fused kernels and a device-side
-  // convergence flag; the host polls once per batch of iterations. The plain
-  // form uses the reference's Jacobi preconditioner; the mixed form uses the
-  // FP32 inverse of every cell block of A (A is block diagonal, so this is
-  // an FP32 solve refined by the FP64 CG).
+  // to 1e-12*|b| from a zero guess with the reference's Jacobi
+  // preconditioner, fused kernels and a device-side convergence flag; the
+  // host polls once per batch of iterations.
   int inner_solve(double* x, const double* b) {
     const long Ns = fine().Ns;
     const long n = 2 * Ns;
     CGState* st = cgst.p + 1;
     const double* PA = fine().inv64[BLK_A].p;
-    const float* Pinv = P.precision == 1 ? blk_inv.p : nullptr;
-    double* z = Pinv ? in_z.p : nullptr;
     hipLaunchKernelGGL(k_set_cg, dim3(1), dim3(1), 0, s_, st, 0.0, P.schur_inner_max, P.schur_inner_tol);
     vec_fill(n, 0.0, x, s_);
-    jcg_init(Ns, fine().m, K, b, PA, Pinv, in_g.p, in_d.p, z, st, rb, s_);
-    int batch = std::max(1, last_inner[Pinv ? 1 : 0]);
+    jcg_init(Ns, fine().m, K, b, PA, in_g.p, in_d.p, st, rb, s_);
+    int batch = std::max(1, last_inner);
     int i = 1, its = 0;
     while (true) {
       for (int r = 0; r < batch; ++r, ++i) {
-        jcg_apply<K>(i, fine().m, rho_fix.p, PA, z, in_g.p, in_d.p, in_h.p, st, rb, Ns, s_);
-        jcg_update(i, Ns, fine().m, K, x, in_g.p, in_d.p, in_h.p, PA, Pinv, z, st, rb, s_);
+        jcg_apply<K>(i, fine().m, rho_fix.p, PA, in_g.p, in_d.p, in_h.p, st, rb, Ns, s_);
+        jcg_update(i, Ns, fine().m, K, x, in_g.p, in_d.p, in_h.p, PA, st, rb, s_);
       }
       read_state(1);
       its = hst[1].it;
@@ -729,19 +733,35 @@ class EngineK : public Engine {
       if (hst[1].done == -1) throw NoConvergence("Schur inner CG: no convergence");
       batch = 2;
     }
-    last_inner[Pinv ? 1 : 0] = its;
+    last_inner = its;
     st_.inner_cg_its += its;
     return its;
   }
 
-  // S p = D p - C A^{-1} B p (apply_add NS 2, euler_operator.h:2374-2408)
-  void schur_vmult(const double* src, double* dst) {
+  // S p = D p - C A^{-1} B p (apply_add NS 2, euler_operator.h:2374-2408),
+  // times the per-dof factor scale when given (GMRES's Jacobi
+  // preconditioner). Plain form: B, the reference's inner CG, then D - C in
+  // one pass. Mixed form: B and the cellwise-direct A^{-1} in one pass (A is
+  // block diagonal; its FP64 cell inverses are rebuilt with every density).
+  void schur_vmult(const double* src, double* dst, const double* scale = nullptr) {
     ++st_.schur_matvecs;
     const long Ns = fine().Ns;
-    launch_B<K>(fine().m, oc_, bc_coeff(), src, s_t1.p, Ns, s_);
-    inner_solve(s_t2.p, s_t1.p);
-    launch_C<K>(fine().m, oc_, bc_coeff(), pres_fix.p, s_t2.p, s_t3.p, Ns, s_);
-    launch_mass<K, double>(fine().m, 1, false, src, nullptr, dst, coeff(BLK_D), s_t3.p, -1.0, Ns, s_);
+    if (P.precision == 1) {
+      launch_b_ainv<K>(fine().m, oc_, bc_coeff(), src, ainv.p, s_t2.p, Ns, s_);
+    } else {
+      launch_B<K>(fine().m, oc_, bc_coeff(), src, s_t1.p, Ns, s_);
+      inner_solve(s_t2.p, s_t1.p);
+    }
+    launch_d_minus_c<K>(fine().m, oc_, bc_coeff(), pres_fix.p, s_t2.p, src, scale, dst, Ns, s_);
+  }
+
+  // The A solve the reference runs to the near-machine-precision tolerance
+  // a_solve_tol (A^{-1} rhs_u, Atmospheric_Flow.cc:571): CG + MG (plain
+  // form); cellwise-direct with the FP64 cell inverses (mixed form). The
+  // velocity updates at the outer tolerance eps keep CG + MG in both forms.
+  void solve_A_tight(double* x, const double* b) {
+    if (P.precision == 1) launch_ainv_apply<K>(fine().m, ainv.p, b, x, fine().Ns, s_);
+    else pcg_mg(BLK_A, x, b, P.a_solve_tol, P.max_its);
   }
 
   // GMRES, left Jacobi (diag D) preconditioning, convergence on the
@@ -785,13 +805,11 @@ class EngineK : public Engine {
         ++acc;
         dim = j + 1;
         double* w = V + (long)(j + 1) * Ns;
-        schur_vmult(V + (long)j * Ns, p);
-        vec_mul(Ns, p, PD, w, s_);
+        schur_vmult(V + (long)j * Ns, w, PD);
         // h_i = w.V_i (i <= j) and |w|^2 in one pass
         gm_mdot(Ns, w, V, Ns, j + 2, coef, rb, s_);
-        gm_maxpy(Ns, w, V, Ns, j + 1, coef, s_);
+        gm_maxpy_norm(Ns, w, V, Ns, j + 1, coef, scal.p + 1, rb, s_);
         ATMO_HIP_CHECK(hipMemcpyAsync(hcol.data(), coef, sizeof(double) * (j + 2), hipMemcpyDeviceToHost, s_));
-        vec_dot(Ns, w, w, scal.p + 1, true, rb, s_);
         ATMO_HIP_CHECK(hipMemcpyAsync(hscal, scal.p, sizeof(double) * 2, hipMemcpyDeviceToHost, s_));
         sync();
         const double nstart = std::sqrt(hcol[j + 1]);
@@ -800,9 +818,8 @@ class EngineK : public Engine {
         for (int i = 0; i <= j; ++i) hc[i] = hcol[i];
         if (nw < 10.0 * nstart * std::sqrt(2.220446049250313e-16)) {
           gm_mdot(Ns, w, V, Ns, j + 1, coef, rb, s_);
-          gm_maxpy(Ns, w, V, Ns, j + 1, coef, s_);
+          gm_maxpy_norm(Ns, w, V, Ns, j + 1, coef, scal.p + 1, rb, s_);
           ATMO_HIP_CHECK(hipMemcpyAsync(hcol.data(), coef, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, s_));
-          vec_dot(Ns, w, w, scal.p + 1, true, rb, s_);
           ATMO_HIP_CHECK(hipMemcpyAsync(hscal, scal.p, sizeof(double) * 2, hipMemcpyDeviceToHost, s_));
           sync();
           for (int i = 0; i <= j; ++i) hc[i] += hcol[i];
@@ -879,7 +896,7 @@ class EngineK : public Engine {
       const double* su[7] = {rho_old.p, u_old.p, p_old.p, rho2.p, u2.p, p2.p, rho3.p};
       rhs_velocity(su, 7, rhs_u.p);
     }
-    pcg_mg(BLK_A, tmp1.p, rhs_u.p, P.a_solve_tol, P.max_its);
+    solve_A_tight(tmp1.p, rhs_u.p);
     const long Ns = fine().Ns;
     vmult_enthalpy(tmp1.p, s_t3.p);
     vec_axpby(Ns, -1.0, s_t3.p, 1.0, rhs_p.p, s_);
@@ -969,8 +986,8 @@ size_t engine_device_bytes(int k, const MeshDesc& mesh, const EngineParams& p) {
   const size_t Ns = (size_t)mesh.nx * mesh.nz * nd2;
   b += RedBuf::doubles() * D + 4 * sizeof(CGState) + 256 * D + sizeof(int) + DIAG_PARTIALS * D + DG_SLOTS * D;
   b += 2 * Ns * D;                        // rho_fix, pres_fix
-  b += 8 * 2 * Ns * D + Ns * D + 2 * Ns * D;  // CG / Schur / inner work
-  if (p.precision == 1) b += (size_t)mesh.nx * mesh.nz * nd2 * nd2 * F;  // FP32 cell-block inverses
+  b += 8 * 2 * Ns * D + Ns * D;           // CG / Schur / inner work
+  if (p.precision == 1) b += (size_t)mesh.nx * mesh.nz * nd2 * nd2 * D;  // FP64 cell-block inverses of A
   b += (size_t)(p.gmres_restart + 2) * Ns * D + Ns * D + (MAX_GMRES + 2) * D;
   b += 10 * Ns * D + 7 * 2 * Ns * D;      // IMEX state and stage fields
   return b;

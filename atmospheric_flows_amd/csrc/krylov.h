@@ -71,26 +71,28 @@ void cg_residual(long n, double* g, const double* b, bool have_ax, hipStream_t s
 void cg_init_check(long n, const double* g, CGState* st, bool tol_from_res0, RedBuf& rb, hipStream_t s);
 
 // --- fused CG on the velocity block A (inner Schur solve), zero initial
-// guess, tolerance tol_factor*|b|. Preconditioner: Jacobi (P = inverse
-// diagonal, FP64) or the FP32 cellwise inverse blocks (Pinv). Iteration i
-// (1-based, passed by the host) is jcg_apply(i) then jcg_update(i).
-void jcg_init(long Ns, const MeshDev& m, int K, const double* b, const double* P, const float* Pinv, double* g,
-              double* d, double* z, CGState* st, RedBuf& rb, hipStream_t s);
+// guess, tolerance tol_factor*|b|, Jacobi preconditioner (P = inverse
+// diagonal). Iteration i (1-based, passed by the host) is jcg_apply(i) then
+// jcg_update(i).
+void jcg_init(long Ns, const MeshDev& m, int K, const double* b, const double* P, double* g, double* d, CGState* st,
+              RedBuf& rb, hipStream_t s);
 template <int K>
-void jcg_apply(int i, const MeshDev& m, const double* rho, const double* P, const double* z, const double* g,
-               double* d, double* h, CGState* st, RedBuf& rb, long Ns, hipStream_t s);
+void jcg_apply(int i, const MeshDev& m, const double* rho, const double* P, const double* g, double* d, double* h,
+               CGState* st, RedBuf& rb, long Ns, hipStream_t s);
 void jcg_update(int i, long Ns, const MeshDev& m, int K, double* x, double* g, const double* d, const double* h,
-                const double* P, const float* Pinv, double* z, CGState* st, RedBuf& rb, hipStream_t s);
+                const double* P, CGState* st, RedBuf& rb, hipStream_t s);
 
-// FP32 inverse of every cell block of A (rho-weighted mass), [cell][i][j]
+// FP64 inverse of every cell block of A (rho-weighted mass), [cell][i][j]
+// (symmetric, so row i is also column i)
 template <int K>
-void build_block_inverse(const MeshDev& m, const double* rho, float* Pinv, hipStream_t s);
+void build_block_inverse(const MeshDev& m, const double* rho, double* Ainv, hipStream_t s);
 
 // --- GMRES pieces
 // out[i] = w . V_i for i < nv (V column-major with leading dimension ldv)
 void gm_mdot(long n, const double* w, const double* V, long ldv, int nv, double* out, RedBuf& rb, hipStream_t s);
-// w -= sum_i coef[i] V_i  (coef device array)
-void gm_maxpy(long n, double* w, const double* V, long ldv, int nv, const double* coef, hipStream_t s);
+// w -= sum_i coef[i] V_i (coef device array) and *norm = |w| of the result (device scalar)
+void gm_maxpy_norm(long n, double* w, const double* V, long ldv, int nv, const double* coef, double* norm, RedBuf& rb,
+                   hipStream_t s);
 // x += sum_i y[i] V_i
 void gm_update(long n, double* x, const double* V, long ldv, int nv, const double* y, hipStream_t s);
 

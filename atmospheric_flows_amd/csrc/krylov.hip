@@ -311,18 +311,16 @@ void cg_init_check(long n, const double* g, CGState* st, bool tol_from_res0, Red
 // Update kernel: one thread per (cell, dof), CB cells per block batch.
 // INIT: g = -b (x = 0), d = 0. Otherwise x += alpha d, g += alpha h with
 // alpha = gh / dh re-reduced from the partials of jcg_apply (pa) and of the
-// previous update (pb[(i-1)&1]). Preconditioned residual z = P g (Jacobi) or
-// z = Pinv_cell g_cell (FP32 cell blocks). Partials (g.g, g.z) -> pb[i&1].
-template <int K, bool BLK, bool INIT>
+// previous update (pb[(i-1)&1]). Jacobi-preconditioned residual z = P g.
+// Partials (g.g, g.z) -> pb[i&1].
+template <int K, bool INIT>
 __global__ __launch_bounds__(256) void k_jcg_upd(int i, long Ns, int ncell, double* __restrict__ x,
                                                  double* __restrict__ g, double* __restrict__ d,
                                                  const double* __restrict__ h, const double* __restrict__ b,
-                                                 const double* __restrict__ P, const float* __restrict__ Pinv,
-                                                 double* __restrict__ z, CGState* st, double* pa, int Ga, double* pb,
-                                                 int Gb) {
+                                                 const double* __restrict__ P, CGState* st, double* pa, int Ga,
+                                                 double* pb, int Gb) {
   constexpr int ND2 = (K + 1) * (K + 1);
   constexpr int CB = 256 / ND2;
-  __shared__ double s_g[CB * ND2];
   if (!INIT && st->done) return;
   double alpha = 0.0;
   if (!INIT) {
@@ -340,38 +338,21 @@ __global__ __launch_bounds__(256) void k_jcg_upd(int i, long Ns, int ncell, doub
   double v[2] = {0.0, 0.0};
   for (int bt = blockIdx.x; bt < nbatch; bt += gridDim.x) {
     const int cell = bt * CB + cb;
-    const bool act = cb < CB && cell < ncell;
+    if (cb >= CB || cell >= ncell) continue;
 #pragma unroll
     for (int comp = 0; comp < 2; ++comp) {
       const long k = comp * Ns + (long)cell * ND2 + li;
-      double gv = 0.0;
-      if (act) {
-        if (INIT) {
-          gv = -b[k];
-          d[k] = 0.0;
-        } else {
-          x[k] += alpha * d[k];
-          gv = g[k] + alpha * h[k];
-        }
-        g[k] = gv;
-      }
-      double zv;
-      if (BLK) {
-        if (act) s_g[cb * ND2 + li] = gv;
-        __syncthreads();
-        zv = 0.0;
-        if (act) {
-          const float* row = Pinv + ((long)cell * ND2 + li) * ND2;
-#pragma unroll
-          for (int j = 0; j < ND2; ++j) zv += (double)row[j] * s_g[cb * ND2 + j];
-          z[k] = zv;
-        }
-        __syncthreads();
+      double gv;
+      if (INIT) {
+        gv = -b[k];
+        d[k] = 0.0;
       } else {
-        zv = act ? P[k] * gv : 0.0;
+        x[k] += alpha * d[k];
+        gv = g[k] + alpha * h[k];
       }
+      g[k] = gv;
       v[0] += gv * gv;
-      v[1] += gv * zv;
+      v[1] += gv * P[k] * gv;
     }
   }
   block_reduce<2, false>(v);
@@ -390,42 +371,34 @@ static int upd_blocks(int ncell, int K) {
 }
 
 template <int K>
-static void launch_upd(bool blk, bool init, int i, long Ns, const MeshDev& m, double* x, double* g, double* d,
-                       const double* h, const double* b, const double* P, const float* Pinv, double* z, CGState* st,
-                       RedBuf& rb, hipStream_t s) {
+static void launch_upd(bool init, int i, long Ns, const MeshDev& m, double* x, double* g, double* d,
+                       const double* h, const double* b, const double* P, CGState* st, RedBuf& rb, hipStream_t s) {
   const int G = upd_blocks(m.ncell, K);
   const int Ga = std::min(RED_MAX_BLOCKS, std::max(1, ceil_div(ceil_div(m.ncell, Cfg<K>::CPW), 4)));
-#define ATMO_UPD(B, I)                                                                                            \
-  hipLaunchKernelGGL((k_jcg_upd<K, B, I>), dim3(G), dim3(256), 0, s, i, Ns, m.ncell, x, g, d, h, b, P, Pinv, z, st, \
-                     rb.pa, Ga, rb.pb, G)
-  if (blk) {
-    if (init) ATMO_UPD(true, true);
-    else ATMO_UPD(true, false);
-  } else {
-    if (init) ATMO_UPD(false, true);
-    else ATMO_UPD(false, false);
-  }
-#undef ATMO_UPD
+  if (init)
+    hipLaunchKernelGGL((k_jcg_upd<K, true>), dim3(G), dim3(256), 0, s, i, Ns, m.ncell, x, g, d, h, b, P, st, rb.pa,
+                       Ga, rb.pb, G);
+  else
+    hipLaunchKernelGGL((k_jcg_upd<K, false>), dim3(G), dim3(256), 0, s, i, Ns, m.ncell, x, g, d, h, b, P, st, rb.pa,
+                       Ga, rb.pb, G);
   ATMO_HIP_CHECK(hipGetLastError());
 }
 
-void jcg_init(long Ns, const MeshDev& m, int K, const double* b, const double* P, const float* Pinv, double* g,
-              double* d, double* z, CGState* st, RedBuf& rb, hipStream_t s) {
-  const bool blk = Pinv != nullptr;
-  if (K == 1) launch_upd<1>(blk, true, 0, Ns, m, nullptr, g, d, nullptr, b, P, Pinv, z, st, rb, s);
-  else if (K == 2) launch_upd<2>(blk, true, 0, Ns, m, nullptr, g, d, nullptr, b, P, Pinv, z, st, rb, s);
-  else launch_upd<3>(blk, true, 0, Ns, m, nullptr, g, d, nullptr, b, P, Pinv, z, st, rb, s);
+void jcg_init(long Ns, const MeshDev& m, int K, const double* b, const double* P, double* g, double* d, CGState* st,
+              RedBuf& rb, hipStream_t s) {
+  if (K == 1) launch_upd<1>(true, 0, Ns, m, nullptr, g, d, nullptr, b, P, st, rb, s);
+  else if (K == 2) launch_upd<2>(true, 0, Ns, m, nullptr, g, d, nullptr, b, P, st, rb, s);
+  else launch_upd<3>(true, 0, Ns, m, nullptr, g, d, nullptr, b, P, st, rb, s);
   const int G = upd_blocks(m.ncell, K);
   fin(rb.pb, G, 2, FIN_JCG_INIT, nullptr, st, s);
 }
 
 void jcg_update(int i, long Ns, const MeshDev& m, int K, double* x, double* g, const double* d, const double* h,
-                const double* P, const float* Pinv, double* z, CGState* st, RedBuf& rb, hipStream_t s) {
-  const bool blk = Pinv != nullptr;
+                const double* P, CGState* st, RedBuf& rb, hipStream_t s) {
   double* dd = const_cast<double*>(d);
-  if (K == 1) launch_upd<1>(blk, false, i, Ns, m, x, g, dd, h, nullptr, P, Pinv, z, st, rb, s);
-  else if (K == 2) launch_upd<2>(blk, false, i, Ns, m, x, g, dd, h, nullptr, P, Pinv, z, st, rb, s);
-  else launch_upd<3>(blk, false, i, Ns, m, x, g, dd, h, nullptr, P, Pinv, z, st, rb, s);
+  if (K == 1) launch_upd<1>(false, i, Ns, m, x, g, dd, h, nullptr, P, st, rb, s);
+  else if (K == 2) launch_upd<2>(false, i, Ns, m, x, g, dd, h, nullptr, P, st, rb, s);
+  else launch_upd<3>(false, i, Ns, m, x, g, dd, h, nullptr, P, st, rb, s);
 }
 
 __device__ inline void wave_sync() {
@@ -436,11 +409,11 @@ __device__ inline void wave_sync() {
 
 // Apply kernel of iteration i: (re-reducing the previous update's partials)
 // convergence test on |g_{i-1}|, beta = gz_{i-1}/gz_{i-2}; then per cell
-// d = -z + beta d (z = P g for Jacobi), h = A d, partial d.h -> pa.
+// d = -P g + beta d, h = A d, partial d.h -> pa.
 // Four independent waves per block, each owning CPW cells at a time.
-template <int K, bool USEZ>
+template <int K>
 __global__ __launch_bounds__(256) void k_jcg_apply(int i, MeshDev m, const double* __restrict__ rho,
-                                                   const double* __restrict__ P, const double* __restrict__ z,
+                                                   const double* __restrict__ P,
                                                    const double* __restrict__ g, double* __restrict__ d,
                                                    double* __restrict__ h, CGState* st, long Ns, double* pa,
                                                    const double* pb, int Gb) {
@@ -488,8 +461,7 @@ __global__ __launch_bounds__(256) void k_jcg_apply(int i, MeshDev m, const doubl
         double dv = 0.0;
         if (gi < lim) {
           const long k = comp * Ns + gi;
-          const double zz = USEZ ? z[k] : P[k] * g[k];
-          dv = -zz + beta * d[k];
+          dv = -P[k] * g[k] + beta * d[k];
           d[k] = dv;
         }
         s_u[w][comp][t] = dv;
@@ -499,12 +471,15 @@ __global__ __launch_bounds__(256) void k_jcg_apply(int i, MeshDev m, const doubl
     wave_sync();
     wave_mass_core<K, double, 2, true>(m, grp * CPW, &s_u[w][0][0], s_r[w], 1.0, &s_red[w][0][0]);
     wave_sync();
+    double Si[NQ];
+    lane_cols_S<K, double>(lane % ND, Si);
     for (int t = lane; t < CPW * ND2; t += 64) {
       const long gi = base + t;
       if (gi >= lim) break;
+      if (64 % ND != 0) lane_cols_S<K, double>(t % ND, Si);
 #pragma unroll
       for (int comp = 0; comp < 2; ++comp) {
-        const double hv = wave_mass_out<K, double>(&s_red[w][comp][0], t);
+        const double hv = wave_mass_out<K, double>(&s_red[w][comp][0], t, Si);
         h[comp * Ns + gi] = hv;
         acc += s_u[w][comp][t] * hv;
       }
@@ -517,26 +492,22 @@ __global__ __launch_bounds__(256) void k_jcg_apply(int i, MeshDev m, const doubl
 }
 
 template <int K>
-void jcg_apply(int i, const MeshDev& m, const double* rho, const double* P, const double* z, const double* g,
-               double* d, double* h, CGState* st, RedBuf& rb, long Ns, hipStream_t s) {
+void jcg_apply(int i, const MeshDev& m, const double* rho, const double* P, const double* g, double* d, double* h,
+               CGState* st, RedBuf& rb, long Ns, hipStream_t s) {
   const int ngroups = ceil_div(m.ncell, Cfg<K>::CPW);
   const int Ga = std::min(RED_MAX_BLOCKS, std::max(1, ceil_div(ngroups, 4)));
   const int Gb = upd_blocks(m.ncell, K);
-  if (z)
-    hipLaunchKernelGGL((k_jcg_apply<K, true>), dim3(Ga), dim3(256), 0, s, i, m, rho, P, z, g, d, h, st, Ns, rb.pa,
-                       rb.pb, Gb);
-  else
-    hipLaunchKernelGGL((k_jcg_apply<K, false>), dim3(Ga), dim3(256), 0, s, i, m, rho, P, z, g, d, h, st, Ns, rb.pa,
-                       rb.pb, Gb);
+  hipLaunchKernelGGL((k_jcg_apply<K>), dim3(Ga), dim3(256), 0, s, i, m, rho, P, g, d, h, st, Ns, rb.pa, rb.pb, Gb);
   ATMO_HIP_CHECK(hipGetLastError());
 }
 
 // ------------------------------------------------ cell block inverses
 // One wave per cell: assemble the (k+1)^2 x (k+1)^2 block of A (rho-weighted
-// mass at the 2k+1 Gauss points) in LDS, invert it in place by Gauss-Jordan
-// (SPD, no pivoting) in FP64, store FP32.
+// mass at the 2k+1 Gauss points) in LDS and invert it in place by
+// Gauss-Jordan (SPD, no pivoting) in FP64.
 template <int K>
-__global__ __launch_bounds__(256) void k_block_inv(MeshDev m, const double* __restrict__ rho, float* __restrict__ Pinv) {
+__global__ __launch_bounds__(256) void k_block_inv(MeshDev m, const double* __restrict__ rho,
+                                                   double* __restrict__ Ainv) {
   using C = Cfg<K>;
   constexpr int ND = C::ND, ND2 = C::ND2, NQ = C::NQ, NQ2 = NQ * NQ;
   constexpr auto& tb = basis::kT<K>;
@@ -562,7 +533,7 @@ __global__ __launch_bounds__(256) void k_block_inv(MeshDev m, const double* __re
   // weights JxW * rho at the NQ^2 points (lane = q)
   for (int q = lane; q < NQ2; q += 64) {
     const int c = q % NQ, r = q / NQ;
-    const ColGeo<double> g = col_geo<K, double>(vv, c);
+    const ColGeo<double> g = col_geo<K, double>(vv, m.dx, c);
     double eta = 0, wr = 0;
 #pragma unroll
     for (int rr = 0; rr < NQ; ++rr)
@@ -570,8 +541,8 @@ __global__ __launch_bounds__(256) void k_block_inv(MeshDev m, const double* __re
         eta = tb.gq[rr];
         wr = tb.gw[rr];
       }
-    const double xs = g.a0 + eta * g.a1, zs = g.b0 + eta * g.b1;
-    const double jxw = (xs * g.ze - g.xe * zs) * g.wc * wr;
+    (void)eta;
+    const double jxw = g.dx * g.ze * g.wc * wr;
     double rv = 0;
     for (int j = 0; j < ND; ++j)
       for (int i = 0; i < ND; ++i) rv += s_S[c * ND + i] * s_S[r * ND + j] * rho[(long)cell * ND2 + j * ND + i];
@@ -603,30 +574,47 @@ __global__ __launch_bounds__(256) void k_block_inv(MeshDev m, const double* __re
     }
     wave_sync();
   }
-  for (int e = lane; e < ND2 * ND2; e += 64) Pinv[(long)cell * ND2 * ND2 + e] = (float)s_A[w][e];
+  for (int e = lane; e < ND2 * ND2; e += 64) Ainv[(long)cell * ND2 * ND2 + e] = s_A[w][e];
 }
 
 template <int K>
-void build_block_inverse(const MeshDev& m, const double* rho, float* Pinv, hipStream_t s) {
-  hipLaunchKernelGGL((k_block_inv<K>), dim3(ceil_div(m.ncell, 4)), dim3(256), 0, s, m, rho, Pinv);
+void build_block_inverse(const MeshDev& m, const double* rho, double* Ainv, hipStream_t s) {
+  hipLaunchKernelGGL((k_block_inv<K>), dim3(ceil_div(m.ncell, 4)), dim3(256), 0, s, m, rho, Ainv);
   ATMO_HIP_CHECK(hipGetLastError());
 }
 
 // ------------------------------------------------------------ GMRES
-__global__ void k_mdot_part(long n, const double* __restrict__ w, const double* __restrict__ V, long ldv, int nv,
-                            double* part) {
-  double acc[8];
-  for (int g0 = 0; g0 < nv; g0 += 8) {
-    const int ng = (nv - g0) < 8 ? (nv - g0) : 8;
+// Partial dot products of w with nv columns of V, MDOT_GROUP columns per
+// pass over w. PAIRS: two consecutive entries per thread and step (16-byte
+// loads; needs n and ldv even, i.e. every column 16-byte aligned).
+constexpr int MDOT_GROUP = 16;
+template <bool PAIRS>
+__global__ __launch_bounds__(RED_THREADS) void k_mdot_part(long n, const double* __restrict__ w,
+                                                            const double* __restrict__ V, long ldv, int nv,
+                                                            double* part) {
+  double acc[MDOT_GROUP];
+  const long nstep = PAIRS ? n / 2 : n;
+  for (int g0 = 0; g0 < nv; g0 += MDOT_GROUP) {
+    const int ng = (nv - g0) < MDOT_GROUP ? (nv - g0) : MDOT_GROUP;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) acc[k] = 0.0;
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-      const double wi = w[i];
+    for (int k = 0; k < MDOT_GROUP; ++k) acc[k] = 0.0;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nstep; i += (long)gridDim.x * blockDim.x) {
+      if (PAIRS) {
+        const double2 wi = reinterpret_cast<const double2*>(w)[i];
 #pragma unroll
-      for (int k = 0; k < 8; ++k)
-        if (k < ng) acc[k] += wi * V[(long)(g0 + k) * ldv + i];
+        for (int k = 0; k < MDOT_GROUP; ++k)
+          if (k < ng) {
+            const double2 v = reinterpret_cast<const double2*>(V + (long)(g0 + k) * ldv)[i];
+            acc[k] += wi.x * v.x + wi.y * v.y;
+          }
+      } else {
+        const double wi = w[i];
+#pragma unroll
+        for (int k = 0; k < MDOT_GROUP; ++k)
+          if (k < ng) acc[k] += wi * V[(long)(g0 + k) * ldv + i];
+      }
     }
-    block_reduce<8, false>(acc);
+    block_reduce<MDOT_GROUP, false>(acc);
     if (threadIdx.x == 0)
       for (int k = 0; k < ng; ++k) part[(long)(g0 + k) * RED_MAX_BLOCKS + blockIdx.x] = acc[k];
   }
@@ -635,7 +623,11 @@ __global__ void k_mdot_part(long n, const double* __restrict__ w, const double* 
 void gm_mdot(long n, const double* w, const double* V, long ldv, int nv, double* out, RedBuf& rb, hipStream_t s) {
   if (nv > RED_MAX_SLOTS) throw std::invalid_argument("gm_mdot: too many vectors");
   const int G = red_blocks(n);
-  hipLaunchKernelGGL(k_mdot_part, dim3(G), dim3(RED_THREADS), 0, s, n, w, V, ldv, nv, rb.gen);
+  const bool pairs = ((n | ldv) & 1) == 0 && ((reinterpret_cast<uintptr_t>(w) | reinterpret_cast<uintptr_t>(V)) & 15) == 0;
+  if (pairs)
+    hipLaunchKernelGGL(k_mdot_part<true>, dim3(G), dim3(RED_THREADS), 0, s, n, w, V, ldv, nv, rb.gen);
+  else
+    hipLaunchKernelGGL(k_mdot_part<false>, dim3(G), dim3(RED_THREADS), 0, s, n, w, V, ldv, nv, rb.gen);
   ATMO_HIP_CHECK(hipGetLastError());
   fin(rb.gen, G, nv, FIN_MDOT, out, nullptr, s);
 }
@@ -649,9 +641,59 @@ __global__ void k_maxpy(long n, double* __restrict__ w, const double* __restrict
   }
 }
 
-void gm_maxpy(long n, double* w, const double* V, long ldv, int nv, const double* coef, hipStream_t s) {
-  hipLaunchKernelGGL(k_maxpy, dim3(ew_blocks(n)), dim3(256), 0, s, n, w, V, ldv, nv, coef, -1.0);
+// w -= sum_k coef[k] V_k and partial |w|^2 of the result in the same pass.
+// PAIRS: two consecutive entries per thread and step (16-byte accesses);
+// four independent partial sums over k keep several loads in flight.
+template <bool PAIRS>
+__global__ __launch_bounds__(RED_THREADS) void k_maxpy_norm(long n, double* __restrict__ w,
+                                                             const double* __restrict__ V, long ldv, int nv,
+                                                             const double* __restrict__ coef, double* part) {
+  constexpr int E = PAIRS ? 2 : 1;
+  double acc[1] = {0.0};
+  const long nstep = n / E;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nstep; i += (long)gridDim.x * blockDim.x) {
+    double a[4][E] = {};
+    int k = 0;
+    for (; k + 4 <= nv; k += 4) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const double* col = V + (long)(k + u) * ldv + i * E;
+        const double c = coef[k + u];
+        if (PAIRS) {
+          const double2 v = *reinterpret_cast<const double2*>(col);
+          a[u][0] += c * v.x;
+          a[u][E - 1] += c * v.y;
+        } else {
+          a[u][0] += c * col[0];
+        }
+      }
+    }
+    for (; k < nv; ++k) {
+      const double* col = V + (long)k * ldv + i * E;
+#pragma unroll
+      for (int e = 0; e < E; ++e) a[0][e] += coef[k] * col[e];
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const double r = w[i * E + e] - ((a[0][e] + a[1][e]) + (a[2][e] + a[3][e]));
+      w[i * E + e] = r;
+      acc[0] += r * r;
+    }
+  }
+  block_reduce<1, false>(acc);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+}
+
+void gm_maxpy_norm(long n, double* w, const double* V, long ldv, int nv, const double* coef, double* norm, RedBuf& rb,
+                   hipStream_t s) {
+  const int G = red_blocks(n);
+  const bool pairs = ((n | ldv) & 1) == 0 && ((reinterpret_cast<uintptr_t>(w) | reinterpret_cast<uintptr_t>(V)) & 15) == 0;
+  if (pairs)
+    hipLaunchKernelGGL(k_maxpy_norm<true>, dim3(G), dim3(RED_THREADS), 0, s, n, w, V, ldv, nv, coef, rb.gen);
+  else
+    hipLaunchKernelGGL(k_maxpy_norm<false>, dim3(G), dim3(RED_THREADS), 0, s, n, w, V, ldv, nv, coef, rb.gen);
   ATMO_HIP_CHECK(hipGetLastError());
+  fin(rb.gen, G, 1, FIN_SQRT, norm, nullptr, s);
 }
 
 void gm_update(long n, double* x, const double* V, long ldv, int nv, const double* y, hipStream_t s) {
@@ -660,9 +702,9 @@ void gm_update(long n, double* x, const double* V, long ldv, int nv, const doubl
 }
 
 #define ATMO_KRY_INST(K)                                                                                          \
-  template void jcg_apply<K>(int, const MeshDev&, const double*, const double*, const double*, const double*,      \
-                             double*, double*, CGState*, RedBuf&, long, hipStream_t);                             \
-  template void build_block_inverse<K>(const MeshDev&, const double*, float*, hipStream_t);
+  template void jcg_apply<K>(int, const MeshDev&, const double*, const double*, const double*, double*, double*,  \
+                             CGState*, RedBuf&, long, hipStream_t);                                               \
+  template void build_block_inverse<K>(const MeshDev&, const double*, double*, hipStream_t);
 
 ATMO_KRY_INST(1)
 ATMO_KRY_INST(2)

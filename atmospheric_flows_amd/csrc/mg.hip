@@ -199,12 +199,15 @@ __device__ void coarse_apply(const MeshDev& m, const double* rho, double coeff, 
     __syncthreads();
     wave_mass_core<K, double, NC, W>(m, cell0, s_u, s_r, coeff, s_red);
     __syncthreads();
+    double Si[C::NQ];
+    lane_cols_S<K, double>(threadIdx.x % C::ND, Si);
     for (int t = threadIdx.x; t < CPW * ND2; t += 64) {
       const long gi = base + t;
       if (gi >= lim) break;
+      if (64 % C::ND != 0) lane_cols_S<K, double>(t % C::ND, Si);
 #pragma unroll
       for (int comp = 0; comp < NC; ++comp)
-        dst[comp * Ns + gi] = wave_mass_out<K, double>(s_red + comp * CPW * C::ND * C::NQ, t);
+        dst[comp * Ns + gi] = wave_mass_out<K, double>(s_red + comp * CPW * C::ND * C::NQ, t, Si);
     }
     __syncthreads();
   }

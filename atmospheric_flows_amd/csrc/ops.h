@@ -36,6 +36,11 @@ void launch_B(const MeshDev& m, const OpConsts& oc, double coeff, const double* 
 template <int K>
 void launch_C(const MeshDev& m, const OpConsts& oc, double coeff, const double* pfix, const double* u,
               double* dst_p, long Ns, hipStream_t s);
+// Schur complement without its inner solve: dst = scale * (D p - C u)
+// (scale: optional per-dof factor, e.g. the Jacobi preconditioner of GMRES)
+template <int K>
+void launch_d_minus_c(const MeshDev& m, const OpConsts& oc, double coeff, const double* pfix, const double* u,
+                      const double* p, const double* scale, double* dst, long Ns, hipStream_t s);
 // explicit residuals; src follows the reference's positional conventions
 template <int K>
 void launch_rhs_rho(const MeshDev& m, const OpConsts& oc, const Tableau& tb, int stage, const double* const* src,

@@ -11,8 +11,8 @@
 //    under exchanging the sides (n -> -n), so this equals the reference's
 //    face-loop contribution to the cell without any atomics; every face flux
 //    is computed twice (once per side).
+#include "cellface.h"
 #include "ops.h"
-#include "wave.h"
 
 namespace atmo {
 
@@ -51,12 +51,15 @@ __global__ __launch_bounds__(64) void k_mass(MeshDev m, const T* __restrict__ sr
   __syncthreads();
   const long base = (long)blockIdx.x * CPW * ND2;
   const long lim = (long)m.ncell * ND2;
+  T Si[C::NQ];
+  lane_cols_S<K, T>(threadIdx.x % C::ND, Si);
   for (int t = threadIdx.x; t < CPW * ND2; t += 64) {
     const long gi = base + t;
     if (gi >= lim) break;
+    if (64 % C::ND != 0) lane_cols_S<K, T>(t % C::ND, Si);
 #pragma unroll
     for (int comp = 0; comp < NC; ++comp) {
-      T acc = wave_mass_out<K, T>(&s_red[comp][0][0][0], t);
+      T acc = wave_mass_out<K, T>(&s_red[comp][0][0][0], t, Si);
       if (add) acc += add_scale * add[comp * Ns + gi];
       dst[comp * Ns + gi] = acc;
     }
@@ -77,16 +80,13 @@ __global__ __launch_bounds__(64) void k_inv_diag(MeshDev m, int ncomp, const T* 
   if (L.active) {
     const int ci = L.cell % m.nx, cj = L.cell / m.nx;
     const CellVerts v = cell_verts(m, ci, cj);
-    const ColGeo<T> g = col_geo<K, T>(v, L.c);
+    const ColGeo<T> g = col_geo<K, T>(v, m.dx, L.c);
     T Sc[ND], Dc[ND];
     lane_rows<K, T>(L.c, Sc, Dc);
     T wq[NQ];
+    const T jc = g.dx * g.ze * g.wc * coeff;
 #pragma unroll
-    for (int r = 0; r < NQ; ++r) {
-      const T eta = (T)tb.gq[r];
-      const T xs = g.a0 + eta * g.a1, zs = g.b0 + eta * g.b1;
-      wq[r] = (xs * g.ze - g.xe * zs) * g.wc * (T)tb.gw[r] * coeff;
-    }
+    for (int r = 0; r < NQ; ++r) wq[r] = jc * (T)tb.gw[r];
     if (W) {
       T tv[ND];
 #pragma unroll
@@ -119,8 +119,8 @@ __global__ __launch_bounds__(64) void k_inv_diag(MeshDev m, int ncomp, const T* 
     const long gi = base + t;
     if (gi >= lim) break;
     const int cw = t / ND2, o = t - cw * ND2, jj = o / ND, ii = o - jj * ND;
-    T Si[NQ], Di[NQ];
-    lane_cols<K, T>(ii, Si, Di);
+    T Si[NQ];
+    lane_cols_S<K, T>(ii, Si);
     T acc = 0;
 #pragma unroll
     for (int c = 0; c < NQ; ++c) acc += Si[c] * Si[c] * s_red[cw][jj][c];
@@ -161,62 +161,13 @@ void launch_inv_diag(const MeshDev& m, int ncomp, bool weighted, const T* rho, T
 }
 
 // ------------------------------------------------------- cell+face family
-template <int NF>
-struct FieldPtrs {
-  const double* p[NF];
-};
-
-// B: euler_operator.h:1510-1617 (cell: submit_divergence, face: {p}[v.n],
-// wall: p_D = p)
-struct PhysB {
-  static constexpr int NF = 1, NO = 2;
-  static constexpr bool BDRY = true;
-  double cf;  // coeff*dt/Ma^2
-  __device__ void cell(const double (&v)[NF], double (&V)[NO], double (&Gx)[NO], double (&Gz)[NO]) const {
-    V[0] = V[1] = 0;
-    Gx[0] = -cf * v[0];
-    Gz[0] = 0;
-    Gx[1] = 0;
-    Gz[1] = -cf * v[0];
-  }
-  __device__ void face(const double (&a)[NF], const double (&b)[NF], double n0, double n1, double (&F)[NO]) const {
-    const double avg = 0.5 * (a[0] + b[0]);
-    F[0] = cf * avg * n0;
-    F[1] = cf * avg * n1;
-  }
-  __device__ void bdry(const double (&a)[NF], double n0, double n1, double (&F)[NO]) const {
-    const double pD = a[0];
-    F[0] = cf * 0.5 * (a[0] + pD) * n0;
-    F[1] = cf * 0.5 * (a[0] + pD) * n1;
-  }
-};
-
-// C: euler_operator.h:1411-1507 (nonlinear in u through lambda)
-struct PhysC {
-  static constexpr int NF = 3, NO = 1;  // pfix, ux, uz
-  static constexpr bool BDRY = false;
-  double cdt, gm;
-  __device__ void cell(const double (&v)[NF], double (&V)[NO], double (&Gx)[NO], double (&Gz)[NO]) const {
-    const double s = -cdt * gm / (gm - 1.0) * v[0];
-    V[0] = 0;
-    Gx[0] = s * v[1];
-    Gz[0] = s * v[2];
-  }
-  __device__ void face(const double (&a)[NF], const double (&b)[NF], double n0, double n1, double (&F)[NO]) const {
-    const double ax = 0.5 * gm / (gm - 1.0) * (a[0] * a[1] + b[0] * b[1]);
-    const double az = 0.5 * gm / (gm - 1.0) * (a[0] * a[2] + b[0] * b[2]);
-    const double lam = fmax(fabs(a[1] * n0 + a[2] * n1), fabs(b[1] * n0 + b[2] * n1));
-    const double jump = 1.0 / (gm - 1.0) * (a[0] - b[0]);
-    F[0] = cdt * (ax * n0 + az * n1 + 0.5 * lam * jump);
-  }
-  __device__ void bdry(const double (&)[NF], double, double, double (&F)[NO]) const { F[0] = 0; }
-};
-
 // density residual: euler_operator.h:344-692
 template <int ST>
 struct PhysRhsRho {
   static constexpr int NF = 3 * ST, NO = 1;  // (rho, ux, uz) per stage
   static constexpr bool BDRY = false;
+  static constexpr bool HAS_V = true;
+  static constexpr unsigned GX = (1u << NO) - 1, GZ = (1u << NO) - 1;
   double wdt[3];
   __device__ void cell(const double (&v)[NF], double (&V)[NO], double (&Gx)[NO], double (&Gz)[NO]) const {
     double gx = 0, gz = 0;
@@ -249,6 +200,8 @@ template <int ST>
 struct PhysRhsP {
   static constexpr int NF = 4 * ST + 3, NO = 1;  // (rho,ux,uz,p) per stage, then rho_k, uk_x, uk_z
   static constexpr bool BDRY = false;
+  static constexpr bool HAS_V = true;
+  static constexpr unsigned GX = (1u << NO) - 1, GZ = (1u << NO) - 1;
   double wa[3], wt[3], gwt[3];  // advective, enthalpy and gravity weights
   double dt, Ma2, gm, iFr2, G, e0, e1;
   __device__ void cell(const double (&v)[NF], double (&V)[NO], double (&Gx)[NO], double (&Gz)[NO]) const {
@@ -323,6 +276,8 @@ struct PhysRhsU {
   static constexpr int NX = ST < 3 ? 1 : 0;  // extra gravity density (rho_2 resp. rho_3)
   static constexpr int NF = 4 * ST + NX, NO = 2;
   static constexpr bool BDRY = true;
+  static constexpr bool HAS_V = true;
+  static constexpr unsigned GX = (1u << NO) - 1, GZ = (1u << NO) - 1;
   double wa[3], wt[3], gwt[3];
   double dt, Ma2, iFr2, G, e0, e1;
   __device__ void cell(const double (&v)[NF], double (&V)[NO], double (&Gx)[NO], double (&Gz)[NO]) const {
@@ -389,166 +344,21 @@ struct PhysRhsU {
 
 template <int K, class Ph>
 __global__ __launch_bounds__(64) void k_cellface(MeshDev m, Ph ph, FieldPtrs<Ph::NF> in, double* __restrict__ dst,
-                                                 long Ns) {
-  using C = Cfg<K>;
-  constexpr int ND = C::ND, NQ = C::NQ, ND2 = C::ND2, CPW = C::CPW;
-  constexpr int NF = Ph::NF, NO = Ph::NO;
-  constexpr auto& tb = basis::kT<K>;
-  __shared__ double s_in[NF][CPW * ND2];
-  __shared__ double s_T[NO][2][CPW][ND][NQ];
-  __shared__ double s_F[NO][2][CPW][NQ];
-  LaneMap<K> L(m.ncell);
-#pragma unroll
-  for (int f = 0; f < NF; ++f) stage_cells<K>(s_in[f], in.p[f], m.ncell);
-  __syncthreads();
-  if (L.active) {
-    const int ci = L.cell % m.nx, cj = L.cell / m.nx;
-    const CellVerts v = cell_verts(m, ci, cj);
-    const ColGeo<double> g = col_geo<K, double>(v, L.c);
-    double Sc[ND], Dc[ND];
-    lane_rows<K, double>(L.c, Sc, Dc);
-    const int off = L.cw * ND2;
-    double tv[NF][ND];
-#pragma unroll
-    for (int f = 0; f < NF; ++f)
-#pragma unroll
-      for (int j = 0; j < ND; ++j) {
-        double s = 0;
-#pragma unroll
-        for (int i = 0; i < ND; ++i) s += Sc[i] * s_in[f][off + j * ND + i];
-        tv[f][j] = s;
-      }
-    double Tv[NO][ND], Td[NO][ND];
-#pragma unroll
-    for (int o = 0; o < NO; ++o)
-#pragma unroll
-      for (int j = 0; j < ND; ++j) Tv[o][j] = Td[o][j] = 0;
-#pragma unroll
-    for (int r = 0; r < NQ; ++r) {
-      double val[NF];
-#pragma unroll
-      for (int f = 0; f < NF; ++f) {
-        double s = 0;
-#pragma unroll
-        for (int j = 0; j < ND; ++j) s += tb.S[r][j] * tv[f][j];
-        val[f] = s;
-      }
-      double V[NO], Gx[NO], Gz[NO];
-      ph.cell(val, V, Gx, Gz);
-      const double eta = tb.gq[r];
-      const double xs = g.a0 + eta * g.a1, zs = g.b0 + eta * g.b1;
-      const double w = g.wc * tb.gw[r];
-      const double JxW = (xs * g.ze - g.xe * zs) * w;
-#pragma unroll
-      for (int o = 0; o < NO; ++o) {
-        const double Vq = V[o] * JxW;
-        const double Gxi = w * (g.ze * Gx[o] - g.xe * Gz[o]);
-        const double Geta = w * (-zs * Gx[o] + xs * Gz[o]);
-#pragma unroll
-        for (int j = 0; j < ND; ++j) {
-          Tv[o][j] += tb.S[r][j] * Vq + tb.D[r][j] * Geta;
-          Td[o][j] += tb.S[r][j] * Gxi;
-        }
-      }
-    }
-    // bottom (j-1) and top (j+1) faces: this lane is face point c
-#pragma unroll
-    for (int side = 0; side < 2; ++side) {
-      const int f = 2 + side;
-      const FaceGeo fg = face_geo(v, f);
-      const double wl = fg.len * g.wc;
-      double own[NF], nb[NF], F[NO];
-#pragma unroll
-      for (int q = 0; q < NF; ++q) own[q] = tv[q][side ? K : 0];
-      const int nj = side ? cj + 1 : cj - 1;
-      if (nj >= 0 && nj < m.nz) {
-        const long nbase = (long)(nj * m.nx + ci) * ND2 + (side ? 0 : K * ND);
-#pragma unroll
-        for (int q = 0; q < NF; ++q) {
-          double s = 0;
-#pragma unroll
-          for (int i = 0; i < ND; ++i) s += Sc[i] * in.p[q][nbase + i];
-          nb[q] = s;
-        }
-        ph.face(own, nb, fg.n0, fg.n1, F);
-      } else if (Ph::BDRY) {
-        ph.bdry(own, fg.n0, fg.n1, F);
-      } else {
-#pragma unroll
-        for (int o = 0; o < NO; ++o) F[o] = 0;
-      }
-#pragma unroll
-      for (int o = 0; o < NO; ++o) Tv[o][side ? K : 0] += F[o] * wl;
-    }
-    // left (i-1) and right (i+1) faces: this lane is face point s = c
-#pragma unroll
-    for (int side = 0; side < 2; ++side) {
-      const int f = side;
-      const FaceGeo fg = face_geo(v, f);
-      const double wl = fg.len * g.wc;
-      double own[NF], nb[NF], F[NO];
-      const int col = side ? K : 0;
-#pragma unroll
-      for (int q = 0; q < NF; ++q) {
-        double s = 0;
-#pragma unroll
-        for (int j = 0; j < ND; ++j) s += Sc[j] * s_in[q][off + j * ND + col];
-        own[q] = s;
-      }
-      int ni = side ? ci + 1 : ci - 1;
-      bool inner = true;
-      if (ni < 0 || ni >= m.nx) {
-        if (m.periodic) ni = (ni + m.nx) % m.nx;
-        else inner = false;
-      }
-      if (inner) {
-        const long nbase = (long)(cj * m.nx + ni) * ND2 + (side ? 0 : K);
-#pragma unroll
-        for (int q = 0; q < NF; ++q) {
-          double s = 0;
-#pragma unroll
-          for (int j = 0; j < ND; ++j) s += Sc[j] * in.p[q][nbase + j * ND];
-          nb[q] = s;
-        }
-        ph.face(own, nb, fg.n0, fg.n1, F);
-      } else if (Ph::BDRY) {
-        ph.bdry(own, fg.n0, fg.n1, F);
-      } else {
-#pragma unroll
-        for (int o = 0; o < NO; ++o) F[o] = 0;
-      }
-#pragma unroll
-      for (int o = 0; o < NO; ++o) s_F[o][side][L.cw][L.c] = F[o] * wl;
-    }
-#pragma unroll
-    for (int o = 0; o < NO; ++o)
-#pragma unroll
-      for (int j = 0; j < ND; ++j) {
-        s_T[o][0][L.cw][j][L.c] = Tv[o][j];
-        s_T[o][1][L.cw][j][L.c] = Td[o][j];
-      }
-  }
-  __syncthreads();
+                                                 const double* __restrict__ scale, long Ns) {
+  constexpr int ND2 = Cfg<K>::ND2, CPW = Cfg<K>::CPW;
+  __shared__ CFSmem<K, Ph> sh;
+  cellface_columns<K, Ph>(m, ph, in, sh);
   const long base = (long)blockIdx.x * CPW * ND2;
   const long lim = (long)m.ncell * ND2;
+  OutRows<K> R = out_rows<K>(threadIdx.x);
   for (int t = threadIdx.x; t < CPW * ND2; t += 64) {
     const long gi = base + t;
     if (gi >= lim) break;
-    const int cw = t / ND2, o2 = t - cw * ND2, jj = o2 / ND, ii = o2 - jj * ND;
-    double Si[NQ], Di[NQ], Sj[NQ], Dj[NQ];
-    lane_cols<K, double>(ii, Si, Di);
-    lane_cols<K, double>(jj, Sj, Dj);
+    if (!rows_invariant<K>()) R = out_rows<K>(t);
 #pragma unroll
-    for (int o = 0; o < NO; ++o) {
-      double acc = 0;
-#pragma unroll
-      for (int c = 0; c < NQ; ++c) acc += Si[c] * s_T[o][0][cw][jj][c] + Di[c] * s_T[o][1][cw][jj][c];
-      if (ii == 0)
-#pragma unroll
-        for (int c = 0; c < NQ; ++c) acc += Sj[c] * s_F[o][0][cw][c];
-      if (ii == K)
-#pragma unroll
-        for (int c = 0; c < NQ; ++c) acc += Sj[c] * s_F[o][1][cw][c];
+    for (int o = 0; o < Ph::NO; ++o) {
+      double acc = cellface_out<K, Ph>(sh, o, t, R);
+      if (scale) acc *= scale[o * Ns + gi];
       dst[o * Ns + gi] = acc;
     }
   }
@@ -556,10 +366,10 @@ __global__ __launch_bounds__(64) void k_cellface(MeshDev m, Ph ph, FieldPtrs<Ph:
 
 template <int K, class Ph>
 static void launch_cf(const MeshDev& m, const Ph& ph, const FieldPtrs<Ph::NF>& in, double* dst, long Ns,
-                      hipStream_t s) {
+                      hipStream_t s, const double* scale = nullptr) {
   const int grid = ceil_div(m.ncell, Cfg<K>::CPW);
   if (grid == 0) return;
-  hipLaunchKernelGGL((k_cellface<K, Ph>), dim3(grid), dim3(64), 0, s, m, ph, in, dst, Ns);
+  hipLaunchKernelGGL((k_cellface<K, Ph>), dim3(grid), dim3(64), 0, s, m, ph, in, dst, scale, Ns);
   ATMO_HIP_CHECK(hipGetLastError());
 }
 
@@ -580,6 +390,17 @@ void launch_C(const MeshDev& m, const OpConsts& oc, double coeff, const double* 
   ph.gm = oc.gamma_gas;
   FieldPtrs<3> in{{pfix, u, u + Ns}};
   launch_cf<K>(m, ph, in, dst_p, Ns, s);
+}
+
+template <int K>
+void launch_d_minus_c(const MeshDev& m, const OpConsts& oc, double coeff, const double* pfix, const double* u,
+                      const double* p, const double* scale, double* dst, long Ns, hipStream_t s) {
+  PhysDminusC ph;
+  ph.c.cdt = coeff * oc.dt;
+  ph.c.gm = oc.gamma_gas;
+  ph.cD = 1.0 / (oc.gamma_gas - 1.0);
+  FieldPtrs<4> in{{pfix, u, u + Ns, p}};
+  launch_cf<K>(m, ph, in, dst, Ns, s, scale);
 }
 
 template <int K>
@@ -699,6 +520,8 @@ void launch_rhs_velocity(const MeshDev& m, const OpConsts& oc, const Tableau& tb
   template void launch_B<K>(const MeshDev&, const OpConsts&, double, const double*, double*, long, hipStream_t);   \
   template void launch_C<K>(const MeshDev&, const OpConsts&, double, const double*, const double*, double*, long, \
                             hipStream_t);                                                                       \
+  template void launch_d_minus_c<K>(const MeshDev&, const OpConsts&, double, const double*, const double*,         \
+                                    const double*, const double*, double*, long, hipStream_t);                   \
   template void launch_rhs_rho<K>(const MeshDev&, const OpConsts&, const Tableau&, int, const double* const*,     \
                                   double*, long, hipStream_t);                                                  \
   template void launch_rhs_pressure<K>(const MeshDev&, const OpConsts&, const Tableau&, int,                     \

@@ -5,24 +5,20 @@
 
 namespace atmo {
 
-// Per-lane table columns for a lane-dependent dof index m: Si[c] = S[c][m].
+// Per-lane table columns for a lane-dependent dof index m: Si[c] = S[c][m],
+// Di[c] = D[c][m] (constant-memory loads of the transposed tables).
 template <int K, typename T>
 __device__ inline void lane_cols(int m, T (&Si)[Cfg<K>::NQ], T (&Di)[Cfg<K>::NQ]) {
-  constexpr auto& tb = basis::kT<K>;
 #pragma unroll
   for (int c = 0; c < Cfg<K>::NQ; ++c) {
-    Si[c] = 0;
-    Di[c] = 0;
+    Si[c] = (T)cT<K>.St[m][c];
+    Di[c] = (T)cT<K>.Dt[m][c];
   }
+}
+template <int K, typename T>
+__device__ inline void lane_cols_S(int m, T (&Si)[Cfg<K>::NQ]) {
 #pragma unroll
-  for (int mm = 0; mm < Cfg<K>::ND; ++mm)
-    if (mm == m) {
-#pragma unroll
-      for (int c = 0; c < Cfg<K>::NQ; ++c) {
-        Si[c] = (T)tb.S[c][mm];
-        Di[c] = (T)tb.D[c][mm];
-      }
-    }
+  for (int c = 0; c < Cfg<K>::NQ; ++c) Si[c] = (T)cT<K>.St[m][c];
 }
 
 // Quadrature weights of the lane's column: wq[r] = JxW(c, r) * coeff, times
@@ -34,13 +30,10 @@ __device__ inline void column_weights(const MeshDev& m, int cell, int c, int cw,
   constexpr auto& tb = basis::kT<K>;
   const int ci = cell % m.nx, cj = cell / m.nx;
   const CellVerts v = cell_verts(m, ci, cj);
-  const ColGeo<T> g = col_geo<K, T>(v, c);
+  const ColGeo<T> g = col_geo<K, T>(v, m.dx, c);
+  const T jc = g.dx * g.ze * g.wc * coeff;
 #pragma unroll
-  for (int r = 0; r < NQ; ++r) {
-    const T eta = (T)tb.gq[r];
-    const T xs = g.a0 + eta * g.a1, zs = g.b0 + eta * g.b1;
-    wq[r] = (xs * g.ze - g.xe * zs) * g.wc * (T)tb.gw[r] * coeff;
-  }
+  for (int r = 0; r < NQ; ++r) wq[r] = jc * (T)tb.gw[r];
   if (W) {
     T tv[ND];
 #pragma unroll
@@ -105,13 +98,12 @@ __device__ inline void wave_mass_core(const MeshDev& m, int cell0, const T* s_u,
   }
 }
 
-// Output phase: value of dof t (= cw*ND2 + jj*ND + ii) of one component.
+// Output phase: value of dof t (= cw*ND2 + jj*ND + ii) of one component;
+// Si = S[.][ii] (lane_cols_S of the dof's x node).
 template <int K, typename T>
-__device__ inline T wave_mass_out(const T* s_red_comp, int t) {
+__device__ inline T wave_mass_out(const T* s_red_comp, int t, const T (&Si)[Cfg<K>::NQ]) {
   constexpr int ND = Cfg<K>::ND, NQ = Cfg<K>::NQ, ND2 = Cfg<K>::ND2;
-  const int cw = t / ND2, o = t - cw * ND2, jj = o / ND, ii = o - jj * ND;
-  T Si[NQ], Di[NQ];
-  lane_cols<K, T>(ii, Si, Di);
+  const int cw = t / ND2, o = t - cw * ND2, jj = o / ND;
   const T* red = s_red_comp + cw * ND * NQ + jj * NQ;
   T acc = 0;
 #pragma unroll

@@ -76,7 +76,8 @@ typedef struct {
   double cheb_range;           /* Atmospheric_Flow.cc:491 (15) */
   int cheb_eig_its;            /* Atmospheric_Flow.cc:493 (10) */
   int gmres_restart;           /* deal.II SolverGMRES default (28) */
-  int precision;               /* 0: FP64 ; 1: FP32 multigrid inside FP64 Krylov */
+  int precision;               /* 0: FP64, the reference's solvers; 1: mixed form: FP32 multigrid inside
+                                  FP64 Krylov, FP64 cellwise-direct A^{-1} (inner Schur and A solves) */
 } atmo_params;
 
 typedef struct {
@@ -181,7 +182,9 @@ atmo_status atmo_init_hill_state(const atmo_mesh_desc* mesh, double theta0, doub
  * HIP events recorded on that stream. which: 0 A apply (k_mass, weighted),
  * 1 fused inner-CG apply (k_jcg_apply), 2 M_rho apply, 3 B, 4 C,
  * 5 stage-3 pressure residual, 6 fused inner-CG update (k_jcg_upd),
- * 7 plain device-to-device copy of one velocity field (calibration). Also reports the kernel's algorithmic HBM
+ * 7 plain device-to-device copy of one velocity field (calibration),
+ * 8 B with the cellwise A^{-1} (k_b_ainv, mixed form only), 9 scaled D - C
+ * (k_cellface<PhysDminusC>). Also reports the kernel's algorithmic HBM
  * bytes per launch. */
 atmo_status atmo_time_kernel(atmo_ctx* ctx, int which, int nrep, double* ms_per_launch, double* bytes_per_launch);
 

@@ -14,16 +14,16 @@ def test_footprint_scales_with_the_mesh_and_precision():
     # ~ 100-130 FP64 fields of N_s on the fine level plus the multigrid levels
     assert 90 * 8 * ns1 < b1 < 140 * 8 * ns1
     assert abs(b2 / b1 - 4.0) < 0.1  # fixed-size reduction and record buffers aside
-    # mixed precision adds exactly the FP32 cell-block inverses: (k+1)^4 floats per cell
+    # the mixed form adds exactly the FP64 inverses of A's cell blocks: (k+1)^4 doubles per cell
     bm = af.device_bytes(3, 128, 64, precision=1)
-    assert bm - b2 == 128 * 64 * 256 * 4
+    assert bm - b2 == 128 * 64 * 256 * 8
     # a longer GMRES basis adds exactly its vectors
     assert af.device_bytes(3, 128, 64, gmres_restart=30) - b2 == 2 * ns2 * 8
 
 
 def test_footprint_of_the_headline_mesh_fits_hbm_many_times():
     b = af.device_bytes(3, 512, 256, precision=1)
-    assert 1.0e9 < b < 3.0e9  # ~ 2 GB: about 100 such members fit in 256 GB
+    assert 1.0e9 < b < 3.0e9  # ~ 2 GB: about 100 such members fit in 256 GB of HBM
 
 
 def test_footprint_rejects_bad_meshes():

@@ -182,50 +182,23 @@ def test_mixed_precision_matches_fp64(k, nx, nz, dt):
         assert e < 1e-8, name
 
 
-def test_mixed_inner_solve_parity():
-    """Schur vmult with the FP32 block-inverse preconditioned inner CG
-    against the oracle (Jacobi inner CG), both at a tight inner tolerance."""
+def test_mixed_schur_vmult_parity():
+    """Schur vmult of the mixed form (B and the cellwise-direct FP64 A^{-1}
+    in one pass, then D - C) against the oracle's reference form (Jacobi
+    inner CG at a tight tolerance): no inner CG iterations, same product."""
     k, nx, nz = 3, 8, 8
     dev, orc = make_pair(k, nx, nz, schur_inner_tol=1e-14, precision=1)
     op = af.EULEROperator(dev)
     (rho, u, p), (rho2, u2, p2) = fields(dev, 11, 2)
-    op.set_HYPERBOLIC_stage(1)
-    orc.set_stage(1, 2)
-    op.set_rho_for_fixed(dev.vector(af.BLOCK_DENSITY, rho2))
-    orc.set_rho_for_fixed(rho2)
-    op.set_pres_fixed(dev.vector(af.BLOCK_PRESSURE, p2))
-    orc.set_pres_fixed(p2)
-    op.set_NS_stage(2)
-    out = dev.vector(af.BLOCK_PRESSURE)
-    op.vmult(out, dev.vector(af.BLOCK_PRESSURE, p))
-    assert rel(out.download(), orc.vmult(p, 2)) < 1e-11
-    st = dev.stats()
-    assert st["inner_cg_its"] <= 4, st
-
-
-@pytest.mark.parametrize("k,nx,nz", [(2, 16, 8), (3, 8, 4)])
-def test_step_diagnostics(k, nx, nz):
-    """Device per-step diagnostics against the oracle's on the same state:
-    after one step (stage-3 density slots and flow slots) and on the
-    initial state (of_state)."""
-    dev, orc = make_pair(k, nx, nz, dt=4.0)
-    rho, u, p = dev.init_hill()
-    dev.upload_state(rho, u, p)
-    d0 = dev.step_diagnostics(of_state=True)
-    r0 = orc.diagnostics(rho, u, p)
-    dev.imex_step(1)
-    d1 = dev.step_diagnostics()
-    r1 = orc.diagnostics(*dev.download_state())
-    for d, r in ((d0, r0), (d1, r1)):
-        assert d["min_density"][2] == pytest.approx(r[0], rel=1e-14)
-        assert d["max_density"][2] == pytest.approx(r[1], rel=1e-15)
-        assert d["max_celerity"] == pytest.approx(r[2], rel=1e-14)
-        assert d["max_velocity"] == pytest.approx(r[3], rel=1e-14)
-        assert d["nonfinite"] == 0 and r[4] == 0
-    assert d1["min_cell_diameter"] == pytest.approx(orc.min_cell_diameter(), rel=1e-15)
-    f = 4.0 * k * np.sqrt(2.0) / orc.min_cell_diameter()
-    assert d1["cfl_c"] == pytest.approx(f * r1[2], rel=1e-13)
-    assert d1["cfl_u"] == pytest.approx(f * r1[3], rel=1e-13)
-    # stage 1 and 2 densities stay within a tiny band around the new state's
-    for s in (0, 1):
-        assert abs(d1["min_density"][s] / r1[0] - 1) < 1e-2 and abs(d1["max_density"][s] / r1[1] - 1) < 1e-2
+    for hs in (1, 2):
+        op.set_HYPERBOLIC_stage(hs)
+        orc.set_stage(hs, 2)
+        op.set_rho_for_fixed(dev.vector(af.BLOCK_DENSITY, rho2))
+        orc.set_rho_for_fixed(rho2)
+        op.set_pres_fixed(dev.vector(af.BLOCK_PRESSURE, p2))
+        orc.set_pres_fixed(p2)
+        op.set_NS_stage(2)
+        out = dev.vector(af.BLOCK_PRESSURE)
+        op.vmult(out, dev.vector(af.BLOCK_PRESSURE, p))
+        assert rel(out.download(), orc.vmult(p, 2)) < 1e-11, f"stage {hs}"
+    assert dev.stats()["inner_cg_its"] == 0
