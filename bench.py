@@ -198,6 +198,42 @@ def run_ensemble(args):
     return out
 
 
+def run_serve(args, precision):
+    """Serving stream: Poisson arrivals at --serve-rate req/s for
+    --serve-seconds s; each request is one hill-flow member with mesh drawn
+    from {64x32 Q2, 128x64 Q2, 256x128 Q3, 512x256 Q3}, parameters drawn as
+    in the ensemble leg and --serve-min-steps..--serve-max-steps steps,
+    admitted by device footprint up to 256 GB. Reports per-step and
+    per-request latency percentiles and the aggregate throughput."""
+    import atmospheric_flows_amd as af
+    rng = np.random.default_rng(20240602)
+    t, arrivals = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / args.serve_rate)
+        if t > args.serve_seconds:
+            break
+        arrivals.append(t)
+    meshes = ((64, 32, 2), (128, 64, 2), (256, 128, 3), (512, 256, 3))
+    ens = af.Ensemble(slots=args.ens_slots, hbm_budget_bytes=256e9)
+    dofs = 0
+    for i, ta in enumerate(arrivals):
+        nx, nz, k = meshes[rng.integers(4)]
+        U = float(rng.uniform(5, 15))
+        dt = min(512.0 / nx, 0.35 * (40000.0 / nx) / (U * k * np.sqrt(2.0)))
+        n = int(rng.integers(args.serve_min_steps, args.serve_max_steps + 1))
+        ens.submit(af.Member(k, nx, nz, nsteps=n, h0=float(rng.uniform(100, 600)), U=U,
+                             N_bv=float(rng.uniform(0.008, 0.012)), seed=5000 + i, arrival_s=ta,
+                             params=dict(dt=dt, eps=1e-10, precision=precision)))
+        dofs += 4 * nx * nz * (k + 1) ** 2 * n
+    print(f"[bench] serve ({'mixed' if precision else 'fp64'}): {len(arrivals)} requests", file=sys.stderr, flush=True)
+    wall = ens.run()
+    res = [ens.result(i) for i in range(len(arrivals))]
+    out = dict(requests=len(arrivals), rate_per_s=args.serve_rate, seconds=args.serve_seconds, wall_s=wall,
+               dof_steps_per_s=dofs / wall, failed=sum(r["status"] != 0 for r in res))
+    out.update(af.latency_summary(res))
+    return out
+
+
 def cpu_baseline(threads):
     """Oracle (C++ FP64, OpenMP) on a bounded sample of the same workload:
     one IMEX step on 128x64 Q3 at dt = 4 s, the same acoustic CFL (~13) and
@@ -230,6 +266,11 @@ def main():
     ap.add_argument("--no-traffic", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-tight", action="store_true")
+    ap.add_argument("--serve", action="store_true", help="also run the Poisson-arrival serving stream")
+    ap.add_argument("--serve-rate", type=float, default=2.0)
+    ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--serve-min-steps", type=int, default=10)
+    ap.add_argument("--serve-max-steps", type=int, default=40)
     ap.add_argument("--ens-members", type=int, default=64)
     ap.add_argument("--ens-steps", type=int, default=5)
     ap.add_argument("--ens-slots", type=int, default=8)
@@ -257,6 +298,7 @@ def main():
                 traffic=traffic["bytes"] if traffic else None, kernel=DOMINANT[1], bytes_per_launch=dom["bytes"],
                 ms_per_launch=dom["ms"], traffic_detail=traffic)
     ensemble = None if args.no_ensemble else run_ensemble(args)
+    serve = {"mixed": run_serve(args, 1), "fp64": run_serve(args, 0)} if args.serve else None
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_threads)
     out = {
         "metric": "DoF-steps/s mixed vs FP64 GPU at <=1e-8 rel err; p50/p99 ensemble step latency",
@@ -272,6 +314,7 @@ def main():
         "solver_stats": {"mixed": r["mixed"]["stats"], "fp64": r["fp64"]["stats"]},
         "converged_fp64_seconds": r["tight"]["seconds"] if "tight" in r else None,
         "kernels": r["kernels"], "roofline": roof, "cpu_baseline": cpu, "ensemble": ensemble,
+        "serve": serve,
     }
     print(json.dumps(out))
 
