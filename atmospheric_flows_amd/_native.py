@@ -67,7 +67,11 @@ SIGNATURES = [
     ("atmo_last_error", ctypes.c_char_p, [ctypes.c_void_p]),
     ("atmo_ctx_create", ctypes.c_int, [ctypes.POINTER(MeshDesc), ctypes.POINTER(Params),
                                        ctypes.POINTER(ctypes.c_void_p)]),
+    ("atmo_ctx_create_amr", ctypes.c_int, [ctypes.POINTER(MeshDesc), ctypes.POINTER(ctypes.c_int), ctypes.c_int,
+                                           ctypes.POINTER(Params), ctypes.POINTER(ctypes.c_void_p)]),
     ("atmo_ctx_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("atmo_ctx_init_hill_state", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_double] * 8 +
+     [ctypes.c_uint64, _dp, _dp, _dp]),
     ("atmo_set_params", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Params)]),
     ("atmo_ctx_info", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_long),
                                      ctypes.POINTER(ctypes.c_int)]),

@@ -52,6 +52,8 @@ struct Tables {
   double D[NQ][ND];
   double St[ND][NQ];  // transposes: St[i][q] = S[q][i]
   double Dt[ND][NQ];
+  double Sh[2][NQ][ND];   // Sh[h][q][i] = l_i((h + gq[q])/2): points of half h of a split face
+  double ShT[2][ND][NQ];  // its transpose
 };
 
 template <int K>
@@ -109,6 +111,14 @@ constexpr Tables<K> make_tables() {
       t.D[q][i] = s;
       t.St[i][q] = t.S[q][i];
       t.Dt[i][q] = s;
+      for (int h = 0; h < 2; ++h) {
+        const double xh = 0.5 * (h + x);
+        double vh = 1.0;
+        for (int m = 0; m < ND; ++m)
+          if (m != i) vh *= (xh - t.gll[m]) / (t.gll[i] - t.gll[m]);
+        t.Sh[h][q][i] = vh;
+        t.ShT[h][i][q] = vh;
+      }
     }
   }
   return t;

@@ -182,6 +182,40 @@ atmo_status atmo_ctx_create(const atmo_mesh_desc* mesh, const atmo_params* param
   });
 }
 
+atmo_status atmo_ctx_create_amr(const atmo_mesh_desc* base, const int* leaves, int nleaves, const atmo_params* params,
+                                atmo_ctx** out) {
+  return guarded(nullptr, [&]() {
+    if (!base || !leaves || !params || !out) throw std::invalid_argument("null argument");
+    if (nleaves < 1) throw std::invalid_argument("locally refined mesh: no leaves");
+    *out = nullptr;
+    MeshDesc m = to_mesh(base);
+    check_params(*params);
+    std::vector<QCell> cells(nleaves);
+    for (int c = 0; c < nleaves; ++c)
+      cells[c] = {leaves[5 * c], leaves[5 * c + 1], leaves[5 * c + 2], leaves[5 * c + 3], leaves[5 * c + 4]};
+    auto c = std::make_unique<atmo_ctx>();
+    c->mesh = std::move(m);
+    c->degree = base->degree;
+    c->eng = Engine::create_amr(base->degree, c->mesh, cells, to_engine(*params));
+    *out = c.release();
+    return ATMO_OK;
+  });
+}
+
+atmo_status atmo_ctx_init_hill_state(atmo_ctx* ctx, double theta0, double p0, double N, double U, double g, double R,
+                                     double gamma_gas, double perturbation, uint64_t seed, double* rho, double* u,
+                                     double* p) {
+  return guarded(ctx, [&]() {
+    if (!ctx || !rho || !u || !p) throw std::invalid_argument("null argument");
+    const std::vector<double>& geo = ctx->eng->cell_geometry();
+    if (geo.empty())
+      hill_initial_state(ctx->degree, ctx->mesh, theta0, p0, N, U, g, R, gamma_gas, perturbation, seed, rho, u, p);
+    else
+      hill_initial_state_cells(ctx->degree, geo, theta0, p0, N, U, g, R, gamma_gas, perturbation, seed, rho, u, p);
+    return ATMO_OK;
+  });
+}
+
 atmo_status atmo_ctx_destroy(atmo_ctx* ctx) {
   if (!ctx) return ATMO_INVALID_ARG;
   delete ctx;
@@ -498,7 +532,8 @@ atmo_status atmo_step_diagnostics(atmo_ctx* ctx, int of_state, atmo_step_diag* o
     for (int s = 0; s < 4; ++s) bad += rec[DG_NONFINITE + s];
     out->nonfinite = (long)bad;
     const MeshDesc& m = ctx->mesh;
-    out->min_cell_diameter = min_cell_diameter(m.nx, m.nz, m.Lx, m.H, m.h.data());
+    out->min_cell_diameter = ctx->eng->min_cell_diameter() > 0 ? ctx->eng->min_cell_diameter()
+                                                                 : min_cell_diameter(m.nx, m.nz, m.Lx, m.H, m.h.data());
     // Courant numbers as printed by EulerSolver::run (Atmospheric_Flow.cc:1225-1235), dim = 2
     const EngineParams& p = ctx->eng->params();
     const double f = p.dt * ctx->degree * std::sqrt(2.0) / out->min_cell_diameter;

@@ -94,23 +94,46 @@ struct PhysDminusC {
   __device__ void bdry(const double (&)[NF], double, double, double (&F)[NO]) const { F[0] = 0; }
 };
 
-// LDS of one 64-lane block of a cell+face kernel.
-template <int K, class Ph>
+// LDS of one 64-lane block of a cell+face kernel. AMR (locally refined
+// mesh): the fluxes of all four faces, per half of a split face, go through
+// LDS with the face type of each cell; otherwise the bottom/top fluxes are
+// folded into the column partials and only left/right go through LDS.
+template <int K, class Ph, bool AMR>
 struct CFSmem {
   static constexpr int ND = K + 1, NQ = 2 * K + 1, ND2 = ND * ND, CPW = 64 / NQ;
+  static constexpr int NFACE = AMR ? 4 : 2, NHALF = AMR ? 2 : 1;
   double in[Ph::NF][CPW * ND2];
-  double T[Ph::NO][2][CPW][ND][NQ];  // column partials of the value/eta and xi test functions
-  double F[Ph::NO][2][CPW][NQ];      // left/right face fluxes times the face weight
+  double T[Ph::NO][2][CPW][ND][NQ];          // column partials of the value/eta and xi test functions
+  double F[Ph::NO][NFACE][NHALF][CPW][NQ];   // face fluxes times the face weight
+  int ft[AMR ? CPW : 1][4];                  // AMR: 0 no flux, 1 full face, 2 split face
 };
+
+// Trace of field q of a cell along face f at lane point c: Tab[i] weights
+// along the face (the cell's rows for bottom/top, columns for left/right).
+template <int K>
+__device__ inline double face_trace(const double* __restrict__ u, int f, const double (&Tab)[K + 1]) {
+  constexpr int ND = K + 1;
+  double s = 0;
+  if (f < 2) {
+    const int col = f == 0 ? 0 : K;
+#pragma unroll
+    for (int j = 0; j < ND; ++j) s += Tab[j] * u[j * ND + col];
+  } else {
+    const int row = f == 2 ? 0 : K;
+#pragma unroll
+    for (int i = 0; i < ND; ++i) s += Tab[i] * u[row * ND + i];
+  }
+  return s;
+}
 
 // Column phase of a cell+face operator for the block's CPW cells: stages
 // the input fields in LDS, evaluates cell and face terms per quadrature
 // column (neighbour traces are read from memory; they are L2/MALL hits)
 // and leaves the column partials in sh.T / sh.F. Ends with a block
 // barrier.
-template <int K, class Ph>
+template <int K, class Ph, bool AMR>
 __device__ inline void cellface_columns(const MeshDev& m, const Ph& ph, const FieldPtrs<Ph::NF>& in,
-                                        CFSmem<K, Ph>& sh) {
+                                        CFSmem<K, Ph, AMR>& sh) {
   using C = Cfg<K>;
   constexpr int ND = C::ND, NQ = C::NQ, ND2 = C::ND2;
   constexpr int NF = Ph::NF, NO = Ph::NO;
@@ -120,9 +143,8 @@ __device__ inline void cellface_columns(const MeshDev& m, const Ph& ph, const Fi
   for (int f = 0; f < NF; ++f) stage_cells<K>(sh.in[f], in.p[f], m.ncell);
   __syncthreads();
   if (L.active) {
-    const int ci = L.cell % m.nx, cj = L.cell / m.nx;
-    const CellVerts v = cell_verts(m, ci, cj);
-    const ColGeo<double> g = col_geo<K, double>(v, m.dx, L.c);
+    const CellVerts v = cell_verts(m, L.cell);
+    const ColGeo<double> g = col_geo<K, double>(v, L.c);
     double Sc[ND], Dc[ND];
     lane_rows<K, double>(L.c, Sc, Dc);
     const int off = L.cw * ND2;
@@ -174,74 +196,116 @@ __device__ inline void cellface_columns(const MeshDev& m, const Ph& ph, const Fi
         }
       }
     }
-    // bottom (j-1) and top (j+1) faces: this lane is face point c
+    if constexpr (!AMR) {
+      const int cj = L.cell / m.nx, ci = L.cell - cj * m.nx;
+      // bottom (j-1) and top (j+1) faces: this lane is face point c
 #pragma unroll
-    for (int side = 0; side < 2; ++side) {
-      const int f = 2 + side;
-      const FaceGeo fg = face_geo(v, m.dx, f);
-      const double wl = fg.len * g.wc;
-      double own[NF], nb[NF], F[NO];
+      for (int side = 0; side < 2; ++side) {
+        const int f = 2 + side;
+        const FaceGeo fg = face_geo(v, f);
+        const double wl = fg.len * g.wc;
+        double own[NF], nb[NF], F[NO];
 #pragma unroll
-      for (int q = 0; q < NF; ++q) own[q] = tv[q][side ? K : 0];
-      const int nj = side ? cj + 1 : cj - 1;
-      if (nj >= 0 && nj < m.nz) {
-        const long nbase = (long)(nj * m.nx + ci) * ND2 + (side ? 0 : K * ND);
+        for (int q = 0; q < NF; ++q) own[q] = tv[q][side ? K : 0];
+        const int nj = side ? cj + 1 : cj - 1;
+        if (nj >= 0 && nj < m.nz) {
+          const long nbase = (long)(nj * m.nx + ci) * ND2;
 #pragma unroll
-        for (int q = 0; q < NF; ++q) {
-          double s = 0;
+          for (int q = 0; q < NF; ++q) nb[q] = face_trace<K>(in.p[q] + nbase, f ^ 1, Sc);
+          ph.face(own, nb, fg.n0, fg.n1, F);
+        } else if (Ph::BDRY) {
+          ph.bdry(own, fg.n0, fg.n1, F);
+        } else {
 #pragma unroll
-          for (int i = 0; i < ND; ++i) s += Sc[i] * in.p[q][nbase + i];
-          nb[q] = s;
+          for (int o = 0; o < NO; ++o) F[o] = 0;
         }
-        ph.face(own, nb, fg.n0, fg.n1, F);
-      } else if (Ph::BDRY) {
-        ph.bdry(own, fg.n0, fg.n1, F);
-      } else {
 #pragma unroll
-        for (int o = 0; o < NO; ++o) F[o] = 0;
+        for (int o = 0; o < NO; ++o) Tv[o][side ? K : 0] += F[o] * wl;
       }
+      // left (i-1) and right (i+1) faces: this lane is face point s = c
 #pragma unroll
-      for (int o = 0; o < NO; ++o) Tv[o][side ? K : 0] += F[o] * wl;
-    }
-    // left (i-1) and right (i+1) faces: this lane is face point s = c
+      for (int side = 0; side < 2; ++side) {
+        const int f = side;
+        const FaceGeo fg = face_geo(v, f);
+        const double wl = fg.len * g.wc;
+        double own[NF], nb[NF], F[NO];
 #pragma unroll
-    for (int side = 0; side < 2; ++side) {
-      const int f = side;
-      const FaceGeo fg = face_geo(v, m.dx, f);
-      const double wl = fg.len * g.wc;
-      double own[NF], nb[NF], F[NO];
-      const int col = side ? K : 0;
-#pragma unroll
-      for (int q = 0; q < NF; ++q) {
-        double s = 0;
-#pragma unroll
-        for (int j = 0; j < ND; ++j) s += Sc[j] * sh.in[q][off + j * ND + col];
-        own[q] = s;
-      }
-      int ni = side ? ci + 1 : ci - 1;
-      bool inner = true;
-      if (ni < 0 || ni >= m.nx) {
-        if (m.periodic) ni = (ni + m.nx) % m.nx;
-        else inner = false;
-      }
-      if (inner) {
-        const long nbase = (long)(cj * m.nx + ni) * ND2 + (side ? 0 : K);
-#pragma unroll
-        for (int q = 0; q < NF; ++q) {
-          double s = 0;
-#pragma unroll
-          for (int j = 0; j < ND; ++j) s += Sc[j] * in.p[q][nbase + j * ND];
-          nb[q] = s;
+        for (int q = 0; q < NF; ++q) own[q] = face_trace<K>(&sh.in[q][off], f, Sc);
+        int ni = side ? ci + 1 : ci - 1;
+        bool inner = true;
+        if (ni < 0 || ni >= m.nx) {
+          if (m.periodic) ni = (ni + m.nx) % m.nx;
+          else inner = false;
         }
-        ph.face(own, nb, fg.n0, fg.n1, F);
-      } else if (Ph::BDRY) {
-        ph.bdry(own, fg.n0, fg.n1, F);
-      } else {
+        if (inner) {
+          const long nbase = (long)(cj * m.nx + ni) * ND2;
 #pragma unroll
-        for (int o = 0; o < NO; ++o) F[o] = 0;
+          for (int q = 0; q < NF; ++q) nb[q] = face_trace<K>(in.p[q] + nbase, f ^ 1, Sc);
+          ph.face(own, nb, fg.n0, fg.n1, F);
+        } else if (Ph::BDRY) {
+          ph.bdry(own, fg.n0, fg.n1, F);
+        } else {
+#pragma unroll
+          for (int o = 0; o < NO; ++o) F[o] = 0;
+        }
+#pragma unroll
+        for (int o = 0; o < NO; ++o) sh.F[o][side][0][L.cw][L.c] = F[o] * wl;
       }
+    } else {
+      // locally refined mesh: per face, full (same-size or larger neighbour)
+      // or split in two halves (two smaller neighbours)
+      double Sh0[ND], Sh1[ND];
 #pragma unroll
-      for (int o = 0; o < NO; ++o) sh.F[o][side][L.cw][L.c] = F[o] * wl;
+      for (int i = 0; i < ND; ++i) {
+        Sh0[i] = cT<K>.Sh[0][L.c][i];
+        Sh1[i] = cT<K>.Sh[1][L.c][i];
+      }
+      const int* rec = m.cface + 16 * (long)L.cell;
+#pragma unroll
+      for (int f = 0; f < 4; ++f) {
+        const int type = rec[4 * f], n0 = rec[4 * f + 1], n1 = rec[4 * f + 2], half = rec[4 * f + 3];
+        const FaceGeo fg = face_geo(v, f);
+        double own[NF], nb[NF], F[NO];
+        if (type != FACE_FINER) {
+          const double wl = fg.len * g.wc;
+#pragma unroll
+          for (int q = 0; q < NF; ++q) own[q] = face_trace<K>(&sh.in[q][off], f, Sc);
+          if (type == FACE_BDRY) {
+            if (Ph::BDRY) ph.bdry(own, fg.n0, fg.n1, F);
+            else
+#pragma unroll
+              for (int o = 0; o < NO; ++o) F[o] = 0;
+          } else {
+            // same-size neighbour at point c, or the larger neighbour at (half + gq[c])/2 of its face
+            const double* Tn = type == FACE_SAME ? Sc : (half ? Sh1 : Sh0);
+            double Tab[ND];
+#pragma unroll
+            for (int i = 0; i < ND; ++i) Tab[i] = Tn[i];
+#pragma unroll
+            for (int q = 0; q < NF; ++q) nb[q] = face_trace<K>(in.p[q] + (long)n0 * ND2, f ^ 1, Tab);
+            ph.face(own, nb, fg.n0, fg.n1, F);
+          }
+#pragma unroll
+          for (int o = 0; o < NO; ++o) sh.F[o][f][0][L.cw][L.c] = F[o] * wl;
+        } else {
+          const double wl = 0.5 * fg.len * g.wc;
+#pragma unroll
+          for (int hh = 0; hh < 2; ++hh) {
+            double Tab[ND];
+#pragma unroll
+            for (int i = 0; i < ND; ++i) Tab[i] = hh ? Sh1[i] : Sh0[i];
+#pragma unroll
+            for (int q = 0; q < NF; ++q) {
+              own[q] = face_trace<K>(&sh.in[q][off], f, Tab);
+              nb[q] = face_trace<K>(in.p[q] + (long)(hh ? n1 : n0) * ND2, f ^ 1, Sc);
+            }
+            ph.face(own, nb, fg.n0, fg.n1, F);
+#pragma unroll
+            for (int o = 0; o < NO; ++o) sh.F[o][f][hh][L.cw][L.c] = F[o] * wl;
+          }
+        }
+        if (L.c == 0) sh.ft[L.cw][f] = type == FACE_FINER ? 2 : 1;
+      }
     }
 #pragma unroll
     for (int o = 0; o < NO; ++o)
@@ -281,8 +345,8 @@ constexpr bool rows_invariant() {
 }
 
 // Output phase: component o of block-local dof t = cw*ND2 + jj*ND + ii.
-template <int K, class Ph>
-__device__ inline double cellface_out(const CFSmem<K, Ph>& sh, int o, int t, const OutRows<K>& R) {
+template <int K, class Ph, bool AMR>
+__device__ inline double cellface_out(const CFSmem<K, Ph, AMR>& sh, int o, int t, const OutRows<K>& R) {
   constexpr int ND = K + 1, NQ = 2 * K + 1, ND2 = ND * ND;
   const int cw = t / ND2, o2 = t - cw * ND2, jj = o2 / ND, ii = o2 - jj * ND;
   const bool gx = (Ph::GX >> o) & 1;
@@ -292,10 +356,29 @@ __device__ inline double cellface_out(const CFSmem<K, Ph>& sh, int o, int t, con
     acc += R.Si[c] * sh.T[o][0][cw][jj][c];
     if (gx) acc += R.Di[c] * sh.T[o][1][cw][jj][c];
   }
-  if (ii == 0 || ii == K) {
-    const int side = ii == 0 ? 0 : 1;
+  if constexpr (!AMR) {
+    if (ii == 0 || ii == K) {
+      const int side = ii == 0 ? 0 : 1;
 #pragma unroll
-    for (int c = 0; c < NQ; ++c) acc += R.Sj[c] * sh.F[o][side][cw][c];
+      for (int c = 0; c < NQ; ++c) acc += R.Sj[c] * sh.F[o][side][0][cw][c];
+    }
+  } else {
+    // faces through this dof: left/right weigh along jj, bottom/top along ii
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+      const bool on = f == 0 ? ii == 0 : f == 1 ? ii == K : f == 2 ? jj == 0 : jj == K;
+      if (!on) continue;
+      const int along = f < 2 ? jj : ii;
+      if (sh.ft[cw][f] == 2) {
+#pragma unroll
+        for (int hh = 0; hh < 2; ++hh)
+#pragma unroll
+          for (int c = 0; c < NQ; ++c) acc += cT<K>.ShT[hh][along][c] * sh.F[o][f][hh][cw][c];
+      } else {
+#pragma unroll
+        for (int c = 0; c < NQ; ++c) acc += cT<K>.St[along][c] * sh.F[o][f][0][cw][c];
+      }
+    }
   }
   return acc;
 }

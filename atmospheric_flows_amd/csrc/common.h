@@ -45,29 +45,63 @@ struct Cfg {
   static constexpr int LANES = CPW * NQ;
 };
 
-// Terrain-following structured mesh of one MG level: vertex (i,j) at
-// x = i*dx, z = h[i] + (H - h[i]) * j * inv_nz. h has nx+1 entries (for a
-// periodic mesh h[nx] == h[0]). Cell columns are vertical: every cell's
-// left and right edges have constant x, so x_eta = 0 and x_xi = dx exactly.
+// Mesh of one MG level. Structured terrain-following form: vertex (i,j) at
+// x = i*dx, z = h[i] + (H - h[i]) * j * inv_nz, cell = j*nx + i; h has nx+1
+// entries (for a periodic mesh h[nx] == h[0]). Locally refined form (cgeo
+// set): per-cell geometry cgeo[6*cell] = (x0, dx, z00, z10, z01, z11) of
+// the cell's bilinear map and, on the finest level, the face table cface.
+// Either way cell columns are vertical: every cell's left and right edges
+// have constant x, so x_eta = 0 and x_xi = dx exactly.
+//
+// cface[16*cell + 4*f + {0,1,2,3}] = (type, n0, n1, half) for face f (0 left,
+// 1 right, 2 bottom, 3 top): FACE_BDRY; FACE_SAME (neighbour n0 of the same
+// size); FACE_COARSER (this face is half `half` of face f^1 of the larger
+// neighbour n0); FACE_FINER (the face is split: half 0 is a face of n0,
+// half 1 of n1).
+//
+// Transfers to the next finer level: child[4*cell + a + 2b] (child (a,b) of
+// a refined cell, or (same cell, -1, -1, -1)); from the next coarser level:
+// parent[2*cell] = (coarse cell, a + 2b or -1 for the same cell). Null:
+// the structured rule (children 2I+a, 2J+b).
+enum FaceType { FACE_BDRY = 0, FACE_SAME = 1, FACE_COARSER = 2, FACE_FINER = 3 };
+
 struct MeshDev {
   const double* h;
   int nx, nz;
   double dx, H, inv_nz;
   int periodic;
   int ncell;
+  const double* cgeo;
+  const int* cface;
+  const int* child;
+  const int* parent;
 };
 
 // The 1D tables in constant memory, for lane-dependent rows and columns.
 template <int K>
 __constant__ basis::Tables<K> cT = basis::make_tables<K>();
 
-// Per-cell vertex data (x00,x10,x01,x11 and z likewise).
+// Per-cell vertex data (x00,x10,x01,x11 and z likewise) and the cell width.
 struct CellVerts {
   double x[4], z[4];
+  double dx;
 };
 
-__device__ inline CellVerts cell_verts(const MeshDev& m, int i, int j) {
+__device__ inline CellVerts cell_verts(const MeshDev& m, int cell) {
   CellVerts v;
+  if (m.cgeo) {
+    const double* g = m.cgeo + 6 * (long)cell;
+    v.dx = g[1];
+    v.x[0] = v.x[2] = g[0];
+    v.x[1] = v.x[3] = g[0] + g[1];
+    v.z[0] = g[2];
+    v.z[1] = g[3];
+    v.z[2] = g[4];
+    v.z[3] = g[5];
+    return v;
+  }
+  const int j = cell / m.nx, i = cell - j * m.nx;
+  v.dx = m.dx;
   const double hl = m.h[i], hr = m.h[i + 1];
   const double fj = j * m.inv_nz, fj1 = (j + 1) * m.inv_nz;
   v.x[0] = i * m.dx;
@@ -93,10 +127,10 @@ struct ColGeo {
 };
 
 template <int K, typename T>
-__device__ inline ColGeo<T> col_geo(const CellVerts& v, double dx, int c) {
+__device__ inline ColGeo<T> col_geo(const CellVerts& v, int c) {
   const double xi = cT<K>.gq[c];
   ColGeo<T> g;
-  g.dx = (T)dx;
+  g.dx = (T)v.dx;
   g.ze = (T)((1 - xi) * (v.z[2] - v.z[0]) + xi * (v.z[3] - v.z[1]));
   g.b0 = (T)(v.z[1] - v.z[0]);
   g.b1 = (T)((v.z[3] - v.z[2]) - (v.z[1] - v.z[0]));
@@ -111,7 +145,8 @@ struct FaceGeo {
 
 // f: 0 left (xi=0), 1 right (xi=1), 2 bottom (eta=0), 3 top (eta=1). The
 // left/right edges are vertical, so their normals are exactly (-1,0)/(1,0).
-__device__ inline FaceGeo face_geo(const CellVerts& v, double dx, int f) {
+__device__ inline FaceGeo face_geo(const CellVerts& v, int f) {
+  const double dx = v.dx;
   FaceGeo g;
   if (f < 2) {
     g.len = f == 0 ? v.z[2] - v.z[0] : v.z[3] - v.z[1];

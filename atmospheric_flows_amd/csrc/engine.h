@@ -32,6 +32,12 @@ struct EngineParams {
   int precision = 0;               // 0: FP64 ; 1: mixed (FP32 MG levels, cellwise-direct A^{-1})
 };
 
+// One cell of a locally refined mesh: base cell (bi, bj), quadtree level l,
+// position (sx, sz) in [0, 2^l)^2 inside the base cell (amr.h).
+struct QCell {
+  int bi, bj, l, sx, sz;
+};
+
 struct MeshDesc {
   int nx = 0, nz = 0, periodic = 1;
   double Lx = 0, H = 0;
@@ -50,6 +56,9 @@ class Engine {
  public:
   virtual ~Engine() = default;
   static std::unique_ptr<Engine> create(int k, const MeshDesc& mesh, const EngineParams& p);
+  // locally refined mesh over a base mesh; leaves as for build_amr_levels (amr.h)
+  static std::unique_ptr<Engine> create_amr(int k, const MeshDesc& base, const std::vector<QCell>& leaves,
+                                            const EngineParams& p);
 
   virtual int degree() const = 0;
   virtual int ncells() const = 0;
@@ -60,6 +69,10 @@ class Engine {
   virtual const EngineParams& params() const = 0;
   virtual EngineStats& stats() = 0;
   virtual size_t device_bytes() const = 0;  // device memory requested by this engine
+  // Finest-level cell geometry (6 per cell: x0, dx, z00, z10, z01, z11) of a
+  // locally refined mesh, empty for a structured one; its minimal diameter.
+  virtual const std::vector<double>& cell_geometry() const = 0;
+  virtual double min_cell_diameter() const = 0;
 
   // ---- EULEROperator interface on device pointers
   virtual void set_stages(int hyperbolic_stage, int ns_stage) = 0;
@@ -97,5 +110,9 @@ size_t engine_device_bytes(int k, const MeshDesc& mesh, const EngineParams& p);
 void hill_initial_state(int k, const MeshDesc& m, double theta0, double p0, double Nbv, double U, double g,
                         double Rgas, double gamma_gas, double pert, uint64_t seed, double* rho, double* u,
                         double* p);
+// the same on cells given by their geometry (6 per cell, Engine::cell_geometry)
+void hill_initial_state_cells(int k, const std::vector<double>& cgeo, double theta0, double p0, double Nbv, double U,
+                              double g, double Rgas, double gamma_gas, double pert, uint64_t seed, double* rho,
+                              double* u, double* p);
 
 }  // namespace atmo

@@ -8,6 +8,7 @@
 #include <functional>
 #include <type_traits>
 
+#include "amr.h"
 #include "diag.h"
 #include "engine.h"
 #include "krylov.h"
@@ -63,6 +64,9 @@ struct Level {
   DBuf<float> x32, b32, t32, d32;
   DBuf<double> eigb1, eigb2;
   DBuf<double> cg_g, cg_d, cg_h;  // coarse-level CG work
+  // locally refined levels: cell geometry, face table (finest), transfer maps
+  DBuf<double> geo;
+  DBuf<int> faces, childmap, parentmap;
 };
 
 // symmetric tridiagonal extreme eigenvalues (Sturm bisection)
@@ -118,6 +122,22 @@ class EngineK : public Engine {
     ATMO_HIP_CHECK(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
     oc_ = consts();
     build_levels(mesh);
+    setup();
+    bytes_ = tl_dev_bytes - bytes0;
+  }
+
+  // locally refined mesh over the base mesh (amr.h)
+  EngineK(const MeshDesc& base, const std::vector<QCell>& leaves, const EngineParams& p) : P(p) {
+    const size_t bytes0 = tl_dev_bytes;
+    ATMO_HIP_CHECK(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
+    oc_ = consts();
+    build_levels_amr(base, leaves);
+    setup();
+    bytes_ = tl_dev_bytes - bytes0;
+  }
+
+  // buffers, reductions and the geometry-only diagonals of an engine whose levels exist
+  void setup() {
     const long Ns = fine().Ns;
     red_partials.alloc(RedBuf::doubles());
     red_partials.zero(s_);
@@ -158,7 +178,6 @@ class EngineK : public Engine {
       estimate_eig(l, BLK_D);
     }
     sync();
-    bytes_ = tl_dev_bytes - bytes0;
   }
 
   ~EngineK() override {
@@ -178,6 +197,8 @@ class EngineK : public Engine {
   EngineStats& stats() override { return st_; }
 
   size_t device_bytes() const override { return bytes_; }
+  const std::vector<double>& cell_geometry() const override { return host_cells_; }
+  double min_cell_diameter() const override { return amr_min_diam_; }
 
   void set_params(const EngineParams& p) override {
     const size_t bytes0 = tl_dev_bytes;
@@ -413,6 +434,8 @@ class EngineK : public Engine {
   EngineStats st_;
   int last_inner = 8;  // iterations of the last inner CG (first batch of the next)
   size_t bytes_ = 0;  // device bytes requested by this engine's buffers
+  double amr_min_diam_ = 0.0;      // locally refined mesh: minimal cell diameter
+  std::vector<double> host_cells_;  // locally refined mesh: finest-level cell geometry (6 per cell)
 
   OpConsts consts() const {
     OpConsts o;
@@ -473,36 +496,84 @@ class EngineK : public Engine {
       L->m.inv_nz = 1.0 / d.nz;
       L->m.periodic = mesh.periodic;
       L->m.ncell = L->ncell;
-      const long n2 = 2 * L->Ns;
-      L->rho64.alloc(L->Ns);
-      L->rho32.alloc(L->Ns);
-      L->rho64.zero(s_);
-      L->rho32.zero(s_);
-      for (int b = 0; b < 3; ++b) {
-        L->inv64[b].alloc(b == BLK_A ? n2 : L->Ns);
-        L->inv32[b].alloc(b == BLK_A ? n2 : L->Ns);
-      }
-      for (auto* b : {&L->x64, &L->b64, &L->t64, &L->d64}) b->alloc(n2);
-      for (auto* b : {&L->x32, &L->b32, &L->t32, &L->d32}) b->alloc(n2);
-      if (l == 0)
-        for (auto* b : {&L->cg_g, &L->cg_d, &L->cg_h}) b->alloc(n2);
-      // Chebyshev eigenvalue start vectors: (i % 11) - mean
-      for (int nc = 1; nc <= 2; ++nc) {
-        const long n = nc * L->Ns;
-        std::vector<double> v(n);
-        double mean = 0;
-        for (long i = 0; i < n; ++i) {
-          v[i] = (double)(i % 11);
-          mean += v[i];
-        }
-        mean /= (double)n;
-        for (auto& x : v) x -= mean;
-        DBuf<double>& dst = nc == 1 ? L->eigb1 : L->eigb2;
-        dst.alloc(n);
-        ATMO_HIP_CHECK(hipMemcpy(dst.p, v.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-      }
+      alloc_level_vectors(*L, l == 0);
       lv.push_back(std::move(L));
     }
+  }
+
+  // work vectors of one level (coarse: with the coarse-CG buffers)
+  void alloc_level_vectors(Level& L, bool coarse) {
+    const long n2 = 2 * L.Ns;
+    L.rho64.alloc(L.Ns);
+    L.rho32.alloc(L.Ns);
+    L.rho64.zero(s_);
+    L.rho32.zero(s_);
+    for (int b = 0; b < 3; ++b) {
+      L.inv64[b].alloc(b == BLK_A ? n2 : L.Ns);
+      L.inv32[b].alloc(b == BLK_A ? n2 : L.Ns);
+    }
+    for (auto* b : {&L.x64, &L.b64, &L.t64, &L.d64}) b->alloc(n2);
+    for (auto* b : {&L.x32, &L.b32, &L.t32, &L.d32}) b->alloc(n2);
+    if (coarse)
+      for (auto* b : {&L.cg_g, &L.cg_d, &L.cg_h}) b->alloc(n2);
+    // Chebyshev eigenvalue start vectors: (i % 11) - mean
+    for (int nc = 1; nc <= 2; ++nc) {
+      const long n = nc * L.Ns;
+      std::vector<double> v(n);
+      double mean = 0;
+      for (long i = 0; i < n; ++i) {
+        v[i] = (double)(i % 11);
+        mean += v[i];
+      }
+      mean /= (double)n;
+      for (auto& x : v) x -= mean;
+      DBuf<double>& dst = nc == 1 ? L.eigb1 : L.eigb2;
+      dst.alloc(n);
+      ATMO_HIP_CHECK(hipMemcpy(dst.p, v.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    }
+  }
+
+  // Levels of a locally refined mesh: the uniformly coarsened base mesh
+  // below, the globally coarsened quadtree levels above (amr.h).
+  void build_levels_amr(const MeshDesc& base, const std::vector<QCell>& leaves) {
+    std::vector<AmrLevelHost> hl = build_amr_levels(base, leaves);
+    amr_min_diam_ = amr_min_cell_diameter(hl.front());
+    build_levels(base);  // structured levels, base mesh finest
+    if (hl.size() == 1) return;  // nothing refined: the base mesh is the mesh
+    auto upload_int = [&](DBuf<int>& d, const std::vector<int>& v) {
+      d.alloc(v.size());
+      ATMO_HIP_CHECK(hipMemcpy(d.p, v.data(), sizeof(int) * v.size(), hipMemcpyHostToDevice));
+    };
+    // the base level's children live in the coarsest quadtree level above it
+    upload_int(lv.back()->childmap, hl[hl.size() - 1].child);
+    lv.back()->m.child = lv.back()->childmap.p;
+    for (int i = (int)hl.size() - 2; i >= 0; --i) {
+      const AmrLevelHost& H = hl[i];
+      auto L = std::make_unique<Level>();
+      L->ncell = H.ncell;
+      L->nx = H.ncell;
+      L->nz = 1;
+      L->Ns = (long)H.ncell * (K + 1) * (K + 1);
+      L->geo.alloc(H.cgeo.size());
+      ATMO_HIP_CHECK(hipMemcpy(L->geo.p, H.cgeo.data(), sizeof(double) * H.cgeo.size(), hipMemcpyHostToDevice));
+      upload_int(L->parentmap, H.parent);
+      if (i > 0) upload_int(L->childmap, H.child);
+      if (i == 0) upload_int(L->faces, H.cface);
+      L->m.nx = H.ncell;
+      L->m.nz = 1;
+      L->m.dx = 0.0;
+      L->m.H = base.H;
+      L->m.inv_nz = 1.0;
+      L->m.periodic = base.periodic;
+      L->m.ncell = H.ncell;
+      L->m.cgeo = L->geo.p;
+      L->m.parent = L->parentmap.p;
+      L->m.child = i > 0 ? L->childmap.p : nullptr;
+      L->m.cface = i == 0 ? L->faces.p : nullptr;
+      alloc_level_vectors(*L, false);
+      lv.push_back(std::move(L));
+    }
+    host_cells_ = hl.front().cgeo;
   }
 
   // level operator in precision T: dst = Op_blk src (+ add_scale*add)
@@ -993,6 +1064,16 @@ size_t engine_device_bytes(int k, const MeshDesc& mesh, const EngineParams& p) {
   return b;
 }
 
+std::unique_ptr<Engine> Engine::create_amr(int k, const MeshDesc& base, const std::vector<QCell>& leaves,
+                                           const EngineParams& p) {
+  switch (k) {
+    case 1: return std::make_unique<EngineK<1>>(base, leaves, p);
+    case 2: return std::make_unique<EngineK<2>>(base, leaves, p);
+    case 3: return std::make_unique<EngineK<3>>(base, leaves, p);
+    default: throw std::invalid_argument("degree must be 1, 2 or 3");
+  }
+}
+
 std::unique_ptr<Engine> Engine::create(int k, const MeshDesc& mesh, const EngineParams& p) {
   switch (k) {
     case 1: return std::make_unique<EngineK<1>>(mesh, p);
@@ -1014,9 +1095,10 @@ static uint64_t splitmix64(uint64_t x) {
 // the FE_DGQ support points (VectorTools::interpolate, Atmospheric_Flow.cc:420-422),
 // density perturbed by 1 + pert*(2U-1), U = top 53 bits of
 // splitmix64(seed ^ global dof id) / 2^53.
-void hill_initial_state(int k, const MeshDesc& m, double theta0, double p0, double Nbv, double U, double g,
-                        double Rgas, double gamma_gas, double pert, uint64_t seed, double* rho, double* u,
-                        double* p) {
+// Hill state on cells given by their four vertex heights zc[4*c..] (x plays no role).
+static void hill_state_cells(int k, long ncell, const double* zc, double theta0, double p0, double Nbv, double U,
+                             double g, double Rgas, double gamma_gas, double pert, uint64_t seed, double* rho,
+                             double* u, double* p) {
   constexpr int KMAX = 3;
   if (k < 1 || k > KMAX) throw std::invalid_argument("degree must be 1..3");
   double gll[KMAX + 1];
@@ -1024,13 +1106,10 @@ void hill_initial_state(int k, const MeshDesc& m, double theta0, double p0, doub
   else if (k == 2) { for (int i = 0; i < 3; ++i) gll[i] = basis::kT<2>.gll[i]; }
   else { for (int i = 0; i < 4; ++i) gll[i] = basis::kT<3>.gll[i]; }
   const int nd = k + 1, nd2 = nd * nd;
-  const long Ns = (long)m.nx * m.nz * nd2;
+  const long Ns = ncell * nd2;
   const double cp = gamma_gas * Rgas / (gamma_gas - 1.0);
-  for (int c = 0; c < m.nx * m.nz; ++c) {
-    const int i = c % m.nx, j = c / m.nx;
-    const double hl = m.h[i], hr = m.h[i + 1];
-    const double z[4] = {hl + (m.H - hl) * (double)j / m.nz, hr + (m.H - hr) * (double)j / m.nz,
-                         hl + (m.H - hl) * (double)(j + 1) / m.nz, hr + (m.H - hr) * (double)(j + 1) / m.nz};
+  for (long c = 0; c < ncell; ++c) {
+    const double* z = zc + 4 * c;
     for (int jj = 0; jj < nd; ++jj)
       for (int ii = 0; ii < nd; ++ii) {
         const double xi = gll[ii], eta = gll[jj];
@@ -1048,6 +1127,30 @@ void hill_initial_state(int k, const MeshDesc& m, double theta0, double p0, doub
         u[Ns + id] = 0.0;
       }
   }
+}
+
+void hill_initial_state(int k, const MeshDesc& m, double theta0, double p0, double Nbv, double U, double g,
+                        double Rgas, double gamma_gas, double pert, uint64_t seed, double* rho, double* u,
+                        double* p) {
+  std::vector<double> zc(4 * (size_t)m.nx * m.nz);
+  for (int c = 0; c < m.nx * m.nz; ++c) {
+    const int i = c % m.nx, j = c / m.nx;
+    const double hl = m.h[i], hr = m.h[i + 1];
+    const double z[4] = {hl + (m.H - hl) * (double)j / m.nz, hr + (m.H - hr) * (double)j / m.nz,
+                         hl + (m.H - hl) * (double)(j + 1) / m.nz, hr + (m.H - hr) * (double)(j + 1) / m.nz};
+    for (int a = 0; a < 4; ++a) zc[4 * c + a] = z[a];
+  }
+  hill_state_cells(k, (long)m.nx * m.nz, zc.data(), theta0, p0, Nbv, U, g, Rgas, gamma_gas, pert, seed, rho, u, p);
+}
+
+void hill_initial_state_cells(int k, const std::vector<double>& cgeo, double theta0, double p0, double Nbv, double U,
+                              double g, double Rgas, double gamma_gas, double pert, uint64_t seed, double* rho,
+                              double* u, double* p) {
+  const long nc = (long)cgeo.size() / 6;
+  std::vector<double> zc(4 * nc);
+  for (long c = 0; c < nc; ++c)
+    for (int a = 0; a < 4; ++a) zc[4 * c + a] = cgeo[6 * c + 2 + a];
+  hill_state_cells(k, nc, zc.data(), theta0, p0, Nbv, U, g, Rgas, gamma_gas, pert, seed, rho, u, p);
 }
 
 }  // namespace atmo

@@ -528,12 +528,11 @@ __global__ __launch_bounds__(256) void k_block_inv(MeshDev m, const double* __re
   __syncthreads();
   const int cell = blockIdx.x * 4 + w;
   if (cell >= m.ncell) return;
-  const int ci = cell % m.nx, cj = cell / m.nx;
-  const CellVerts vv = cell_verts(m, ci, cj);
+  const CellVerts vv = cell_verts(m, cell);
   // weights JxW * rho at the NQ^2 points (lane = q)
   for (int q = lane; q < NQ2; q += 64) {
     const int c = q % NQ, r = q / NQ;
-    const ColGeo<double> g = col_geo<K, double>(vv, m.dx, c);
+    const ColGeo<double> g = col_geo<K, double>(vv, c);
     double eta = 0, wr = 0;
 #pragma unroll
     for (int rr = 0; rr < NQ; ++rr)

@@ -1,5 +1,7 @@
-// Multigrid kernels on the uniformly coarsened terrain mesh: child (a,b) of
-// coarse cell (I,J) is fine cell (2I+a, 2J+b).
+// Multigrid kernels. Between structured levels child (a,b) of coarse cell
+// (I,J) is fine cell (2I+a, 2J+b); between levels of a locally refined mesh
+// the coarse level's child map and the fine level's parent map say which
+// cells were merged (DG embedding) and which are the same cell (identity).
 #include "mg.h"
 #include "wave.h"
 
@@ -43,13 +45,17 @@ __global__ void k_restrict(MeshDev mf, MeshDev mc, const T* __restrict__ fine, T
     const int comp = (int)(t / Nc);
     const long r = t - comp * Nc;
     const int cc = (int)(r / ND2), o = (int)(r - (long)cc * ND2), m = o / ND, n = o - m * ND;
-    const int I = cc % mc.nx, J = cc / mc.nx;
+    if (mc.child && mc.child[4 * cc + 1] < 0) {
+      coarse[t] = fine[comp * Nf + (long)mc.child[4 * cc] * ND2 + o];
+      continue;
+    }
+    const int I = mc.child ? 0 : cc % mc.nx, J = mc.child ? 0 : cc / mc.nx;
     double acc = 0;
 #pragma unroll
     for (int b = 0; b < 2; ++b)
 #pragma unroll
       for (int a = 0; a < 2; ++a) {
-        const long cf = (long)(2 * J + b) * mf.nx + 2 * I + a;
+        const long cf = mc.child ? (long)mc.child[4 * cc + a + 2 * b] : (long)(2 * J + b) * mf.nx + 2 * I + a;
         const T* u = fine + comp * Nf + cf * ND2;
 #pragma unroll
         for (int jj = 0; jj < ND; ++jj) {
@@ -72,8 +78,22 @@ __global__ void k_prolongate_add(MeshDev mf, MeshDev mc, const T* __restrict__ c
     const int comp = (int)(t / Nf);
     const long r = t - comp * Nf;
     const int cf = (int)(r / ND2), o = (int)(r - (long)cf * ND2), jj = o / ND, ii = o - jj * ND;
-    const int i = cf % mf.nx, j = cf / mf.nx;
-    const int cc = (j >> 1) * mc.nx + (i >> 1), a = i & 1, b = j & 1;
+    int cc, a, b;
+    if (mf.parent) {
+      cc = mf.parent[2 * cf];
+      const int code = mf.parent[2 * cf + 1];
+      if (code < 0) {
+        fine[t] += coarse[comp * Nc + (long)cc * ND2 + o];
+        continue;
+      }
+      a = code & 1;
+      b = code >> 1;
+    } else {
+      const int i = cf % mf.nx, j = cf / mf.nx;
+      cc = (j >> 1) * mc.nx + (i >> 1);
+      a = i & 1;
+      b = j & 1;
+    }
     const T* u = coarse + comp * Nc + (long)cc * ND2;
     double acc = 0;
 #pragma unroll
@@ -92,13 +112,17 @@ __global__ void k_project_down(MeshDev mf, MeshDev mc, const double* __restrict_
   const long tot = (long)mc.ncell * ND2;
   for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < tot; t += (long)gridDim.x * blockDim.x) {
     const int cc = (int)(t / ND2), o = (int)(t - (long)cc * ND2), m = o / ND, n = o - m * ND;
-    const int I = cc % mc.nx, J = cc / mc.nx;
+    if (mc.child && mc.child[4 * cc + 1] < 0) {
+      coarse[t] = fine[(long)mc.child[4 * cc] * ND2 + o];
+      continue;
+    }
+    const int I = mc.child ? 0 : cc % mc.nx, J = mc.child ? 0 : cc / mc.nx;
     double acc = 0;
 #pragma unroll
     for (int b = 0; b < 2; ++b)
 #pragma unroll
       for (int a = 0; a < 2; ++a) {
-        const long cf = (long)(2 * J + b) * mf.nx + 2 * I + a;
+        const long cf = mc.child ? (long)mc.child[4 * cc + a + 2 * b] : (long)(2 * J + b) * mf.nx + 2 * I + a;
         const double* u = fine + cf * ND2;
 #pragma unroll
         for (int jj = 0; jj < ND; ++jj) {

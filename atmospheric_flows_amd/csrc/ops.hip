@@ -78,9 +78,8 @@ __global__ __launch_bounds__(64) void k_inv_diag(MeshDev m, int ncomp, const T* 
   if (W) stage_cells<K>(s_r, rho, m.ncell);
   __syncthreads();
   if (L.active) {
-    const int ci = L.cell % m.nx, cj = L.cell / m.nx;
-    const CellVerts v = cell_verts(m, ci, cj);
-    const ColGeo<T> g = col_geo<K, T>(v, m.dx, L.c);
+    const CellVerts v = cell_verts(m, L.cell);
+    const ColGeo<T> g = col_geo<K, T>(v, L.c);
     T Sc[ND], Dc[ND];
     lane_rows<K, T>(L.c, Sc, Dc);
     T wq[NQ];
@@ -342,12 +341,12 @@ struct PhysRhsU {
   }
 };
 
-template <int K, class Ph>
+template <int K, class Ph, bool AMR>
 __global__ __launch_bounds__(64) void k_cellface(MeshDev m, Ph ph, FieldPtrs<Ph::NF> in, double* __restrict__ dst,
                                                  const double* __restrict__ scale, long Ns) {
   constexpr int ND2 = Cfg<K>::ND2, CPW = Cfg<K>::CPW;
-  __shared__ CFSmem<K, Ph> sh;
-  cellface_columns<K, Ph>(m, ph, in, sh);
+  __shared__ CFSmem<K, Ph, AMR> sh;
+  cellface_columns<K, Ph, AMR>(m, ph, in, sh);
   const long base = (long)blockIdx.x * CPW * ND2;
   const long lim = (long)m.ncell * ND2;
   OutRows<K> R = out_rows<K>(threadIdx.x);
@@ -357,7 +356,7 @@ __global__ __launch_bounds__(64) void k_cellface(MeshDev m, Ph ph, FieldPtrs<Ph:
     if (!rows_invariant<K>()) R = out_rows<K>(t);
 #pragma unroll
     for (int o = 0; o < Ph::NO; ++o) {
-      double acc = cellface_out<K, Ph>(sh, o, t, R);
+      double acc = cellface_out<K, Ph, AMR>(sh, o, t, R);
       if (scale) acc *= scale[o * Ns + gi];
       dst[o * Ns + gi] = acc;
     }
@@ -369,7 +368,10 @@ static void launch_cf(const MeshDev& m, const Ph& ph, const FieldPtrs<Ph::NF>& i
                       hipStream_t s, const double* scale = nullptr) {
   const int grid = ceil_div(m.ncell, Cfg<K>::CPW);
   if (grid == 0) return;
-  hipLaunchKernelGGL((k_cellface<K, Ph>), dim3(grid), dim3(64), 0, s, m, ph, in, dst, scale, Ns);
+  if (m.cface)
+    hipLaunchKernelGGL((k_cellface<K, Ph, true>), dim3(grid), dim3(64), 0, s, m, ph, in, dst, scale, Ns);
+  else
+    hipLaunchKernelGGL((k_cellface<K, Ph, false>), dim3(grid), dim3(64), 0, s, m, ph, in, dst, scale, Ns);
   ATMO_HIP_CHECK(hipGetLastError());
 }
 

@@ -10,20 +10,20 @@
 
 namespace atmo {
 
-template <int K>
+template <int K, bool AMR>
 __global__ __launch_bounds__(64) void k_b_ainv(MeshDev m, PhysB ph, const double* __restrict__ p,
                                                const double* __restrict__ Ainv, double* __restrict__ y, long Ns) {
   constexpr int ND2 = Cfg<K>::ND2, CPW = Cfg<K>::CPW;
-  __shared__ CFSmem<K, PhysB> sh;
+  __shared__ CFSmem<K, PhysB, AMR> sh;
   __shared__ double s_t[2][CPW * ND2];
   FieldPtrs<1> in{{p}};
-  cellface_columns<K, PhysB>(m, ph, in, sh);
+  cellface_columns<K, PhysB, AMR>(m, ph, in, sh);
   const int cell0 = blockIdx.x * CPW;
   OutRows<K> R = out_rows<K>(threadIdx.x);
   for (int t = threadIdx.x; t < CPW * ND2; t += 64) {
     if (!rows_invariant<K>()) R = out_rows<K>(t);
-    s_t[0][t] = cellface_out<K, PhysB>(sh, 0, t, R);
-    s_t[1][t] = cellface_out<K, PhysB>(sh, 1, t, R);
+    s_t[0][t] = cellface_out<K, PhysB, AMR>(sh, 0, t, R);
+    s_t[1][t] = cellface_out<K, PhysB, AMR>(sh, 1, t, R);
   }
   __syncthreads();
   for (int t = threadIdx.x; t < CPW * ND2; t += 64) {
@@ -82,7 +82,10 @@ void launch_b_ainv(const MeshDev& m, const OpConsts& oc, double coeff, const dou
   if (grid == 0) return;
   PhysB ph;
   ph.cf = coeff * oc.dt / (oc.Ma * oc.Ma);
-  hipLaunchKernelGGL((k_b_ainv<K>), dim3(grid), dim3(64), 0, s, m, ph, p, Ainv, y, Ns);
+  if (m.cface)
+    hipLaunchKernelGGL((k_b_ainv<K, true>), dim3(grid), dim3(64), 0, s, m, ph, p, Ainv, y, Ns);
+  else
+    hipLaunchKernelGGL((k_b_ainv<K, false>), dim3(grid), dim3(64), 0, s, m, ph, p, Ainv, y, Ns);
   ATMO_HIP_CHECK(hipGetLastError());
 }
 

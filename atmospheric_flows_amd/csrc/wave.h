@@ -28,9 +28,8 @@ __device__ inline void column_weights(const MeshDev& m, int cell, int c, int cw,
                                       T (&wq)[Cfg<K>::NQ], const T (&Sc)[Cfg<K>::ND]) {
   constexpr int ND = Cfg<K>::ND, NQ = Cfg<K>::NQ, ND2 = Cfg<K>::ND2;
   constexpr auto& tb = basis::kT<K>;
-  const int ci = cell % m.nx, cj = cell / m.nx;
-  const CellVerts v = cell_verts(m, ci, cj);
-  const ColGeo<T> g = col_geo<K, T>(v, m.dx, c);
+  const CellVerts v = cell_verts(m, cell);
+  const ColGeo<T> g = col_geo<K, T>(v, c);
   const T jc = g.dx * g.ze * g.wc * coeff;
 #pragma unroll
   for (int r = 0; r < NQ; ++r) wq[r] = jc * (T)tb.gw[r];

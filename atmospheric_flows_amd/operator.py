@@ -65,14 +65,23 @@ class DeviceVector:
 
 
 class DeviceContext:
-    def __init__(self, degree, nx, nz, Lx=40000.0, H=20000.0, h=None, periodic=True, **params):
+    """One simulation on the GPU. `leaves` (optional, int array (n, 5) of
+    (bi, bj, level, sx, sz)) makes the mesh a locally refined one over the
+    nx x nz base mesh (atmo_ctx_create_amr)."""
+
+    def __init__(self, degree, nx, nz, Lx=40000.0, H=20000.0, h=None, periodic=True, leaves=None, **params):
         L = N.load()
         self.degree, self.nx, self.nz = degree, nx, nz
         self._h = np.zeros(nx + 1) if h is None else np.ascontiguousarray(h, dtype=np.float64)
         self.mesh = N.MeshDesc(degree, nx, nz, 1 if periodic else 0, Lx, H, _ptr(self._h))
         self.params = default_params(**params)
         h_ = ctypes.c_void_p()
-        N.check(L.atmo_ctx_create(ctypes.byref(self.mesh), ctypes.byref(self.params), ctypes.byref(h_)))
+        if leaves is None:
+            N.check(L.atmo_ctx_create(ctypes.byref(self.mesh), ctypes.byref(self.params), ctypes.byref(h_)))
+        else:
+            lv = np.ascontiguousarray(leaves, dtype=np.int32).reshape(-1, 5)
+            N.check(L.atmo_ctx_create_amr(ctypes.byref(self.mesh), lv.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                          lv.shape[0], ctypes.byref(self.params), ctypes.byref(h_)))
         self.handle = h_
         nc, ns, nl = ctypes.c_int(), ctypes.c_long(), ctypes.c_int()
         L.atmo_ctx_info(self.handle, ctypes.byref(nc), ctypes.byref(ns), ctypes.byref(nl))
@@ -104,8 +113,8 @@ class DeviceContext:
 
     def init_hill(self, theta0=300.0, p0=1e5, N_bv=0.01, U=10.0, g=9.81, R=287.0, gamma=1.4, pert=1e-4, seed=1234):
         rho, u, p = np.zeros(self.ns), np.zeros(2 * self.ns), np.zeros(self.ns)
-        N.check(N.load().atmo_init_hill_state(ctypes.byref(self.mesh), theta0, p0, N_bv, U, g, R, gamma, pert, seed,
-                                              _ptr(rho), _ptr(u), _ptr(p)))
+        N.check(N.load().atmo_ctx_init_hill_state(self.handle, theta0, p0, N_bv, U, g, R, gamma, pert, seed,
+                                                  _ptr(rho), _ptr(u), _ptr(p)), self.handle)
         return rho, u, p
 
     def upload_state(self, rho, u, p):

@@ -95,7 +95,23 @@ const char* atmo_last_error(const atmo_ctx* ctx);
 /* EulerSolver constructor part: mesh, MG hierarchy, device buffers
  * (Atmospheric_Flow.cc:217-272, setup_dofs :296-409). */
 atmo_status atmo_ctx_create(const atmo_mesh_desc* mesh, const atmo_params* params, atmo_ctx** out);
+/* Locally refined mesh over the base mesh `base` (refine_global replaced by
+ * local quadtree refinement; the reference's triangulation is created with
+ * limit_level_difference_at_vertices + construct_multigrid_hierarchy,
+ * Atmospheric_Flow.cc:225-226, 286). leaves[5*c .. 5*c+4] = (bi, bj, level,
+ * sx, sz) of cell c: base cell (bi, bj), quadtree level, position in
+ * [0, 2^level)^2; listed base cell by base cell in the order bj*nx + bi; the
+ * cell order of every field. Neighbours may differ by one level at most.
+ * Hanging faces are integrated on the smaller side's face (deal.II's
+ * subfaces); the multigrid levels coarsen globally down to the base mesh. */
+atmo_status atmo_ctx_create_amr(const atmo_mesh_desc* base, const int* leaves, int nleaves, const atmo_params* params,
+                                atmo_ctx** out);
 atmo_status atmo_ctx_destroy(atmo_ctx* ctx);
+/* Hill-flow initial state on the context's own mesh (structured or locally
+ * refined), as atmo_init_hill_state. */
+atmo_status atmo_ctx_init_hill_state(atmo_ctx* ctx, double theta0, double p0, double N, double U, double g, double R,
+                                     double gamma_gas, double perturbation, uint64_t seed, double* rho, double* u,
+                                     double* p);
 atmo_status atmo_set_params(atmo_ctx* ctx, const atmo_params* params);
 atmo_status atmo_ctx_info(atmo_ctx* ctx, int* ncells, long* scalar_dofs, int* nlevels);
 atmo_status atmo_sync(atmo_ctx* ctx);

@@ -26,6 +26,8 @@
 //   (deal.II internals not importable here) -> "parity unpinned" vs deal.II.
 
 #include <algorithm>
+#include <array>
+#include <unordered_map>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -50,6 +52,7 @@ struct Tables {
   Vec gll, gq, gw;  // GLL nodes (nd) / Gauss points, weights (nq) on [0,1]
   Vec S, Dv;        // S[q*nd+i] = l_i(gq[q]), Dv[q*nd+i] = l_i'(gq[q])
   Vec f0, f1;       // l_i(0), l_i(1)
+  Vec Sh[2];        // Sh[h][q*nd+i] = l_i((h + gq[q])/2): half h of a face split by refinement
 };
 
 static void legendre(int n, ld z, ld& p, ld& pm1) {
@@ -144,6 +147,11 @@ static Tables make_tables(int k) {
     T.f0[i] = lagrange(T.gll, i, 0.0);
     T.f1[i] = lagrange(T.gll, i, 1.0);
   }
+  for (int h = 0; h < 2; ++h) {
+    T.Sh[h].resize(T.nq * T.nd);
+    for (int q = 0; q < T.nq; ++q)
+      for (int i = 0; i < T.nd; ++i) T.Sh[h][q * T.nd + i] = lagrange(T.gll, i, 0.5 * (h + T.gq[q]));
+  }
   return T;
 }
 
@@ -151,12 +159,30 @@ static Tables make_tables(int k) {
 // Structured quad mesh with bilinear cells. Vertex (i,j) at X/Z[j*(nx+1)+i].
 // Cell (i,j) has index j*nx+i. Bottom/top are walls; left/right are
 // periodic or walls.
+//
+// Unstructured form (locally refined meshes): per-cell vertices, the inner
+// faces as (c+, f+, c-, f-) with the "+" side always a full face and the
+// "-" side either a full face or half h of a face split by refinement,
+// encoded as f- + 4*(1+h) (deal.II's subface of the coarse neighbour), and
+// the boundary faces (c, f).
 struct Mesh {
   int nx = 0, nz = 0;
   bool periodic = true;
   Vec X, Z;
-  int ncell() const { return nx * nz; }
+  bool unstructured = false;
+  int nc = 0;
+  Vec cx, cz;                              // 4 vertices per cell (unstructured)
+  std::vector<std::array<int, 4>> inner;   // (c+, f+, c-, f- encoded)
+  std::vector<std::array<int, 2>> bdry;    // (c, f)
+  int ncell() const { return unstructured ? nc : nx * nz; }
   void verts(int c, double x[4], double z[4]) const {
+    if (unstructured) {
+      for (int a = 0; a < 4; ++a) {
+        x[a] = cx[4 * c + a];
+        z[a] = cz[4 * c + a];
+      }
+      return;
+    }
     int i = c % nx, j = c / nx;
     int w = nx + 1;
     int id[4] = {j * w + i, j * w + i + 1, (j + 1) * w + i, (j + 1) * w + i + 1};
@@ -179,6 +205,98 @@ static Mesh make_terrain_mesh(int nx, int nz, bool periodic, double Lx, double H
       m.X[j * (nx + 1) + i] = Lx * (double)i / (double)nx;
       m.Z[j * (nx + 1) + i] = hx[i] + (H - hx[i]) * (double)j / (double)nz;
     }
+  return m;
+}
+
+// ------------------------------------------ locally refined (quadtree) mesh
+// A cell of a locally refined mesh: base cell (bi, bj) of the nx0 x nz0
+// terrain-following mesh, refinement level l and position (sx, sz) in
+// [0, 2^l)^2 inside it. Its vertices are the images of the reference
+// sub-box under the base cell's bilinear map (the children deal.II creates
+// for straight-edged cells).
+struct QCell {
+  int bi, bj, l, sx, sz;
+};
+
+static uint64_t qkey(int tree, int l, int sx, int sz) {
+  return ((uint64_t)tree << 40) | ((uint64_t)l << 32) | ((uint64_t)sx << 16) | (uint64_t)sz;
+}
+
+struct QuadMeshSpec {
+  int nx0, nz0;
+  bool periodic;
+  double Lx, H;
+  Vec hx;
+};
+
+static Mesh quad_mesh(const QuadMeshSpec& q, const std::vector<QCell>& cells, bool with_faces) {
+  Mesh m;
+  m.unstructured = true;
+  m.nx = q.nx0;
+  m.nz = q.nz0;
+  m.periodic = q.periodic;
+  m.nc = (int)cells.size();
+  m.cx.resize(4 * m.nc);
+  m.cz.resize(4 * m.nc);
+  std::unordered_map<uint64_t, int> idx;
+  for (int c = 0; c < m.nc; ++c) {
+    const QCell& e = cells[c];
+    if (e.bi < 0 || e.bi >= q.nx0 || e.bj < 0 || e.bj >= q.nz0 || e.l < 0 || e.l > 15 || e.sx < 0 || e.sz < 0 ||
+        e.sx >= (1 << e.l) || e.sz >= (1 << e.l))
+      throw std::invalid_argument("quad mesh: cell out of range");
+    if (!idx.emplace(qkey(e.bj * q.nx0 + e.bi, e.l, e.sx, e.sz), c).second)
+      throw std::invalid_argument("quad mesh: duplicate cell");
+    const double n = (double)(1 << e.l);
+    const double xl = q.Lx * e.bi / q.nx0, xr = q.Lx * (e.bi + 1) / q.nx0;
+    const double z00 = q.hx[e.bi] + (q.H - q.hx[e.bi]) * e.bj / q.nz0;
+    const double z10 = q.hx[e.bi + 1] + (q.H - q.hx[e.bi + 1]) * e.bj / q.nz0;
+    const double z01 = q.hx[e.bi] + (q.H - q.hx[e.bi]) * (e.bj + 1) / q.nz0;
+    const double z11 = q.hx[e.bi + 1] + (q.H - q.hx[e.bi + 1]) * (e.bj + 1) / q.nz0;
+    for (int v = 0; v < 4; ++v) {
+      const double xi = (e.sx + (v & 1)) / n, eta = (e.sz + (v >> 1)) / n;
+      m.cx[4 * c + v] = xl + xi * (xr - xl);
+      m.cz[4 * c + v] = (1 - xi) * (1 - eta) * z00 + xi * (1 - eta) * z10 + (1 - xi) * eta * z01 + xi * eta * z11;
+    }
+  }
+  if (!with_faces) return m;
+  const int opp[4] = {1, 0, 3, 2};
+  for (int c = 0; c < m.nc; ++c) {
+    const QCell& e = cells[c];
+    const int n = 1 << e.l;
+    for (int f = 0; f < 4; ++f) {
+      int bi = e.bi, bj = e.bj, sx = e.sx, sz = e.sz;
+      if (f == 0 && --sx < 0) { sx = n - 1; --bi; }
+      if (f == 1 && ++sx >= n) { sx = 0; ++bi; }
+      if (f == 2 && --sz < 0) { sz = n - 1; --bj; }
+      if (f == 3 && ++sz >= n) { sz = 0; ++bj; }
+      if (bj < 0 || bj >= q.nz0 || ((bi < 0 || bi >= q.nx0) && !q.periodic)) {
+        m.bdry.push_back({c, f});
+        continue;
+      }
+      bi = (bi + q.nx0) % q.nx0;
+      const int tree = bj * q.nx0 + bi;
+      auto it = idx.find(qkey(tree, e.l, sx, sz));
+      if (it != idx.end()) {
+        if (f == 1 || f == 3) m.inner.push_back({c, f, it->second, opp[f]});
+        continue;
+      }
+      if (e.l > 0) {
+        auto ip = idx.find(qkey(tree, e.l - 1, sx / 2, sz / 2));
+        if (ip != idx.end()) {
+          const int half = f < 2 ? (e.sz & 1) : (e.sx & 1);
+          m.inner.push_back({c, f, ip->second, opp[f] + 4 * (1 + half)});
+          continue;
+        }
+      }
+      // the neighbour is refined once more: its two cells along this face register the face
+      for (int a = 0; a < 2; ++a) {
+        const int cx = f < 2 ? 2 * sx + (f == 0 ? 1 : 0) : 2 * sx + a;
+        const int cz = f < 2 ? 2 * sz + a : 2 * sz + (f == 2 ? 1 : 0);
+        if (!idx.count(qkey(tree, e.l + 1, cx, cz)))
+          throw std::invalid_argument("quad mesh: not a conforming 2:1-balanced cover of the base mesh");
+      }
+    }
+  }
   return m;
 }
 
@@ -308,20 +426,27 @@ static void integrate(const Tables& T, const double* V, const double* Gxi, const
     }
 }
 
+// Lagrange value along face f at face point s: the full face (f < 4) or half
+// h of it (f = face + 4*(1+h)), whose points are (h + gq[s])/2 of the face.
+static double along(const Tables& T, int f, int s, int idx) {
+  const int h = (f >> 2) - 1;
+  return h < 0 ? T.S[s * T.nd + idx] : T.Sh[h][s * T.nd + idx];
+}
+
 // trace at face q-points of dofs u on face f (generic: uses l_i(0), l_i(1))
 static void face_values(const Tables& T, int f, const double* u, double* v) {
-  const int nd = T.nd, nq = T.nq;
+  const int nd = T.nd, nq = T.nq, fb = f & 3;
   for (int s = 0; s < nq; ++s) {
     double acc = 0;
     for (int j = 0; j < nd; ++j)
       for (int i = 0; i < nd; ++i) {
         double bx, by;
-        if (f < 2) {
-          bx = (f == 0) ? T.f0[i] : T.f1[i];
-          by = T.S[s * nd + j];
+        if (fb < 2) {
+          bx = (fb == 0) ? T.f0[i] : T.f1[i];
+          by = along(T, f, s, j);
         } else {
-          bx = T.S[s * nd + i];
-          by = (f == 2) ? T.f0[j] : T.f1[j];
+          bx = along(T, f, s, i);
+          by = (fb == 2) ? T.f0[j] : T.f1[j];
         }
         acc += bx * by * u[j * nd + i];
       }
@@ -331,18 +456,18 @@ static void face_values(const Tables& T, int f, const double* u, double* v) {
 
 // out += sum_s phi_i(face point s) * val[s] (val carries face JxW)
 static void face_integrate(const Tables& T, int f, const double* val, double* out) {
-  const int nd = T.nd, nq = T.nq;
+  const int nd = T.nd, nq = T.nq, fb = f & 3;
   for (int j = 0; j < nd; ++j)
     for (int i = 0; i < nd; ++i) {
       double acc = 0;
       for (int s = 0; s < nq; ++s) {
         double bx, by;
-        if (f < 2) {
-          bx = (f == 0) ? T.f0[i] : T.f1[i];
-          by = T.S[s * nd + j];
+        if (fb < 2) {
+          bx = (fb == 0) ? T.f0[i] : T.f1[i];
+          by = along(T, f, s, j);
         } else {
-          bx = T.S[s * nd + i];
-          by = (f == 2) ? T.f0[j] : T.f1[j];
+          bx = along(T, f, s, i);
+          by = (fb == 2) ? T.f0[j] : T.f1[j];
         }
         acc += bx * by * val[s];
       }
@@ -391,6 +516,22 @@ static void run_loop(const Disc& d, const CellFn& cf, const FaceFn& ff, const Bd
       cell_geometry(d.T, m, c, g);
       cf(c, g);
     }
+  }
+  if (m.unstructured) {
+    // face lists in order, serially (two faces of one cell may be adjacent in the list)
+    if (ff)
+      for (const auto& fc : m.inner) {
+        FaceGeo g;
+        face_geometry(d.T, m, fc[0], fc[1], g);
+        ff(fc[0], fc[1], fc[2], fc[3], g);
+      }
+    if (bf)
+      for (const auto& b : m.bdry) {
+        FaceGeo g;
+        face_geometry(d.T, m, b[0], b[1], g);
+        bf(b[0], b[1], g);
+      }
+    return;
   }
   if (ff) {
     // vertical inner faces: each row j touches only row-j cells
@@ -1146,8 +1287,47 @@ static void tridiag_extreme(const std::vector<double>& a, const std::vector<doub
 }
 
 // ------------------------------------------------------------ multigrid
+// Child map of a coarse level into the next finer level: 4 entries per
+// coarse cell, either (f, -1, -1, -1) for a cell that is also a cell of the
+// finer level, or the four children, child (a, b) at entry a + 2b.
+typedef std::vector<int> ChildMap;
+
+// Global coarsening of a quadtree cell list: every cell whose three siblings
+// are also cells of the level is replaced by its parent, which takes the
+// place of the first of its children in the order.
+static std::vector<QCell> quad_coarsen(int nx0, const std::vector<QCell>& cells, ChildMap& ch) {
+  std::unordered_map<uint64_t, int> count;
+  for (const QCell& e : cells)
+    if (e.l > 0) ++count[qkey(e.bj * nx0 + e.bi, e.l - 1, e.sx / 2, e.sz / 2)];
+  std::vector<QCell> out;
+  std::unordered_map<uint64_t, int> parent_at;
+  ch.clear();
+  for (int c = 0; c < (int)cells.size(); ++c) {
+    const QCell& e = cells[c];
+    const uint64_t pk = e.l > 0 ? qkey(e.bj * nx0 + e.bi, e.l - 1, e.sx / 2, e.sz / 2) : 0;
+    if (e.l > 0 && count[pk] == 4) {
+      auto it = parent_at.find(pk);
+      int pc;
+      if (it == parent_at.end()) {
+        pc = (int)out.size();
+        parent_at[pk] = pc;
+        out.push_back({e.bi, e.bj, e.l - 1, e.sx / 2, e.sz / 2});
+        ch.insert(ch.end(), {-1, -1, -1, -1});
+      } else {
+        pc = it->second;
+      }
+      ch[4 * pc + (e.sx & 1) + 2 * (e.sz & 1)] = c;
+    } else {
+      out.push_back(e);
+      ch.insert(ch.end(), {c, -1, -1, -1});
+    }
+  }
+  return out;
+}
+
 struct Level {
   Mesh m;
+  ChildMap child;  // children of this level's cells in the next finer level (empty on the finest)
   Disc* d = nullptr;
   EulerOp* op = nullptr;
   Vec invdiag;
@@ -1214,74 +1394,101 @@ static Transfer1D make_transfer(const Tables& T) {
   return t;
 }
 
-// coarse (Mc) -> fine (Mf) prolongation, added into fine; ncomp blocks
-static void prolongate_add(const Transfer1D& t, const Mesh& mf, const Mesh& mc, Vec& fine, const Vec& coarse, int ncomp) {
-  const int nd = t.nd, nd2 = nd * nd;
-  const int Nf = mf.ncell() * nd2, Nc = mc.ncell() * nd2;
-#pragma omp parallel for schedule(static)
-  for (int cf = 0; cf < mf.ncell(); ++cf) {
-    int i = cf % mf.nx, j = cf / mf.nx;
-    int cc = (j / 2) * mc.nx + i / 2;
-    int a = i & 1, b = j & 1;
-    for (int comp = 0; comp < ncomp; ++comp) {
-      const double* u = &coarse[comp * Nc + cc * nd2];
-      double* o = &fine[comp * Nf + cf * nd2];
-      for (int jj = 0; jj < nd; ++jj)
-        for (int ii = 0; ii < nd; ++ii) {
-          double s = 0;
-          for (int m = 0; m < nd; ++m)
-            for (int n = 0; n < nd; ++n) s += t.P[b][jj * nd + m] * t.P[a][ii * nd + n] * u[m * nd + n];
-          o[jj * nd + ii] += s;
-        }
-    }
+
+static ChildMap structured_children(const Mesh& mf, const Mesh& mc) {
+  ChildMap ch(4 * mc.ncell());
+  for (int cc = 0; cc < mc.ncell(); ++cc) {
+    const int I = cc % mc.nx, J = cc / mc.nx;
+    for (int b = 0; b < 2; ++b)
+      for (int a = 0; a < 2; ++a) ch[4 * cc + a + 2 * b] = (2 * J + b) * mf.nx + 2 * I + a;
   }
+  return ch;
+}
+
+// coarse -> fine prolongation (DG embedding), added into fine; ncomp blocks
+static void prolongate_add(const Transfer1D& t, const ChildMap& ch, int ncf, int ncc, Vec& fine, const Vec& coarse,
+                           int ncomp) {
+  const int nd = t.nd, nd2 = nd * nd;
+  const int Nf = ncf * nd2, Nc = ncc * nd2;
+#pragma omp parallel for schedule(static)
+  for (int cc = 0; cc < ncc; ++cc)
+    for (int k = 0; k < 4; ++k) {
+      const int cf = ch[4 * cc + k];
+      if (cf < 0) continue;
+      const bool same = ch[4 * cc + 1] < 0;
+      const int a = k & 1, b = k >> 1;
+      for (int comp = 0; comp < ncomp; ++comp) {
+        const double* u = &coarse[comp * Nc + cc * nd2];
+        double* o = &fine[comp * Nf + cf * nd2];
+        for (int jj = 0; jj < nd; ++jj)
+          for (int ii = 0; ii < nd; ++ii) {
+            if (same) {
+              o[jj * nd + ii] += u[jj * nd + ii];
+              continue;
+            }
+            double s = 0;
+            for (int m = 0; m < nd; ++m)
+              for (int n = 0; n < nd; ++n) s += t.P[b][jj * nd + m] * t.P[a][ii * nd + n] * u[m * nd + n];
+            o[jj * nd + ii] += s;
+          }
+      }
+    }
 }
 
 // transpose of prolongation: coarse += P^T fine
-static void restrict_add(const Transfer1D& t, const Mesh& mf, const Mesh& mc, Vec& coarse, const Vec& fine, int ncomp) {
+static void restrict_add(const Transfer1D& t, const ChildMap& ch, int ncf, int ncc, Vec& coarse, const Vec& fine,
+                         int ncomp) {
   const int nd = t.nd, nd2 = nd * nd;
-  const int Nf = mf.ncell() * nd2, Nc = mc.ncell() * nd2;
+  const int Nf = ncf * nd2, Nc = ncc * nd2;
 #pragma omp parallel for schedule(static)
-  for (int cc = 0; cc < mc.ncell(); ++cc) {
-    int I = cc % mc.nx, J = cc / mc.nx;
-    for (int comp = 0; comp < ncomp; ++comp) {
-      double* o = &coarse[comp * Nc + cc * nd2];
-      for (int b = 0; b < 2; ++b)
-        for (int a = 0; a < 2; ++a) {
-          int cf = (2 * J + b) * mf.nx + 2 * I + a;
-          const double* r = &fine[comp * Nf + cf * nd2];
-          for (int m = 0; m < nd; ++m)
-            for (int n = 0; n < nd; ++n) {
-              double s = 0;
-              for (int jj = 0; jj < nd; ++jj)
-                for (int ii = 0; ii < nd; ++ii) s += t.P[b][jj * nd + m] * t.P[a][ii * nd + n] * r[jj * nd + ii];
-              o[m * nd + n] += s;
-            }
-        }
-    }
-  }
-}
-
-// L2 projection fine -> coarse (interpolate_to_mg for the level densities)
-static void project_down(const Transfer1D& t, const Mesh& mf, const Mesh& mc, Vec& coarse, const Vec& fine) {
-  const int nd = t.nd, nd2 = nd * nd;
-  coarse.assign(mc.ncell() * nd2, 0.0);
-#pragma omp parallel for schedule(static)
-  for (int cc = 0; cc < mc.ncell(); ++cc) {
-    int I = cc % mc.nx, J = cc / mc.nx;
-    double* o = &coarse[cc * nd2];
-    for (int b = 0; b < 2; ++b)
-      for (int a = 0; a < 2; ++a) {
-        int cf = (2 * J + b) * mf.nx + 2 * I + a;
-        const double* r = &fine[cf * nd2];
+  for (int cc = 0; cc < ncc; ++cc)
+    for (int k = 0; k < 4; ++k) {
+      const int cf = ch[4 * cc + k];
+      if (cf < 0) continue;
+      const bool same = ch[4 * cc + 1] < 0;
+      const int a = k & 1, b = k >> 1;
+      for (int comp = 0; comp < ncomp; ++comp) {
+        double* o = &coarse[comp * Nc + cc * nd2];
+        const double* r = &fine[comp * Nf + cf * nd2];
         for (int m = 0; m < nd; ++m)
           for (int n = 0; n < nd; ++n) {
+            if (same) {
+              o[m * nd + n] += r[m * nd + n];
+              continue;
+            }
             double s = 0;
             for (int jj = 0; jj < nd; ++jj)
-              for (int ii = 0; ii < nd; ++ii) s += t.R[b][m * nd + jj] * t.R[a][n * nd + ii] * r[jj * nd + ii];
+              for (int ii = 0; ii < nd; ++ii) s += t.P[b][jj * nd + m] * t.P[a][ii * nd + n] * r[jj * nd + ii];
             o[m * nd + n] += s;
           }
       }
+    }
+}
+
+// L2 projection fine -> coarse (interpolate_to_mg for the level densities)
+static void project_down(const Transfer1D& t, const ChildMap& ch, int ncc, Vec& coarse, const Vec& fine) {
+  const int nd = t.nd, nd2 = nd * nd;
+  coarse.assign((size_t)ncc * nd2, 0.0);
+#pragma omp parallel for schedule(static)
+  for (int cc = 0; cc < ncc; ++cc) {
+    double* o = &coarse[cc * nd2];
+    for (int k = 0; k < 4; ++k) {
+      const int cf = ch[4 * cc + k];
+      if (cf < 0) continue;
+      const double* r = &fine[cf * nd2];
+      if (ch[4 * cc + 1] < 0) {
+        for (int i = 0; i < nd2; ++i) o[i] = r[i];
+        continue;
+      }
+      const int a = k & 1, b = k >> 1;
+      for (int m = 0; m < nd; ++m)
+        for (int n = 0; n < nd; ++n) {
+          double s = 0;
+          for (int jj = 0; jj < nd; ++jj)
+            for (int ii = 0; ii < nd; ++ii) s += t.R[b][m * nd + jj] * t.R[a][n * nd + ii] * r[jj * nd + ii];
+          o[m * nd + n] += s;
+        }
+    }
   }
 }
 
@@ -1300,19 +1507,21 @@ struct Solver {
   Transfer1D tr;
   Stats stats;
 
-  Solver(int k_, const Mesh& m, const Params& p) : P(p), k(k_) {
-    fine = new Disc(k, m);
+  // ms: the MG hierarchy, finest first; maps[i]: children of ms[i+1]'s cells in ms[i]
+  Solver(int k_, std::vector<Mesh> ms, std::vector<ChildMap> maps, const Params& p) : P(p), k(k_) {
+    fine = new Disc(k, ms[0]);
     op = new EulerOp(*fine, P);
-    // MG hierarchy by uniform coarsening of the structured mesh
-    std::vector<Mesh> ms;
-    ms.push_back(m);
-    while (ms.back().nx % 2 == 0 && ms.back().nz % 2 == 0 && ms.back().nx >= 4 &&
-           (ms.back().ncell() / 4) >= P.mg_min_cells)
+    // continue by uniform coarsening while the coarsest level is structured
+    while (!ms.back().unstructured && ms.back().nx % 2 == 0 && ms.back().nz % 2 == 0 && ms.back().nx >= 4 &&
+           (ms.back().ncell() / 4) >= P.mg_min_cells) {
       ms.push_back(coarsen(ms.back()));
+      maps.push_back(structured_children(ms[ms.size() - 2], ms.back()));
+    }
     int L = (int)ms.size();
     lv.resize(L);
     for (int l = 0; l < L; ++l) {
       lv[l].m = ms[L - 1 - l];
+      if (l < L - 1) lv[l].child = maps[L - 2 - l];
       lv[l].d = new Disc(k, lv[l].m);
       lv[l].op = new EulerOp(*lv[l].d, P);
     }
@@ -1444,9 +1653,9 @@ struct Solver {
     lvmult(l, t, x);
     for (size_t i = 0; i < t.size(); ++i) t[i] = b[i] - t[i];
     Vec bc((size_t)nc * lv[l - 1].d->Ns, 0.0), xc(bc.size(), 0.0);
-    restrict_add(tr, lv[l].m, lv[l - 1].m, bc, t, nc);
+    restrict_add(tr, lv[l - 1].child, lv[l].m.ncell(), lv[l - 1].m.ncell(), bc, t, nc);
     vcycle(l - 1, xc, bc, coarse_tol, ns);
-    prolongate_add(tr, lv[l].m, lv[l - 1].m, x, xc, nc);
+    prolongate_add(tr, lv[l - 1].child, lv[l].m.ncell(), lv[l - 1].m.ncell(), x, xc, nc);
     chebyshev(l, x, b, false);
   }
 
@@ -1460,7 +1669,8 @@ struct Solver {
   void set_level_rho(const Vec& rho_fine) {
     int L = (int)lv.size();
     lv[L - 1].op->rho_for_fixed = rho_fine;
-    for (int l = L - 1; l > 0; --l) project_down(tr, lv[l].m, lv[l - 1].m, lv[l - 1].op->rho_for_fixed, lv[l].op->rho_for_fixed);
+    for (int l = L - 1; l > 0; --l)
+      project_down(tr, lv[l - 1].child, lv[l - 1].m.ncell(), lv[l - 1].op->rho_for_fixed, lv[l].op->rho_for_fixed);
   }
 
   // CG + MG solve on the mass-type block ns (update_density / update_velocity
@@ -1688,9 +1898,53 @@ void* orc_create(int k, int nx, int nz, int periodic, double Lx, double H, const
     Params p;
     p.mg_min_cells = mg_min_cells;
     OrcHandle* h = new OrcHandle;
-    h->s = new Solver(k, m, p);
+    h->s = new Solver(k, {m}, {}, p);
     return h;
   } catch (...) {
+    return nullptr;
+  }
+}
+
+// Locally refined mesh: leaves[5*c..] = (bi, bj, level, sx, sz) of cell c,
+// listed base cell by base cell in the base mesh's order (bj*nx0 + bi).
+// The multigrid hierarchy coarsens globally down to the base mesh, then
+// uniformly as for a structured mesh.
+void* orc_create_amr(int k, int nx0, int nz0, int periodic, double Lx, double H, const double* hx, const int* leaves,
+                     int nleaves, int mg_min_cells) {
+  try {
+    QuadMeshSpec q{nx0, nz0, periodic != 0, Lx, H, Vec(hx, hx + nx0 + 1)};
+    std::vector<QCell> cells(nleaves);
+    for (int c = 0; c < nleaves; ++c)
+      cells[c] = {leaves[5 * c], leaves[5 * c + 1], leaves[5 * c + 2], leaves[5 * c + 3], leaves[5 * c + 4]};
+    std::vector<Mesh> ms;
+    std::vector<ChildMap> maps;
+    ms.push_back(quad_mesh(q, cells, true));
+    std::vector<QCell> cur = cells;
+    while (true) {
+      bool refined = false;
+      for (const QCell& e : cur) refined |= e.l > 0;
+      if (!refined) break;
+      ChildMap ch;
+      cur = quad_coarsen(nx0, cur, ch);
+      maps.push_back(ch);
+      refined = false;
+      for (const QCell& e : cur) refined |= e.l > 0;
+      if (refined) {
+        ms.push_back(quad_mesh(q, cur, false));
+      } else {
+        for (int c = 0; c < (int)cur.size(); ++c)
+          if (cur[c].bj * nx0 + cur[c].bi != c) throw std::invalid_argument("quad mesh: leaves not in base-cell order");
+        if ((int)cur.size() != nx0 * nz0) throw std::invalid_argument("quad mesh: leaves do not cover the base mesh");
+        ms.push_back(make_terrain_mesh(nx0, nz0, periodic != 0, Lx, H, hx));
+      }
+    }
+    Params p;
+    p.mg_min_cells = mg_min_cells;
+    OrcHandle* h = new OrcHandle;
+    h->s = new Solver(k, ms, maps, p);
+    return h;
+  } catch (std::exception& e) {
+    std::fprintf(stderr, "orc_create_amr: %s\n", e.what());
     return nullptr;
   }
 }
@@ -1853,13 +2107,18 @@ int orc_solve_mass_mg(void* hp, int ns, double* x, const double* b, double tol) 
 
 // MG prolongation / restriction between the two finest levels (for tests)
 int orc_mg_level_cells(void* hp, int l) { return ((OrcHandle*)hp)->s->lv[l].m.ncell(); }
+// the four vertices (x00, x10, x01, x11 and z likewise) of every cell of MG level l
+void orc_level_vertices(void* hp, int l, double* x, double* z) {
+  const Mesh& m = ((OrcHandle*)hp)->s->lv[l].m;
+  for (int c = 0; c < m.ncell(); ++c) m.verts(c, x + 4 * c, z + 4 * c);
+}
 void orc_prolongate(void* hp, int lfine, double* fine, const double* coarse) {
   Solver* s = ((OrcHandle*)hp)->s;
   const Mesh& mf = s->lv[lfine].m;
   const Mesh& mc = s->lv[lfine - 1].m;
   int nd2 = s->fine->nd2;
   Vec f(mf.ncell() * nd2, 0.0), c(coarse, coarse + mc.ncell() * nd2);
-  prolongate_add(s->tr, mf, mc, f, c, 1);
+  prolongate_add(s->tr, s->lv[lfine - 1].child, mf.ncell(), mc.ncell(), f, c, 1);
   std::memcpy(fine, f.data(), sizeof(double) * f.size());
 }
 void orc_project_down(void* hp, int lfine, double* coarse, const double* fine) {
@@ -1868,7 +2127,7 @@ void orc_project_down(void* hp, int lfine, double* coarse, const double* fine) {
   const Mesh& mc = s->lv[lfine - 1].m;
   int nd2 = s->fine->nd2;
   Vec f(fine, fine + mf.ncell() * nd2), c;
-  project_down(s->tr, mf, mc, c, f);
+  project_down(s->tr, s->lv[lfine - 1].child, mc.ncell(), c, f);
   std::memcpy(coarse, c.data(), sizeof(double) * c.size());
 }
 

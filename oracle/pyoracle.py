@@ -36,10 +36,14 @@ def lib():
         L.orc_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                  ctypes.c_double, _dp, ctypes.c_int]
         L.orc_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_create_amr.restype = ctypes.c_void_p
+        L.orc_create_amr.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                     ctypes.c_double, _dp, ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int]
         for name in ("orc_ncells", "orc_nlevels"):
             getattr(L, name).argtypes = [ctypes.c_void_p]
             getattr(L, name).restype = ctypes.c_int
         L.orc_mg_level_cells.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_level_vertices.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
         L.orc_mg_level_cells.restype = ctypes.c_int
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_get_threads.restype = ctypes.c_int
@@ -97,13 +101,21 @@ def witch_of_agnesi(nx, Lx=40000.0, h0=400.0, a=1000.0, xc=20000.0):
 class Oracle:
     """One discretization (mesh + degree) of the oracle with its MG hierarchy."""
 
-    def __init__(self, k, nx, nz, Lx=40000.0, H=20000.0, hx=None, periodic=True, mg_min_cells=8):
+    def __init__(self, k, nx, nz, Lx=40000.0, H=20000.0, hx=None, periodic=True, mg_min_cells=8, leaves=None):
+        """leaves: optional (n, 5) int rows (bi, bj, level, sx, sz) of a
+        locally refined mesh over the nx x nz base mesh."""
         self.k, self.nx, self.nz = k, nx, nz
         self.nd2 = (k + 1) ** 2
-        self.ncell = nx * nz
+        self.ncell = nx * nz if leaves is None else len(leaves)
         self.Ns = self.ncell * self.nd2
         hx = np.zeros(nx + 1) if hx is None else np.ascontiguousarray(hx, dtype=np.float64)
-        self._h = lib().orc_create(k, nx, nz, 1 if periodic else 0, Lx, H, ptr(hx), mg_min_cells)
+        if leaves is None:
+            self._h = lib().orc_create(k, nx, nz, 1 if periodic else 0, Lx, H, ptr(hx), mg_min_cells)
+        else:
+            lv = np.ascontiguousarray(leaves, dtype=np.int32).reshape(-1, 5)
+            self._h = lib().orc_create_amr(k, nx, nz, 1 if periodic else 0, Lx, H, ptr(hx),
+                                           lv.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), lv.shape[0],
+                                           mg_min_cells)
         if not self._h:
             raise RuntimeError("orc_create failed")
         self.set_params()
