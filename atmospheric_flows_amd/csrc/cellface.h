@@ -164,20 +164,26 @@ __device__ inline void cellface_columns(const MeshDev& m, const Ph& ph, const Fi
 #pragma unroll
       for (int j = 0; j < ND; ++j) Tv[o][j] = Td[o][j] = 0;
     const double wdx = g.wc * g.dx, wze = g.wc * g.ze, jc = wdx * g.ze;
-#pragma unroll
+    // Few fields: the point loop is unrolled (tables become literals). Many
+    // fields (the explicit residuals): a rolled loop with the tables read
+    // from constant memory keeps the register footprint in bounds.
+    constexpr bool UNROLL = NF * ND <= 16;
+#pragma unroll(UNROLL ? NQ : 1)
     for (int r = 0; r < NQ; ++r) {
+      const double* Sr = UNROLL ? tb.S[r] : cT<K>.S[r];
+      const double* Dr = UNROLL ? tb.D[r] : cT<K>.D[r];
       double val[NF];
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
         double s = 0;
 #pragma unroll
-        for (int j = 0; j < ND; ++j) s += tb.S[r][j] * tv[f][j];
+        for (int j = 0; j < ND; ++j) s += Sr[j] * tv[f][j];
         val[f] = s;
       }
       double V[NO], Gx[NO], Gz[NO];
       ph.cell(val, V, Gx, Gz);
-      const double wr = tb.gw[r];
-      const double zs = g.b0 + tb.gq[r] * g.b1;
+      const double wr = UNROLL ? tb.gw[r] : cT<K>.gw[r];
+      const double zs = g.b0 + (UNROLL ? tb.gq[r] : cT<K>.gq[r]) * g.b1;
 #pragma unroll
       for (int o = 0; o < NO; ++o) {
         const bool gx = (Ph::GX >> o) & 1, gz = (Ph::GZ >> o) & 1;
@@ -190,9 +196,9 @@ __device__ inline void cellface_columns(const MeshDev& m, const Ph& ph, const Fi
         const double Vq = Ph::HAS_V ? V[o] * (jc * wr) : 0.0;
 #pragma unroll
         for (int j = 0; j < ND; ++j) {
-          if (Ph::HAS_V) Tv[o][j] += tb.S[r][j] * Vq;
-          if (gx || gz) Tv[o][j] += tb.D[r][j] * Geta;
-          if (gx) Td[o][j] += tb.S[r][j] * Gxi;
+          if (Ph::HAS_V) Tv[o][j] += Sr[j] * Vq;
+          if (gx || gz) Tv[o][j] += Dr[j] * Geta;
+          if (gx) Td[o][j] += Sr[j] * Gxi;
         }
       }
     }
@@ -205,8 +211,9 @@ __device__ inline void cellface_columns(const MeshDev& m, const Ph& ph, const Fi
         const FaceGeo fg = face_geo(v, f);
         const double wl = fg.len * g.wc;
         double own[NF], nb[NF], F[NO];
+        // own trace from LDS (as for the other faces): keeps tv dead after the cell integral
 #pragma unroll
-        for (int q = 0; q < NF; ++q) own[q] = tv[q][side ? K : 0];
+        for (int q = 0; q < NF; ++q) own[q] = face_trace<K>(&sh.in[q][off], f, Sc);
         const int nj = side ? cj + 1 : cj - 1;
         if (nj >= 0 && nj < m.nz) {
           const long nbase = (long)(nj * m.nx + ci) * ND2;
