@@ -198,6 +198,49 @@ def run_ensemble(args):
     return out
 
 
+def run_amr(args):
+    """Locally refined hill flow: 40x20 base mesh, 3 refinement levels in
+    |x - 20 km| < 3 km, z < 6 km plus one level where the density gradient
+    exceeds its 90th percentile, 2:1 balanced; Q2, dt = 0.5 s; --amr-steps
+    steps of the mixed and the plain FP64 form from the same state."""
+    import atmospheric_flows_amd as af
+    from atmospheric_flows_amd.amr import hill_amr_leaves
+    leaves, h = hill_amr_leaves(nx=40, nz=20, levels=3)
+    out = dict(workload="hill_amr_40x20_3levels_Q2_dt0.5", cells=int(len(leaves)),
+               cells_per_level=np.bincount(leaves[:, 2]).tolist(), steps=args.amr_steps)
+    states = {}
+    forms = [("mixed", dict(precision=1)), ("fp64", dict(precision=0))]
+    if not args.no_tight:
+        forms.append(("tight", dict(precision=0, eps=1e-13, a_solve_tol=1e-14, schur_inner_tol=1e-14)))
+    for name, prm in forms:
+        ctx = af.DeviceContext(2, 40, 20, h=h, leaves=leaves, Ma=1.0, Fr=1.0 / np.sqrt(9.81), dt=0.5,
+                               **{"eps": 1e-10, **prm})
+        rho, u, p = ctx.init_hill(seed=20240603)
+        ctx.upload_state(rho, u, p)
+        ctx.imex_step(1)
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.imex_step(args.amr_steps)
+        ctx.sync()
+        el = time.perf_counter() - t0
+        dofs = 4 * ctx.ns
+        out[name] = dict(dof_steps_per_s=dofs * args.amr_steps / el, ms_per_step=1e3 * el / args.amr_steps,
+                         mg_levels=ctx.nlevels, stats=ctx.stats())
+        d = ctx.step_diagnostics()
+        out[name]["cfl_c"], out[name]["nonfinite"] = d["cfl_c"], d["nonfinite"]
+        states[name] = ctx.download_state()
+        print(f"[bench] amr {name}: {args.amr_steps} steps in {el:.3f} s", file=sys.stderr, flush=True)
+    def rel_errs(x, y):
+        return {nm: float(np.linalg.norm(a - b) / np.linalg.norm(b)) for nm, a, b in zip(("rho", "u", "p"), x, y)}
+
+    out["rel_err_mixed_vs_fp64"] = rel_errs(states["mixed"], states["fp64"])
+    if "tight" in states:
+        out["rel_err_vs_converged_fp64"] = {"mixed": rel_errs(states["mixed"], states["tight"]),
+                                            "fp64": rel_errs(states["fp64"], states["tight"])}
+        del out["tight"]
+    return out
+
+
 def run_serve(args, precision):
     """Serving stream: Poisson arrivals at --serve-rate req/s for
     --serve-seconds s; each request is one hill-flow member with mesh drawn
@@ -266,6 +309,8 @@ def main():
     ap.add_argument("--no-traffic", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-tight", action="store_true")
+    ap.add_argument("--no-amr", action="store_true")
+    ap.add_argument("--amr-steps", type=int, default=50)
     ap.add_argument("--serve", action="store_true", help="also run the Poisson-arrival serving stream")
     ap.add_argument("--serve-rate", type=float, default=2.0)
     ap.add_argument("--serve-seconds", type=float, default=20.0)
@@ -298,6 +343,7 @@ def main():
                 traffic=traffic["bytes"] if traffic else None, kernel=DOMINANT[1], bytes_per_launch=dom["bytes"],
                 ms_per_launch=dom["ms"], traffic_detail=traffic)
     ensemble = None if args.no_ensemble else run_ensemble(args)
+    amr = None if args.no_amr else run_amr(args)
     serve = {"mixed": run_serve(args, 1), "fp64": run_serve(args, 0)} if args.serve else None
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_threads)
     out = {
@@ -314,7 +360,7 @@ def main():
         "solver_stats": {"mixed": r["mixed"]["stats"], "fp64": r["fp64"]["stats"]},
         "converged_fp64_seconds": r["tight"]["seconds"] if "tight" in r else None,
         "kernels": r["kernels"], "roofline": roof, "cpu_baseline": cpu, "ensemble": ensemble,
-        "serve": serve,
+        "serve": serve, "amr": amr,
     }
     print(json.dumps(out))
 
