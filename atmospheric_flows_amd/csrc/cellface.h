@@ -11,6 +11,7 @@
 #include "common.h"
 #include "wave.h"
 
+
 namespace atmo {
 
 template <int NF>
@@ -52,18 +53,29 @@ struct PhysC {
   static constexpr bool BDRY = false;
   static constexpr bool HAS_V = false;
   static constexpr unsigned GX = 0b1, GZ = 0b1;
-  double cdt, gm;
+  double cdt;
+  // gamma factors, formed on the host in the order of the reference's
+  // expressions (bit-identical to forming them per point)
+  double cs;  // cdt*gamma/(gamma-1)
+  double hg;  // 0.5*gamma/(gamma-1)
+  double ig;  // 1/(gamma-1)
+  void set(double cdt_, double gm) {
+    cdt = cdt_;
+    cs = cdt * gm / (gm - 1.0);
+    hg = 0.5 * gm / (gm - 1.0);
+    ig = 1.0 / (gm - 1.0);
+  }
   __device__ void cell(const double (&v)[NF], double (&V)[NO], double (&Gx)[NO], double (&Gz)[NO]) const {
-    const double s = -cdt * gm / (gm - 1.0) * v[0];
+    const double s = -cs * v[0];
     V[0] = 0;
     Gx[0] = s * v[1];
     Gz[0] = s * v[2];
   }
   __device__ void face(const double (&a)[NF], const double (&b)[NF], double n0, double n1, double (&F)[NO]) const {
-    const double ax = 0.5 * gm / (gm - 1.0) * (a[0] * a[1] + b[0] * b[1]);
-    const double az = 0.5 * gm / (gm - 1.0) * (a[0] * a[2] + b[0] * b[2]);
+    const double ax = hg * (a[0] * a[1] + b[0] * b[1]);
+    const double az = hg * (a[0] * a[2] + b[0] * b[2]);
     const double lam = fmax(fabs(a[1] * n0 + a[2] * n1), fabs(b[1] * n0 + b[2] * n1));
-    const double jump = 1.0 / (gm - 1.0) * (a[0] - b[0]);
+    const double jump = ig * (a[0] - b[0]);
     F[0] = cdt * (ax * n0 + az * n1 + 0.5 * lam * jump);
   }
   __device__ void bdry(const double (&)[NF], double, double, double (&F)[NO]) const { F[0] = 0; }
@@ -80,7 +92,7 @@ struct PhysDminusC {
   PhysC c;
   double cD;  // 1/(gamma-1)
   __device__ void cell(const double (&v)[NF], double (&V)[NO], double (&Gx)[NO], double (&Gz)[NO]) const {
-    const double s = c.cdt * c.gm / (c.gm - 1.0) * v[0];
+    const double s = c.cs * v[0];
     V[0] = cD * v[3];
     Gx[0] = s * v[1];
     Gz[0] = s * v[2];
@@ -102,7 +114,8 @@ template <int K, class Ph, bool AMR>
 struct CFSmem {
   static constexpr int ND = K + 1, NQ = 2 * K + 1, ND2 = ND * ND, CPW = 64 / NQ;
   static constexpr int NFACE = AMR ? 4 : 2, NHALF = AMR ? 2 : 1;
-  double in[Ph::NF][CPW * ND2];
+  static constexpr int CS = lds_cell_stride<K>();
+  double in[Ph::NF][CPW * CS];               // staged inputs, cell w at w*CS
   double T[Ph::NO][2][CPW][ND][NQ];          // column partials of the value/eta and xi test functions
   double F[Ph::NO][NFACE][NHALF][CPW][NQ];   // face fluxes times the face weight
   int ft[AMR ? CPW : 1][4];                  // AMR: 0 no flux, 1 full face, 2 split face
@@ -137,17 +150,16 @@ __device__ inline void cellface_columns(const MeshDev& m, const Ph& ph, const Fi
   using C = Cfg<K>;
   constexpr int ND = C::ND, NQ = C::NQ, ND2 = C::ND2;
   constexpr int NF = Ph::NF, NO = Ph::NO;
-  constexpr auto& tb = basis::kT<K>;
   LaneMap<K> L(m.ncell);
 #pragma unroll
-  for (int f = 0; f < NF; ++f) stage_cells<K>(sh.in[f], in.p[f], m.ncell);
+  for (int f = 0; f < NF; ++f) stage_cells<K, double, double, CFSmem<K, Ph, AMR>::CS>(sh.in[f], in.p[f], m.ncell);
   __syncthreads();
   if (L.active) {
     const CellVerts v = cell_verts(m, L.cell);
     const ColGeo<double> g = col_geo<K, double>(v, L.c);
     double Sc[ND], Dc[ND];
     lane_rows<K, double>(L.c, Sc, Dc);
-    const int off = L.cw * ND2;
+    const int off = L.cw * CFSmem<K, Ph, AMR>::CS;
     double tv[NF][ND];
 #pragma unroll
     for (int f = 0; f < NF; ++f)
@@ -164,14 +176,13 @@ __device__ inline void cellface_columns(const MeshDev& m, const Ph& ph, const Fi
 #pragma unroll
       for (int j = 0; j < ND; ++j) Tv[o][j] = Td[o][j] = 0;
     const double wdx = g.wc * g.dx, wze = g.wc * g.ze, jc = wdx * g.ze;
-    // Few fields: the point loop is unrolled (tables become literals). Many
-    // fields (the explicit residuals): a rolled loop with the tables read
-    // from constant memory keeps the register footprint in bounds.
-    constexpr bool UNROLL = NF * ND <= 16;
-#pragma unroll(UNROLL ? NQ : 1)
+    // Rolled point loop, table rows read from constant memory (scalar loads
+    // per row): unrolled, every FP64 table entry becomes an SGPR-pair
+    // literal and the kernels spill SGPRs into VGPR lanes.
+#pragma unroll 1
     for (int r = 0; r < NQ; ++r) {
-      const double* Sr = UNROLL ? tb.S[r] : cT<K>.S[r];
-      const double* Dr = UNROLL ? tb.D[r] : cT<K>.D[r];
+      const double* Sr = cT<K>.S[r];
+      const double* Dr = cT<K>.D[r];
       double val[NF];
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
@@ -182,8 +193,8 @@ __device__ inline void cellface_columns(const MeshDev& m, const Ph& ph, const Fi
       }
       double V[NO], Gx[NO], Gz[NO];
       ph.cell(val, V, Gx, Gz);
-      const double wr = UNROLL ? tb.gw[r] : cT<K>.gw[r];
-      const double zs = g.b0 + (UNROLL ? tb.gq[r] : cT<K>.gq[r]) * g.b1;
+      const double wr = cT<K>.gw[r];
+      const double zs = g.b0 + cT<K>.gq[r] * g.b1;
 #pragma unroll
       for (int o = 0; o < NO; ++o) {
         const bool gx = (Ph::GX >> o) & 1, gz = (Ph::GZ >> o) & 1;

@@ -176,16 +176,25 @@ struct LaneMap {
   }
 };
 
-// Contiguous staging of the CPW cells of this wave: dst[t] = src[cell0*ND2 + t]
-template <int K, typename T, typename S>
+// Staging of the CPW cells of this wave: dof d of block cell w goes to
+// lds[w*STRIDE + d] (STRIDE > ND2 pads the cells apart in the LDS banks).
+template <int K, typename T, typename S, int STRIDE = Cfg<K>::ND2>
 __device__ inline void stage_cells(T* __restrict__ lds, const S* __restrict__ src, int ncell) {
-  constexpr int N = Cfg<K>::CPW * Cfg<K>::ND2;
-  const long base = (long)blockIdx.x * Cfg<K>::CPW * Cfg<K>::ND2;
-  const long lim = (long)ncell * Cfg<K>::ND2;
+  constexpr int ND2 = Cfg<K>::ND2, N = Cfg<K>::CPW * ND2;
+  const long base = (long)blockIdx.x * Cfg<K>::CPW * ND2;
+  const long lim = (long)ncell * ND2;
   for (int t = threadIdx.x; t < N; t += 64) {
     long g = base + t;
-    lds[t] = (g < lim) ? (T)src[g] : (T)0;
+    const int dst = STRIDE == ND2 ? t : (t / ND2) * STRIDE + t % ND2;
+    lds[dst] = (g < lim) ? (T)src[g] : (T)0;
   }
+}
+
+// LDS stride of one staged cell: ND2 rounded up to an odd number of
+// doubles, so the CPW cells of a wave start in different banks.
+template <int K>
+constexpr int lds_cell_stride() {
+  return Cfg<K>::ND2 % 2 == 0 ? Cfg<K>::ND2 + 1 : Cfg<K>::ND2;
 }
 
 // Per-lane copies of the lane-dependent table rows Sc[i] = S[c][i],

@@ -388,8 +388,7 @@ template <int K>
 void launch_C(const MeshDev& m, const OpConsts& oc, double coeff, const double* pfix, const double* u,
               double* dst_p, long Ns, hipStream_t s) {
   PhysC ph;
-  ph.cdt = coeff * oc.dt;
-  ph.gm = oc.gamma_gas;
+  ph.set(coeff * oc.dt, oc.gamma_gas);
   FieldPtrs<3> in{{pfix, u, u + Ns}};
   launch_cf<K>(m, ph, in, dst_p, Ns, s);
 }
@@ -398,8 +397,7 @@ template <int K>
 void launch_d_minus_c(const MeshDev& m, const OpConsts& oc, double coeff, const double* pfix, const double* u,
                       const double* p, const double* scale, double* dst, long Ns, hipStream_t s) {
   PhysDminusC ph;
-  ph.c.cdt = coeff * oc.dt;
-  ph.c.gm = oc.gamma_gas;
+  ph.c.set(coeff * oc.dt, oc.gamma_gas);
   ph.cD = 1.0 / (oc.gamma_gas - 1.0);
   FieldPtrs<4> in{{pfix, u, u + Ns, p}};
   launch_cf<K>(m, ph, in, dst, Ns, s, scale);

@@ -13,17 +13,18 @@ namespace atmo {
 template <int K, bool AMR>
 __global__ __launch_bounds__(64) void k_b_ainv(MeshDev m, PhysB ph, const double* __restrict__ p,
                                                const double* __restrict__ Ainv, double* __restrict__ y, long Ns) {
-  constexpr int ND2 = Cfg<K>::ND2, CPW = Cfg<K>::CPW;
+  constexpr int ND2 = Cfg<K>::ND2, CPW = Cfg<K>::CPW, CS = lds_cell_stride<K>();
   __shared__ CFSmem<K, PhysB, AMR> sh;
-  __shared__ double s_t[2][CPW * ND2];
+  __shared__ double s_t[2][CPW * CS];
   FieldPtrs<1> in{{p}};
   cellface_columns<K, PhysB, AMR>(m, ph, in, sh);
   const int cell0 = blockIdx.x * CPW;
   OutRows<K> R = out_rows<K>(threadIdx.x);
   for (int t = threadIdx.x; t < CPW * ND2; t += 64) {
     if (!rows_invariant<K>()) R = out_rows<K>(t);
-    s_t[0][t] = cellface_out<K, PhysB, AMR>(sh, 0, t, R);
-    s_t[1][t] = cellface_out<K, PhysB, AMR>(sh, 1, t, R);
+    const int st = (t / ND2) * CS + t % ND2;
+    s_t[0][st] = cellface_out<K, PhysB, AMR>(sh, 0, t, R);
+    s_t[1][st] = cellface_out<K, PhysB, AMR>(sh, 1, t, R);
   }
   __syncthreads();
   for (int t = threadIdx.x; t < CPW * ND2; t += 64) {
@@ -35,8 +36,8 @@ __global__ __launch_bounds__(64) void k_b_ainv(MeshDev m, PhysB ph, const double
 #pragma unroll
     for (int b = 0; b < ND2; ++b) {
       const double v = col[b * ND2];
-      y0 += v * s_t[0][cw * ND2 + b];
-      y1 += v * s_t[1][cw * ND2 + b];
+      y0 += v * s_t[0][cw * CS + b];
+      y1 += v * s_t[1][cw * CS + b];
     }
     const long gi = cell * ND2 + a;
     y[gi] = y0;
