@@ -197,6 +197,17 @@ constexpr int lds_cell_stride() {
   return Cfg<K>::ND2 % 2 == 0 ? Cfg<K>::ND2 + 1 : Cfg<K>::ND2;
 }
 
+// Symmetric ND2 x ND2 cell blocks stored as their upper triangle, row by
+// row: SIZE doubles per cell, entry (a, b) = (b, a) at index(a, b).
+// LDS_STRIDE (odd) spaces staged blocks apart in the LDS banks.
+template <int K>
+struct SymBlock {
+  static constexpr int N = Cfg<K>::ND2, SIZE = N * (N + 1) / 2, LDS_STRIDE = SIZE | 1;
+  __host__ __device__ static constexpr int index(int a, int b) {
+    return a <= b ? a * (2 * N - a + 1) / 2 + (b - a) : b * (2 * N - b + 1) / 2 + (a - b);
+  }
+};
+
 // Per-lane copies of the lane-dependent table rows Sc[i] = S[c][i],
 // Dc[i] = D[c][i] (constant-memory loads).
 template <int K, typename T>

@@ -159,7 +159,7 @@ class EngineK : public Engine {
     pres_fix.zero(s_);
     for (auto* b : {&cg_g, &cg_d, &cg_h, &s_t1, &s_t2, &in_g, &in_d, &in_h}) b->alloc(2 * Ns);
     s_t3.alloc(Ns);
-    if (P.precision == 1) ainv.alloc((size_t)fine().ncell * (K + 1) * (K + 1) * (K + 1) * (K + 1));
+    if (P.precision == 1) ainv.alloc((size_t)fine().ncell * SymBlock<K>::SIZE);
     gm_V.alloc((size_t)(P.gmres_restart + 2) * Ns);
     gm_p.alloc(Ns);
     gm_coef.alloc(MAX_GMRES + 2);
@@ -214,7 +214,7 @@ class EngineK : public Engine {
     oc_ = consts();
     if (rebuild) gm_V.alloc((size_t)(P.gmres_restart + 2) * fine().Ns);
     if (prec_change) {
-      if (P.precision == 1 && !ainv.p) ainv.alloc((size_t)fine().ncell * (K + 1) * (K + 1) * (K + 1) * (K + 1));
+      if (P.precision == 1 && !ainv.p) ainv.alloc((size_t)fine().ncell * SymBlock<K>::SIZE);
       refresh_levels_rho();
     }
   }
@@ -343,7 +343,8 @@ class EngineK : public Engine {
         case 8:  // mixed Schur: B and the cellwise A^{-1} in one pass
           if (!ainv.p) throw std::invalid_argument("time_kernel 8 needs the mixed form (precision 1)");
           launch_b_ainv<K>(m, oc_, bc_coeff(), p_old.p, ainv.p, in_h.p, Ns, s_);
-          fields = 3 + (K + 1) * (K + 1);
+          // p, y (2 components), and the packed inverse block per cell
+          fields = 3 + (double)SymBlock<K>::SIZE / ((K + 1) * (K + 1));
           break;
         case 9:  // D p - C u, scaled (GMRES's preconditioned Schur product)
           launch_d_minus_c<K>(m, oc_, bc_coeff(), pres_fix.p, u_old.p, p_old.p, fine().inv64[BLK_D].p, gm_p.p, Ns,
@@ -419,7 +420,7 @@ class EngineK : public Engine {
   RedBuf rb;
   DBuf<double> red_partials;
   DBuf<double> diag_part, diag_rec;  // per-step diagnostics: block partials and the step record
-  DBuf<double> ainv;  // FP64 inverse cell blocks of A (mixed form)
+  DBuf<double> ainv;  // FP64 inverse cell blocks of A, packed symmetric (mixed form)
   DBuf<CGState> cgst;
   DBuf<double> scal;
   DBuf<int> coarse_its;
@@ -1058,7 +1059,7 @@ size_t engine_device_bytes(int k, const MeshDesc& mesh, const EngineParams& p) {
   b += RedBuf::doubles() * D + 4 * sizeof(CGState) + 256 * D + sizeof(int) + DIAG_PARTIALS * D + DG_SLOTS * D;
   b += 2 * Ns * D;                        // rho_fix, pres_fix
   b += 8 * 2 * Ns * D + Ns * D;           // CG / Schur / inner work
-  if (p.precision == 1) b += (size_t)mesh.nx * mesh.nz * nd2 * nd2 * D;  // FP64 cell-block inverses of A
+  if (p.precision == 1) b += (size_t)mesh.nx * mesh.nz * nd2 * (nd2 + 1) / 2 * D;  // packed FP64 cell-block inverses of A
   b += (size_t)(p.gmres_restart + 2) * Ns * D + Ns * D + (MAX_GMRES + 2) * D;
   b += 10 * Ns * D + 7 * 2 * Ns * D;      // IMEX state and stage fields
   return b;

@@ -504,7 +504,8 @@ void jcg_apply(int i, const MeshDev& m, const double* rho, const double* P, cons
 // ------------------------------------------------ cell block inverses
 // One wave per cell: assemble the (k+1)^2 x (k+1)^2 block of A (rho-weighted
 // mass at the 2k+1 Gauss points) in LDS and invert it in place by
-// Gauss-Jordan (SPD, no pivoting) in FP64.
+// Gauss-Jordan (SPD, no pivoting) in FP64. Stored packed (SymBlock), each
+// off-diagonal pair as the mean of its two computed entries.
 template <int K>
 __global__ __launch_bounds__(256) void k_block_inv(MeshDev m, const double* __restrict__ rho,
                                                    double* __restrict__ Ainv) {
@@ -573,7 +574,11 @@ __global__ __launch_bounds__(256) void k_block_inv(MeshDev m, const double* __re
     }
     wave_sync();
   }
-  for (int e = lane; e < ND2 * ND2; e += 64) Ainv[(long)cell * ND2 * ND2 + e] = s_A[w][e];
+  for (int e = lane; e < ND2 * ND2; e += 64) {
+    const int a = e / ND2, b = e - a * ND2;
+    if (b >= a)
+      Ainv[(long)cell * SymBlock<K>::SIZE + SymBlock<K>::index(a, b)] = 0.5 * (s_A[w][e] + s_A[w][b * ND2 + a]);
+  }
 }
 
 template <int K>

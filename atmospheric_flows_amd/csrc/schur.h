@@ -8,7 +8,7 @@
 namespace atmo {
 
 // y = A^{-1} (B p): B (vmult_pressure, euler_operator.h:2420-2435) with the
-// cell blocks' FP64 inverses Ainv ([cell][ND2][ND2], symmetric) applied in
+// cell blocks' FP64 inverses Ainv (symmetric, packed per cell: SymBlock) applied in
 // the same pass; t = B p never leaves the chip.
 template <int K>
 void launch_b_ainv(const MeshDev& m, const OpConsts& oc, double coeff, const double* p, const double* Ainv,

@@ -1,45 +1,75 @@
 // Cellwise-direct A^{-1} kernels (schur.h).
 //
-// The inverse blocks are stored per cell as ND2 x ND2 FP64 matrices. Dof a
-// of a cell is produced by the thread (cell, a): it walks row a (= column a,
-// the blocks are symmetric) while the 16 threads of the cell read 16
-// consecutive doubles per step, so every load instruction moves whole
-// 128-byte lines.
+// The inverse blocks are symmetric and stored packed (SymBlock: the upper
+// triangle, N (N + 1) / 2 doubles per cell for N = (k+1)^2). A block's
+// packed entries are loaded coalesced into LDS; dof a of a cell is then
+// produced by the thread (cell, a) walking row a of the block in LDS.
 #include "cellface.h"
 #include "schur.h"
 
 namespace atmo {
 
+// Stages the packed inverse blocks of nc consecutive cells from Ab into LDS
+// (cell w at w * LDS_STRIDE): coalesced loads, all issued before the first
+// LDS write.
+template <int K, int NT, int MAXC>
+__device__ inline void stage_sym_blocks(double* __restrict__ lds, const double* __restrict__ Ab, int nc) {
+  using SB = SymBlock<K>;
+  constexpr int NPASS = (MAXC * SB::SIZE + NT - 1) / NT;
+  const int lim = nc * SB::SIZE;
+  double v[NPASS];
+#pragma unroll
+  for (int it = 0; it < NPASS; ++it) {
+    const int e = threadIdx.x + it * NT;
+    v[it] = e < lim ? Ab[e] : 0.0;
+  }
+#pragma unroll
+  for (int it = 0; it < NPASS; ++it) {
+    const int e = threadIdx.x + it * NT;
+    if (e < lim) {
+      const int w = e / SB::SIZE;
+      lds[e + w * (SB::LDS_STRIDE - SB::SIZE)] = v[it];
+    }
+  }
+}
+
 template <int K, bool AMR>
 __global__ __launch_bounds__(64) void k_b_ainv(MeshDev m, PhysB ph, const double* __restrict__ p,
                                                const double* __restrict__ Ainv, double* __restrict__ y, long Ns) {
   constexpr int ND2 = Cfg<K>::ND2, CPW = Cfg<K>::CPW, CS = lds_cell_stride<K>();
-  __shared__ CFSmem<K, PhysB, AMR> sh;
+  using SB = SymBlock<K>;
+  // the inverse blocks take the place of the cell+face partials once t = B p is formed
+  __shared__ union {
+    CFSmem<K, PhysB, AMR> cf;
+    double A[CPW * SB::LDS_STRIDE];
+  } sh;
   __shared__ double s_t[2][CPW * CS];
   FieldPtrs<1> in{{p}};
-  cellface_columns<K, PhysB, AMR>(m, ph, in, sh);
+  cellface_columns<K, PhysB, AMR>(m, ph, in, sh.cf);
   const int cell0 = blockIdx.x * CPW;
   OutRows<K> R = out_rows<K>(threadIdx.x);
   for (int t = threadIdx.x; t < CPW * ND2; t += 64) {
     if (!rows_invariant<K>()) R = out_rows<K>(t);
     const int st = (t / ND2) * CS + t % ND2;
-    s_t[0][st] = cellface_out<K, PhysB, AMR>(sh, 0, t, R);
-    s_t[1][st] = cellface_out<K, PhysB, AMR>(sh, 1, t, R);
+    s_t[0][st] = cellface_out<K, PhysB, AMR>(sh.cf, 0, t, R);
+    s_t[1][st] = cellface_out<K, PhysB, AMR>(sh.cf, 1, t, R);
   }
+  __syncthreads();
+  const int nc = min(CPW, m.ncell - cell0);
+  stage_sym_blocks<K, 64, CPW>(sh.A, Ainv + (long)cell0 * SB::SIZE, nc);
   __syncthreads();
   for (int t = threadIdx.x; t < CPW * ND2; t += 64) {
     const int cw = t / ND2, a = t - cw * ND2;
-    const long cell = cell0 + cw;
-    if (cell >= m.ncell) break;
-    const double* col = Ainv + cell * ND2 * ND2 + a;
+    if (cw >= nc) break;
+    const double* Ac = sh.A + cw * SB::LDS_STRIDE;
     double y0 = 0.0, y1 = 0.0;
 #pragma unroll
     for (int b = 0; b < ND2; ++b) {
-      const double v = col[b * ND2];
+      const double v = Ac[SB::index(a, b)];
       y0 += v * s_t[0][cw * CS + b];
       y1 += v * s_t[1][cw * CS + b];
     }
-    const long gi = cell * ND2 + a;
+    const long gi = (long)(cell0 + cw) * ND2 + a;
     y[gi] = y0;
     y[Ns + gi] = y1;
   }
@@ -48,30 +78,31 @@ __global__ __launch_bounds__(64) void k_b_ainv(MeshDev m, PhysB ph, const double
 template <int K>
 __global__ __launch_bounds__(256) void k_ainv_apply(int ncell, const double* __restrict__ Ainv,
                                                     const double* __restrict__ b, double* __restrict__ x, long Ns) {
-  constexpr int ND2 = (K + 1) * (K + 1), CB = 256 / ND2;
-  __shared__ double s_b[2][CB * ND2];
-  const long base = (long)blockIdx.x * CB * ND2;
-  const long lim = (long)ncell * ND2;
+  constexpr int ND2 = (K + 1) * (K + 1), CB = 256 / ND2, CS = lds_cell_stride<K>();
+  using SB = SymBlock<K>;
+  __shared__ double s_A[CB * SB::LDS_STRIDE];
+  __shared__ double s_b[2][CB * CS];
+  const long cell0 = (long)blockIdx.x * CB;
+  const int nc = (int)min((long)CB, ncell - cell0);
   const int t = threadIdx.x;
-  if (t < CB * ND2) {
-    const long gi = base + t;
-    s_b[0][t] = gi < lim ? b[gi] : 0.0;
-    s_b[1][t] = gi < lim ? b[Ns + gi] : 0.0;
-  }
-  __syncthreads();
-  if (t >= CB * ND2) return;
   const int cw = t / ND2, a = t - cw * ND2;
-  const long cell = (long)blockIdx.x * CB + cw;
-  if (cell >= ncell) return;
-  const double* col = Ainv + cell * ND2 * ND2 + a;
+  if (t < CB * ND2 && cw < nc) {
+    const long gi = cell0 * ND2 + t;
+    s_b[0][cw * CS + a] = b[gi];
+    s_b[1][cw * CS + a] = b[Ns + gi];
+  }
+  stage_sym_blocks<K, 256, CB>(s_A, Ainv + cell0 * SB::SIZE, nc);
+  __syncthreads();
+  if (t >= CB * ND2 || cw >= nc) return;
+  const double* Ac = s_A + cw * SB::LDS_STRIDE;
   double x0 = 0.0, x1 = 0.0;
 #pragma unroll
   for (int k = 0; k < ND2; ++k) {
-    const double v = col[k * ND2];
-    x0 += v * s_b[0][cw * ND2 + k];
-    x1 += v * s_b[1][cw * ND2 + k];
+    const double v = Ac[SB::index(a, k)];
+    x0 += v * s_b[0][cw * CS + k];
+    x1 += v * s_b[1][cw * CS + k];
   }
-  const long gi = cell * ND2 + a;
+  const long gi = cell0 * ND2 + t;
   x[gi] = x0;
   x[Ns + gi] = x1;
 }

@@ -14,9 +14,10 @@ def test_footprint_scales_with_the_mesh_and_precision():
     # ~ 100-130 FP64 fields of N_s on the fine level plus the multigrid levels
     assert 90 * 8 * ns1 < b1 < 140 * 8 * ns1
     assert abs(b2 / b1 - 4.0) < 0.1  # fixed-size reduction and record buffers aside
-    # the mixed form adds exactly the FP64 inverses of A's cell blocks: (k+1)^4 doubles per cell
+    # the mixed form adds exactly the FP64 inverses of A's cell blocks, symmetric and packed:
+    # N (N + 1) / 2 doubles per cell for N = (k+1)^2
     bm = af.device_bytes(3, 128, 64, precision=1)
-    assert bm - b2 == 128 * 64 * 256 * 8
+    assert bm - b2 == 128 * 64 * 136 * 8
     # a longer GMRES basis adds exactly its vectors
     assert af.device_bytes(3, 128, 64, gmres_restart=30) - b2 == 2 * ns2 * 8
 
