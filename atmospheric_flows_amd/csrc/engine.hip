@@ -153,6 +153,8 @@ class EngineK : public Engine {
     ATMO_HIP_CHECK(hipHostMalloc((void**)&hst, sizeof(CGState) * 4, hipHostMallocDefault));
     ATMO_HIP_CHECK(hipHostMalloc((void**)&hscal, sizeof(double) * 256, hipHostMallocDefault));
     ATMO_HIP_CHECK(hipHostMalloc((void**)&hint, sizeof(int) * 4, hipHostMallocDefault));
+    ATMO_HIP_CHECK(hipHostMalloc((void**)&hgm, sizeof(double) * GM_HOST, hipHostMallocDefault));
+    for (auto& e : gm_ev) ATMO_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     rho_fix.alloc(Ns);
     pres_fix.alloc(Ns);
     rho_fix.zero(s_);
@@ -163,6 +165,8 @@ class EngineK : public Engine {
     gm_V.alloc((size_t)(P.gmres_restart + 2) * Ns);
     gm_p.alloc(Ns);
     gm_coef.alloc(MAX_GMRES + 2);
+    gm_rec.alloc((size_t)MAX_GMRES * GM_REC);
+    gm_sig.alloc(MAX_GMRES + 2);
     for (auto* b : {&rho_old, &rho2, &rho3, &rho_curr, &p_old, &p2, &p3, &p_fixed, &p_fixed_old, &rhs_p}) b->alloc(Ns);
     for (auto* b : {&u_old, &u2, &u3, &u_curr, &u_fixed, &tmp1, &rhs_u}) b->alloc(2 * Ns);
     for (auto* b : {&rho_old, &rho2, &rho3, &rho_curr, &p_old, &p2, &p3, &p_fixed, &p_fixed_old, &rhs_p, &u_old, &u2,
@@ -185,6 +189,9 @@ class EngineK : public Engine {
     if (hscal) (void)hipHostFree(hscal);
     if (hint) (void)hipHostFree(hint);
     (void)hipStreamSynchronize(s_);
+    if (hgm) (void)hipHostFree(hgm);
+    for (auto& e : gm_ev)
+      if (e) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(s_);
   }
 
@@ -427,6 +434,12 @@ class EngineK : public Engine {
   CGState* hst = nullptr;
   double* hscal = nullptr;
   int* hint = nullptr;
+  // GMRES: pinned host copies of the per-iteration records followed by the
+  // update coefficients, and one event per iteration
+  static constexpr int GM_HOST = MAX_GMRES * GM_REC + MAX_GMRES + 2;
+  double* hgm = nullptr;
+  hipEvent_t gm_ev[MAX_GMRES] = {};
+  DBuf<double> gm_rec, gm_sig;
   int hyper = 1, nsst = 1;
   DBuf<double> rho_fix, pres_fix;
   DBuf<double> cg_g, cg_d, cg_h, s_t1, s_t2, s_t3, in_g, in_d, in_h, gm_V, gm_p, gm_coef;
@@ -677,19 +690,19 @@ class EngineK : public Engine {
     const double theta = 0.5 * (L.lmax[blk] + L.lmin[blk]), delta = 0.5 * (L.lmax[blk] - L.lmin[blk]);
     const double sigma = theta / delta;
     double rho = 1.0 / sigma;
-    T* t = LT<T>(L);
     T* d = LD<T>(L);
     const T* Pd = LP<T>(L, blk);
-    if (zero_guess) {
-      cheb_first<T>(n, b, Pd, x, d, (T)(1.0 / theta), s_);
-    } else {
-      level_apply<T>(L, blk, x, t);
-      cheb_step<T>(n, b, t, Pd, x, d, (T)0, (T)(1.0 / theta), s_);
-    }
+    const T* rw = nullptr;
+    if (blk == BLK_A) rw = std::is_same<T, double>::value ? (const T*)L.rho64.p : (const T*)L.rho32.p;
+    // one smoothing step: x += d with d = f1 d + f2 P (b - A x), operator and update in one kernel
+    auto step = [&](double f1, double f2) {
+      launch_mass_cheb<K, T>(L.m, ncomp(blk), blk == BLK_A, x, rw, (T)coeff(blk), b, Pd, d, (T)f1, (T)f2, L.Ns, s_);
+    };
+    if (zero_guess) cheb_first<T>(n, b, Pd, x, d, (T)(1.0 / theta), s_);
+    else step(0.0, 1.0 / theta);
     for (int k = 0; k < P.cheb_degree - 1; ++k) {
-      level_apply<T>(L, blk, x, t);
       const double rn = 1.0 / (2.0 * sigma - rho);
-      cheb_step<T>(n, b, t, Pd, x, d, (T)(rn * rho), (T)(2.0 * rn / delta), s_);
+      step(rn * rho, 2.0 * rn / delta);
       rho = rn;
     }
   }
@@ -838,16 +851,41 @@ class EngineK : public Engine {
 
   // GMRES, left Jacobi (diag D) preconditioning, convergence on the
   // preconditioned residual against tol_factor*|b| (deal.II SolverGMRES
-  // defaults). Orthogonalization: classical Gram-Schmidt with one
-  // re-orthogonalization when the norm drops below 10*sqrt(eps) of its
-  // start (same criterion as deal.II's iterated Gram-Schmidt).
+  // defaults). Orthogonalization: classical Gram-Schmidt with a second
+  // projection when the norm drops below 10*sqrt(eps) of its start (same
+  // criterion as deal.II's iterated Gram-Schmidt).
+  //
+  // The Arnoldi vectors stay unnormalized on the device, v_i = sig_i V_i
+  // with sig_i = 1/|V_i| kept in device memory, so iteration j+1 (S V_{j+1},
+  // its dot products and its projection) needs nothing from the host: the
+  // host keeps GM_LOOKAHEAD iterations queued and works one record behind
+  // (Hessenberg column, Givens rotations, convergence), from a copy of each
+  // iteration's record in pinned memory. Iterations queued past the
+  // converged one are discarded; when a record asks for the second
+  // projection the host drains the stream, projects again and re-queues
+  // from there.
+  static constexpr int GM_LOOKAHEAD = 2;
+
+  void gm_enqueue(int j, const double* PD) {
+    const long Ns = fine().Ns;
+    double* V = gm_V.p;
+    double* W = V + (long)(j + 1) * Ns;
+    double* rec = gm_rec.p + (long)j * GM_REC;
+    schur_vmult(V + (long)j * Ns, W, PD);
+    gm_mdot(Ns, W, V, Ns, j + 2, rec, rb, s_);  // W.V_i (i <= j) and W.W
+    gm_maxpy_norm(Ns, W, V, Ns, j + 1, rec, gm_sig.p, rec + GM_REC - 1, gm_sig.p + j + 1, rb, s_);
+    ATMO_HIP_CHECK(hipMemcpyAsync(hgm + (long)j * GM_REC, rec, sizeof(double) * GM_REC, hipMemcpyDeviceToHost, s_));
+    ATMO_HIP_CHECK(hipEventRecord(gm_ev[j], s_));
+  }
+
   int gmres(double* x, const double* b, double tol_factor) {
     const long Ns = fine().Ns;
     const int m = P.gmres_restart;
     const double* PD = fine().inv64[BLK_D].p;
     double* V = gm_V.p;
     double* p = gm_p.p;
-    double* coef = gm_coef.p;
+    double* sig = gm_sig.p;
+    double* hy = hgm + (long)MAX_GMRES * GM_REC;
     vec_dot(Ns, b, b, scal.p + 0, true, rb, s_);
     ATMO_HIP_CHECK(hipMemcpyAsync(hscal, scal.p, sizeof(double), hipMemcpyDeviceToHost, s_));
     sync();
@@ -857,13 +895,14 @@ class EngineK : public Engine {
       if (step >= P.max_its || std::isnan(res)) return -1;
       return 0;
     };
-    std::vector<double> H((size_t)(m + 1) * m, 0.0), gam(m + 1), cs(m), sn(m), hcol(m + 2);
+    const double reorth_fac = 10.0 * std::sqrt(2.220446049250313e-16);
+    std::vector<double> H((size_t)(m + 1) * m, 0.0), gam(m + 1), cs(m), sn(m), hs(m + 2);
     int acc = 0, state = 0;
     while (true) {
       schur_vmult(x, p);
       vec_axpby(Ns, 1.0, b, -1.0, p, s_);  // p = b - S x
-      vec_mul(Ns, p, PD, V, s_);
-      vec_dot(Ns, V, V, scal.p + 0, true, rb, s_);
+      vec_mul(Ns, p, PD, V, s_);           // V_0 = P (b - S x), unnormalized
+      gm_norm(Ns, V, scal.p + 0, sig, rb, s_);
       ATMO_HIP_CHECK(hipMemcpyAsync(hscal, scal.p, sizeof(double), hipMemcpyDeviceToHost, s_));
       sync();
       double rho = hscal[0];
@@ -871,42 +910,42 @@ class EngineK : public Engine {
       if (state != 0) break;
       std::fill(gam.begin(), gam.end(), 0.0);
       gam[0] = rho;
-      vec_axpby(Ns, 1.0 / rho, V, 0.0, V, s_);
-      int dim = 0;
+      hs[0] = 1.0 / rho;  // same correctly rounded quotient as sig_0 on the device
+      int queued = 0, dim = 0;
       for (int j = 0; j < m && state == 0; ++j) {
+        while (queued < m && queued - j < GM_LOOKAHEAD) gm_enqueue(queued++, PD);
+        ATMO_HIP_CHECK(hipEventSynchronize(gm_ev[j]));
+        ++st_.host_syncs;
         ++acc;
         dim = j + 1;
-        double* w = V + (long)(j + 1) * Ns;
-        schur_vmult(V + (long)j * Ns, w, PD);
-        // h_i = w.V_i (i <= j) and |w|^2 in one pass
-        gm_mdot(Ns, w, V, Ns, j + 2, coef, rb, s_);
-        gm_maxpy_norm(Ns, w, V, Ns, j + 1, coef, scal.p + 1, rb, s_);
-        ATMO_HIP_CHECK(hipMemcpyAsync(hcol.data(), coef, sizeof(double) * (j + 2), hipMemcpyDeviceToHost, s_));
-        ATMO_HIP_CHECK(hipMemcpyAsync(hscal, scal.p, sizeof(double) * 2, hipMemcpyDeviceToHost, s_));
-        sync();
-        const double nstart = std::sqrt(hcol[j + 1]);
-        double nw = hscal[1];
+        const double* r = hgm + (long)j * GM_REC;
         double* hc = &H[(size_t)j * (m + 1)];
-        for (int i = 0; i <= j; ++i) hc[i] = hcol[i];
-        if (nw < 10.0 * nstart * std::sqrt(2.220446049250313e-16)) {
-          gm_mdot(Ns, w, V, Ns, j + 1, coef, rb, s_);
-          gm_maxpy_norm(Ns, w, V, Ns, j + 1, coef, scal.p + 1, rb, s_);
-          ATMO_HIP_CHECK(hipMemcpyAsync(hcol.data(), coef, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, s_));
-          ATMO_HIP_CHECK(hipMemcpyAsync(hscal, scal.p, sizeof(double) * 2, hipMemcpyDeviceToHost, s_));
+        for (int i = 0; i <= j; ++i) hc[i] = hs[j] * hs[i] * r[i];
+        double nraw = r[GM_REC - 1];
+        if (nraw < reorth_fac * std::sqrt(r[j + 1])) {
+          // second projection of W = V_{j+1}; iterations queued after j used the first one
           sync();
-          for (int i = 0; i <= j; ++i) hc[i] += hcol[i];
-          nw = hscal[1];
+          double* W = V + (long)(j + 1) * Ns;
+          double* c2 = gm_coef.p;
+          gm_mdot(Ns, W, V, Ns, j + 1, c2, rb, s_);
+          gm_maxpy_norm(Ns, W, V, Ns, j + 1, c2, sig, scal.p + 1, sig + j + 1, rb, s_);
+          ATMO_HIP_CHECK(hipMemcpyAsync(hscal + 2, c2, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, s_));
+          ATMO_HIP_CHECK(hipMemcpyAsync(hscal + 1, scal.p + 1, sizeof(double), hipMemcpyDeviceToHost, s_));
+          sync();
+          for (int i = 0; i <= j; ++i) hc[i] += hs[j] * hs[i] * hscal[2 + i];
+          nraw = hscal[1];
+          queued = j + 1;
         }
-        hc[j + 1] = nw;
-        if (nw != 0.0) vec_axpby(Ns, 1.0 / nw, w, 0.0, w, s_);
+        hs[j + 1] = nraw > 0.0 ? 1.0 / nraw : 0.0;
+        hc[j + 1] = hs[j] * nraw;  // |w| of the actual (normalized-basis) Arnoldi vector
         for (int i = 0; i < j; ++i) {
           const double t = hc[i];
           hc[i] = cs[i] * t + sn[i] * hc[i + 1];
           hc[i + 1] = -sn[i] * t + cs[i] * hc[i + 1];
         }
-        const double r = 1.0 / std::sqrt(hc[j] * hc[j] + hc[j + 1] * hc[j + 1]);
-        sn[j] = hc[j + 1] * r;
-        cs[j] = hc[j] * r;
+        const double rr = 1.0 / std::sqrt(hc[j] * hc[j] + hc[j + 1] * hc[j + 1]);
+        sn[j] = hc[j + 1] * rr;
+        cs[j] = hc[j] * rr;
         hc[j] = cs[j] * hc[j] + sn[j] * hc[j + 1];
         gam[j + 1] = -sn[j] * gam[j];
         gam[j] *= cs[j];
@@ -919,12 +958,17 @@ class EngineK : public Engine {
         for (int l = i + 1; l < dim; ++l) sacc -= H[(size_t)l * (m + 1) + i] * y[l];
         y[i] = sacc / H[(size_t)i * (m + 1) + i];
       }
-      ATMO_HIP_CHECK(hipMemcpyAsync(coef, y.data(), sizeof(double) * dim, hipMemcpyHostToDevice, s_));
-      gm_update(Ns, x, V, Ns, dim, coef, s_);
-      sync();  // y lives on the host stack
+      // x += sum_i y_i v_i = sum_i (y_i sig_i) V_i; the pinned coefficients are
+      // not rewritten before the next drain of the stream
+      for (int i = 0; i < dim; ++i) hy[i] = y[i] * hs[i];
+      ATMO_HIP_CHECK(hipMemcpyAsync(gm_coef.p, hy, sizeof(double) * dim, hipMemcpyHostToDevice, s_));
+      gm_update(Ns, x, V, Ns, dim, gm_coef.p, s_);
       if (state != 0) break;
     }
-    if (state == -1) throw NoConvergence("GMRES: no convergence");
+    if (state == -1) {
+      sync();
+      throw NoConvergence("GMRES: no convergence");
+    }
     st_.gmres_its += acc;
     return acc;
   }
@@ -1061,6 +1105,7 @@ size_t engine_device_bytes(int k, const MeshDesc& mesh, const EngineParams& p) {
   b += 8 * 2 * Ns * D + Ns * D;           // CG / Schur / inner work
   if (p.precision == 1) b += (size_t)mesh.nx * mesh.nz * nd2 * (nd2 + 1) / 2 * D;  // packed FP64 cell-block inverses of A
   b += (size_t)(p.gmres_restart + 2) * Ns * D + Ns * D + (MAX_GMRES + 2) * D;
+  b += (size_t)MAX_GMRES * GM_REC * D + (MAX_GMRES + 2) * D;  // GMRES records and basis scales
   b += 10 * Ns * D + 7 * 2 * Ns * D;      // IMEX state and stage fields
   return b;
 }

@@ -17,6 +17,9 @@ constexpr int RED_THREADS = 256;
 constexpr int RED_MAX_BLOCKS = 1024;
 constexpr int MAX_GMRES = 64;
 constexpr int RED_MAX_SLOTS = MAX_GMRES + 2;
+// GMRES record of iteration j: w.V_0 .. w.V_{j+1}, then |w| after the
+// projection in the last slot
+constexpr int GM_REC = MAX_GMRES + 3;
 
 // Partial buffers: gen[RED_MAX_SLOTS][RED_MAX_BLOCKS] for host-driven
 // reductions, pa[RED_MAX_BLOCKS] and pb[2][2][RED_MAX_BLOCKS] for the fused
@@ -90,9 +93,12 @@ void build_block_inverse(const MeshDev& m, const double* rho, double* Ainv, hipS
 // --- GMRES pieces
 // out[i] = w . V_i for i < nv (V column-major with leading dimension ldv)
 void gm_mdot(long n, const double* w, const double* V, long ldv, int nv, double* out, RedBuf& rb, hipStream_t s);
-// w -= sum_i coef[i] V_i (coef device array) and *norm = |w| of the result (device scalar)
-void gm_maxpy_norm(long n, double* w, const double* V, long ldv, int nv, const double* coef, double* norm, RedBuf& rb,
-                   hipStream_t s);
+// w -= sum_i sig[i]^2 dots[i] V_i (device arrays) and *norm = |w| of the
+// result, *inv_norm = 1/|w| (0 for w = 0) (device scalars)
+void gm_maxpy_norm(long n, double* w, const double* V, long ldv, int nv, const double* dots, const double* sig,
+                   double* norm, double* inv_norm, RedBuf& rb, hipStream_t s);
+// *norm = |v|, *inv_norm = 1/|v| (0 for v = 0) (device scalars)
+void gm_norm(long n, const double* v, double* norm, double* inv_norm, RedBuf& rb, hipStream_t s);
 // x += sum_i y[i] V_i
 void gm_update(long n, double* x, const double* V, long ldv, int nv, const double* y, hipStream_t s);
 

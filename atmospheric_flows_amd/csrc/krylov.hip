@@ -87,21 +87,25 @@ __device__ inline void cg_check(CGState* st, double res) {
 
 enum FinMode {
   FIN_DOT = 0, FIN_SQRT, FIN_MAX, FIN_CG_ALPHA, FIN_CG_GH, FIN_CG_BETA, FIN_CG_UPDATE, FIN_CG_INIT,
-  FIN_CG_INIT_TOL, FIN_JCG_INIT, FIN_MDOT
+  FIN_CG_INIT_TOL, FIN_JCG_INIT, FIN_MDOT, FIN_GM_NORM
 };
 
-// One block: sums nslot partial slots of G blocks and finalizes.
-__global__ void k_fin(const double* __restrict__ part, int G, int nslot, int mode, double* out, CGState* st) {
+// One block: sums each of the nslot partial slots over its G blocks in
+// block order (one wave per slot, lanes strided over the blocks, then a
+// fixed shuffle tree) and finalizes. FIN_GM_NORM: *out = |v|, *out2 = 1/|v|
+// (0 for a zero vector).
+__global__ void k_fin(const double* __restrict__ part, int G, int nslot, int mode, double* out, CGState* st,
+                      double* out2) {
   __shared__ double s_sum[RED_MAX_SLOTS];
-  for (int k = 0; k < nslot; ++k) {
-    double v[1] = {mode == FIN_MAX ? 0.0 : 0.0};
-    for (int b = threadIdx.x; b < G; b += blockDim.x) {
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, nw = blockDim.x >> 6;
+  for (int k = w; k < nslot; k += nw) {
+    double v = 0.0;
+    for (int b = l; b < G; b += 64) {
       const double p = part[(long)k * RED_MAX_BLOCKS + b];
-      v[0] = mode == FIN_MAX ? fmax(v[0], p) : v[0] + p;
+      v = mode == FIN_MAX ? fmax(v, p) : v + p;
     }
-    if (mode == FIN_MAX) block_reduce<1, true>(v);
-    else block_reduce<1, false>(v);
-    if (threadIdx.x == 0) s_sum[k] = v[0];
+    v = mode == FIN_MAX ? wave_max(v) : wave_sum(v);
+    if (l == 0) s_sum[k] = v;
   }
   __syncthreads();
   if (threadIdx.x != 0) return;
@@ -109,6 +113,12 @@ __global__ void k_fin(const double* __restrict__ part, int G, int nslot, int mod
   switch (mode) {
     case FIN_DOT: *out = s0; break;
     case FIN_SQRT: *out = sqrt(s0); break;
+    case FIN_GM_NORM: {
+      const double r = sqrt(s0);
+      *out = r;
+      *out2 = r > 0.0 ? 1.0 / r : 0.0;
+      break;
+    }
     case FIN_MAX: *out = s0; break;
     case FIN_CG_ALPHA:
       if (!st->done) { st->dh = s0; st->alpha = st->gh / s0; }
@@ -150,8 +160,9 @@ __global__ void k_fin(const double* __restrict__ part, int G, int nslot, int mod
   }
 }
 
-static void fin(const double* part, int G, int nslot, int mode, double* out, CGState* st, hipStream_t s) {
-  hipLaunchKernelGGL(k_fin, dim3(1), dim3(RED_THREADS), 0, s, part, G, nslot, mode, out, st);
+static void fin(const double* part, int G, int nslot, int mode, double* out, CGState* st, hipStream_t s,
+                double* out2 = nullptr) {
+  hipLaunchKernelGGL(k_fin, dim3(1), dim3(RED_THREADS), 0, s, part, G, nslot, mode, out, st, out2);
   ATMO_HIP_CHECK(hipGetLastError());
 }
 
@@ -645,14 +656,20 @@ __global__ void k_maxpy(long n, double* __restrict__ w, const double* __restrict
   }
 }
 
-// w -= sum_k coef[k] V_k and partial |w|^2 of the result in the same pass.
-// PAIRS: two consecutive entries per thread and step (16-byte accesses);
-// four independent partial sums over k keep several loads in flight.
+// w -= sum_k c_k V_k with c_k = sig[k]^2 dots[k] (the projection on the
+// unnormalized basis vectors V_k = v_k / sig[k]) and partial |w|^2 of the
+// result in the same pass. PAIRS: two consecutive entries per thread and
+// step (16-byte accesses); four independent partial sums over k keep
+// several loads in flight.
 template <bool PAIRS>
 __global__ __launch_bounds__(RED_THREADS) void k_maxpy_norm(long n, double* __restrict__ w,
                                                              const double* __restrict__ V, long ldv, int nv,
-                                                             const double* __restrict__ coef, double* part) {
+                                                             const double* __restrict__ dots,
+                                                             const double* __restrict__ sig, double* part) {
   constexpr int E = PAIRS ? 2 : 1;
+  __shared__ double coef[RED_MAX_SLOTS];
+  for (int k = threadIdx.x; k < nv; k += blockDim.x) coef[k] = sig[k] * sig[k] * dots[k];
+  __syncthreads();
   double acc[1] = {0.0};
   const long nstep = n / E;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nstep; i += (long)gridDim.x * blockDim.x) {
@@ -688,16 +705,24 @@ __global__ __launch_bounds__(RED_THREADS) void k_maxpy_norm(long n, double* __re
   if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
 }
 
-void gm_maxpy_norm(long n, double* w, const double* V, long ldv, int nv, const double* coef, double* norm, RedBuf& rb,
-                   hipStream_t s) {
+void gm_maxpy_norm(long n, double* w, const double* V, long ldv, int nv, const double* dots, const double* sig,
+                   double* norm, double* inv_norm, RedBuf& rb, hipStream_t s) {
+  if (nv > RED_MAX_SLOTS) throw std::invalid_argument("gm_maxpy_norm: too many vectors");
   const int G = red_blocks(n);
   const bool pairs = ((n | ldv) & 1) == 0 && ((reinterpret_cast<uintptr_t>(w) | reinterpret_cast<uintptr_t>(V)) & 15) == 0;
   if (pairs)
-    hipLaunchKernelGGL(k_maxpy_norm<true>, dim3(G), dim3(RED_THREADS), 0, s, n, w, V, ldv, nv, coef, rb.gen);
+    hipLaunchKernelGGL(k_maxpy_norm<true>, dim3(G), dim3(RED_THREADS), 0, s, n, w, V, ldv, nv, dots, sig, rb.gen);
   else
-    hipLaunchKernelGGL(k_maxpy_norm<false>, dim3(G), dim3(RED_THREADS), 0, s, n, w, V, ldv, nv, coef, rb.gen);
+    hipLaunchKernelGGL(k_maxpy_norm<false>, dim3(G), dim3(RED_THREADS), 0, s, n, w, V, ldv, nv, dots, sig, rb.gen);
   ATMO_HIP_CHECK(hipGetLastError());
-  fin(rb.gen, G, 1, FIN_SQRT, norm, nullptr, s);
+  fin(rb.gen, G, 1, FIN_GM_NORM, norm, nullptr, s, inv_norm);
+}
+
+void gm_norm(long n, const double* v, double* norm, double* inv_norm, RedBuf& rb, hipStream_t s) {
+  const int G = red_blocks(n);
+  hipLaunchKernelGGL(k_dot_part, dim3(G), dim3(RED_THREADS), 0, s, n, v, v, rb.gen, (const CGState*)nullptr);
+  ATMO_HIP_CHECK(hipGetLastError());
+  fin(rb.gen, G, 1, FIN_GM_NORM, norm, nullptr, s, inv_norm);
 }
 
 void gm_update(long n, double* x, const double* V, long ldv, int nv, const double* y, hipStream_t s) {

@@ -22,13 +22,11 @@ void launch_prolongate_add(const MeshDev& mf, const MeshDev& mc, const T* coarse
 template <int K>
 void launch_project_down(const MeshDev& mf, const MeshDev& mc, const double* fine, double* coarse, hipStream_t s);
 
-// Chebyshev updates (P = inverse diagonal):
-// first step from zero guess: d = P b / theta ; x = d
+// Chebyshev updates (P = inverse diagonal): the first step from a zero
+// guess, d = P b / theta ; x = d (the later steps need A x and are fused
+// with it: launch_mass_cheb)
 template <typename T>
 void cheb_first(long n, const T* b, const T* P, T* x, T* d, T inv_theta, hipStream_t s);
-// d = f1 d + f2 P (b - t) ; x += d
-template <typename T>
-void cheb_step(long n, const T* b, const T* t, const T* P, T* x, T* d, T f1, T f2, hipStream_t s);
 
 // Single-workgroup CG on the coarsest level (FP64, identity preconditioner,
 // absolute tolerance read from *tol, at most max_its iterations). Work

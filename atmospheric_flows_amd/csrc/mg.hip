@@ -177,24 +177,8 @@ __global__ void k_cheb_first(long n, const T* __restrict__ b, const T* __restric
 }
 
 template <typename T>
-__global__ void k_cheb_step(long n, const T* __restrict__ b, const T* __restrict__ t, const T* __restrict__ P,
-                            T* __restrict__ x, T* __restrict__ d, T f1, T f2) {
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    const T v = f1 * d[i] + f2 * P[i] * (b[i] - t[i]);
-    d[i] = v;
-    x[i] += v;
-  }
-}
-
-template <typename T>
 void cheb_first(long n, const T* b, const T* P, T* x, T* d, T inv_theta, hipStream_t s) {
   hipLaunchKernelGGL(k_cheb_first<T>, dim3(grid_for(n)), dim3(256), 0, s, n, b, P, x, d, inv_theta);
-  ATMO_HIP_CHECK(hipGetLastError());
-}
-
-template <typename T>
-void cheb_step(long n, const T* b, const T* t, const T* P, T* x, T* d, T f1, T f2, hipStream_t s) {
-  hipLaunchKernelGGL(k_cheb_step<T>, dim3(grid_for(n)), dim3(256), 0, s, n, b, t, P, x, d, f1, f2);
   ATMO_HIP_CHECK(hipGetLastError());
 }
 
@@ -326,9 +310,5 @@ ATMO_MG_INST(3)
 
 template void cheb_first<double>(long, const double*, const double*, double*, double*, double, hipStream_t);
 template void cheb_first<float>(long, const float*, const float*, float*, float*, float, hipStream_t);
-template void cheb_step<double>(long, const double*, const double*, const double*, double*, double*, double, double,
-                                hipStream_t);
-template void cheb_step<float>(long, const float*, const float*, const float*, float*, float*, float, float,
-                               hipStream_t);
 
 }  // namespace atmo
