@@ -144,15 +144,21 @@ __device__ inline double face_trace(const double* __restrict__ u, int f, const d
 // column (neighbour traces are read from memory; they are L2/MALL hits)
 // and leaves the column partials in sh.T / sh.F. Ends with a block
 // barrier.
-template <int K, class Ph, bool AMR>
+struct NoPrefetch {
+  __device__ void operator()() const {}
+};
+
+// pre(): issued after the staging loads and before the first barrier (a
+// caller's own global loads, in flight during the column phase).
+template <int K, class Ph, bool AMR, class Pre = NoPrefetch>
 __device__ inline void cellface_columns(const MeshDev& m, const Ph& ph, const FieldPtrs<Ph::NF>& in,
-                                        CFSmem<K, Ph, AMR>& sh) {
+                                        CFSmem<K, Ph, AMR>& sh, const Pre& pre = Pre()) {
   using C = Cfg<K>;
   constexpr int ND = C::ND, NQ = C::NQ, ND2 = C::ND2;
   constexpr int NF = Ph::NF, NO = Ph::NO;
   LaneMap<K> L(m.ncell);
-#pragma unroll
-  for (int f = 0; f < NF; ++f) stage_cells<K, double, double, CFSmem<K, Ph, AMR>::CS>(sh.in[f], in.p[f], m.ncell);
+  stage_fields<K, NF, double, double, CFSmem<K, Ph, AMR>::CS>(&sh.in[0][0], in.p, m.ncell);
+  pre();
   __syncthreads();
   if (L.active) {
     const CellVerts v = cell_verts(m, L.cell);

@@ -176,17 +176,35 @@ struct LaneMap {
   }
 };
 
-// Staging of the CPW cells of this wave: dof d of block cell w goes to
-// lds[w*STRIDE + d] (STRIDE > ND2 pads the cells apart in the LDS banks).
-template <int K, typename T, typename S, int STRIDE = Cfg<K>::ND2>
-__device__ inline void stage_cells(T* __restrict__ lds, const S* __restrict__ src, int ncell) {
-  constexpr int ND2 = Cfg<K>::ND2, N = Cfg<K>::CPW * ND2;
-  const long base = (long)blockIdx.x * Cfg<K>::CPW * ND2;
+// Staging of NF fields of the CPW cells of this wave (64 threads): dof d of
+// block cell w of field f goes to lds[f*CPW*STRIDE + w*STRIDE + d] (STRIDE >
+// ND2 pads the cells apart in the LDS banks). The passes have a fixed trip
+// count and every load of a group of up to four fields is issued before the
+// group's first LDS write, so the group costs one memory latency.
+template <int K, int NF, typename T, typename S, int STRIDE = Cfg<K>::ND2>
+__device__ inline void stage_fields(T* __restrict__ lds, const S* const (&src)[NF], int ncell) {
+  constexpr int ND2 = Cfg<K>::ND2, N = Cfg<K>::CPW * ND2, NPASS = (N + 63) / 64, FS = Cfg<K>::CPW * STRIDE;
+  constexpr int G = 4;
+  const long base = (long)blockIdx.x * N;
   const long lim = (long)ncell * ND2;
-  for (int t = threadIdx.x; t < N; t += 64) {
-    long g = base + t;
-    const int dst = STRIDE == ND2 ? t : (t / ND2) * STRIDE + t % ND2;
-    lds[dst] = (g < lim) ? (T)src[g] : (T)0;
+#pragma unroll
+  for (int f0 = 0; f0 < NF; f0 += G) {
+    T v[G][NPASS];
+#pragma unroll
+    for (int f = 0; f < G; ++f)
+#pragma unroll
+      for (int it = 0; it < NPASS; ++it) {
+        const int t = threadIdx.x + it * 64;
+        const long g = base + t;
+        v[f][it] = (f0 + f < NF && t < N && g < lim) ? (T)src[f0 + f < NF ? f0 + f : 0][g] : (T)0;
+      }
+#pragma unroll
+    for (int f = 0; f < G; ++f)
+#pragma unroll
+      for (int it = 0; it < NPASS; ++it) {
+        const int t = threadIdx.x + it * 64;
+        if (f0 + f < NF && t < N) lds[(f0 + f) * FS + (STRIDE == ND2 ? t : (t / ND2) * STRIDE + t % ND2)] = v[f][it];
+      }
   }
 }
 

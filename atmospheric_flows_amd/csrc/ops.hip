@@ -40,14 +40,16 @@ __global__ __launch_bounds__(64) void k_mass(MeshDev m, const T* __restrict__ sr
                                              long Ns) {
   using C = Cfg<K>;
   constexpr int ND2 = C::ND2, CPW = C::CPW;
-  __shared__ T s_u[NC][CPW * ND2];
-  __shared__ T s_r[W ? CPW * ND2 : 1];
+  constexpr int NS = NC + (W ? 1 : 0);
+  __shared__ T s_u[NS][CPW * ND2];  // the components, then rho when W
   __shared__ T s_red[NC][CPW][C::ND][C::NQ];
+  const T* srcs[NS];
 #pragma unroll
-  for (int comp = 0; comp < NC; ++comp) stage_cells<K>(s_u[comp], src + comp * Ns, m.ncell);
-  if (W) stage_cells<K>(s_r, rho, m.ncell);
+  for (int comp = 0; comp < NC; ++comp) srcs[comp] = src + comp * Ns;
+  if constexpr (W) srcs[NS - 1] = rho;
+  stage_fields<K, NS, T, T>(&s_u[0][0], srcs, m.ncell);
   __syncthreads();
-  wave_mass_core<K, T, NC, W>(m, blockIdx.x * CPW, &s_u[0][0], s_r, coeff, &s_red[0][0][0][0]);
+  wave_mass_core<K, T, NC, W>(m, blockIdx.x * CPW, &s_u[0][0], &s_u[NS - 1][0], coeff, &s_red[0][0][0][0]);
   __syncthreads();
   const long base = (long)blockIdx.x * CPW * ND2;
   const long lim = (long)m.ncell * ND2;
@@ -77,14 +79,16 @@ __global__ __launch_bounds__(64) void k_mass_cheb(MeshDev m, T* __restrict__ x, 
                                                   T* __restrict__ d, T f1, T f2, long Ns) {
   using C = Cfg<K>;
   constexpr int ND2 = C::ND2, CPW = C::CPW;
-  __shared__ T s_u[NC][CPW * ND2];
-  __shared__ T s_r[W ? CPW * ND2 : 1];
+  constexpr int NS = NC + (W ? 1 : 0);
+  __shared__ T s_u[NS][CPW * ND2];  // the components of x, then rho when W
   __shared__ T s_red[NC][CPW][C::ND][C::NQ];
+  const T* srcs[NS];
 #pragma unroll
-  for (int comp = 0; comp < NC; ++comp) stage_cells<K>(s_u[comp], x + comp * Ns, m.ncell);
-  if (W) stage_cells<K>(s_r, rho, m.ncell);
+  for (int comp = 0; comp < NC; ++comp) srcs[comp] = x + comp * Ns;
+  if constexpr (W) srcs[NS - 1] = rho;
+  stage_fields<K, NS, T, T>(&s_u[0][0], srcs, m.ncell);
   __syncthreads();
-  wave_mass_core<K, T, NC, W>(m, blockIdx.x * CPW, &s_u[0][0], s_r, coeff, &s_red[0][0][0][0]);
+  wave_mass_core<K, T, NC, W>(m, blockIdx.x * CPW, &s_u[0][0], &s_u[NS - 1][0], coeff, &s_red[0][0][0][0]);
   __syncthreads();
   const long base = (long)blockIdx.x * CPW * ND2;
   const long lim = (long)m.ncell * ND2;
@@ -114,7 +118,10 @@ __global__ __launch_bounds__(64) void k_inv_diag(MeshDev m, int ncomp, const T* 
   __shared__ T s_r[W ? CPW * ND2 : 1];
   __shared__ T s_red[CPW][ND][NQ];
   LaneMap<K> L(m.ncell);
-  if (W) stage_cells<K>(s_r, rho, m.ncell);
+  if constexpr (W) {
+    const T* s1[1] = {rho};
+    stage_fields<K, 1, T, T>(s_r, s1, m.ncell);
+  }
   __syncthreads();
   if (L.active) {
     const CellVerts v = cell_verts(m, L.cell);

@@ -9,29 +9,33 @@
 
 namespace atmo {
 
-// Stages the packed inverse blocks of nc consecutive cells from Ab into LDS
-// (cell w at w * LDS_STRIDE): coalesced loads, all issued before the first
-// LDS write.
+// The packed inverse blocks of nc consecutive cells, loaded coalesced into
+// registers (load) and written to LDS (store: cell w at w * LDS_STRIDE).
 template <int K, int NT, int MAXC>
-__device__ inline void stage_sym_blocks(double* __restrict__ lds, const double* __restrict__ Ab, int nc) {
+struct SymBlockStage {
   using SB = SymBlock<K>;
-  constexpr int NPASS = (MAXC * SB::SIZE + NT - 1) / NT;
-  const int lim = nc * SB::SIZE;
+  static constexpr int NPASS = (MAXC * SB::SIZE + NT - 1) / NT;
   double v[NPASS];
+  __device__ void load(const double* __restrict__ Ab, int nc) {
+    const int lim = nc * SB::SIZE;
 #pragma unroll
-  for (int it = 0; it < NPASS; ++it) {
-    const int e = threadIdx.x + it * NT;
-    v[it] = e < lim ? Ab[e] : 0.0;
-  }
-#pragma unroll
-  for (int it = 0; it < NPASS; ++it) {
-    const int e = threadIdx.x + it * NT;
-    if (e < lim) {
-      const int w = e / SB::SIZE;
-      lds[e + w * (SB::LDS_STRIDE - SB::SIZE)] = v[it];
+    for (int it = 0; it < NPASS; ++it) {
+      const int e = threadIdx.x + it * NT;
+      v[it] = e < lim ? Ab[e] : 0.0;
     }
   }
-}
+  __device__ void store(double* __restrict__ lds, int nc) const {
+    const int lim = nc * SB::SIZE;
+#pragma unroll
+    for (int it = 0; it < NPASS; ++it) {
+      const int e = threadIdx.x + it * NT;
+      if (e < lim) {
+        const int w = e / SB::SIZE;
+        lds[e + w * (SB::LDS_STRIDE - SB::SIZE)] = v[it];
+      }
+    }
+  }
+};
 
 template <int K, bool AMR>
 __global__ __launch_bounds__(64) void k_b_ainv(MeshDev m, PhysB ph, const double* __restrict__ p,
@@ -45,8 +49,11 @@ __global__ __launch_bounds__(64) void k_b_ainv(MeshDev m, PhysB ph, const double
   } sh;
   __shared__ double s_t[2][CPW * CS];
   FieldPtrs<1> in{{p}};
-  cellface_columns<K, PhysB, AMR>(m, ph, in, sh.cf);
   const int cell0 = blockIdx.x * CPW;
+  const int nc = min(CPW, m.ncell - cell0);
+  // the inverse blocks are loaded while B p is formed
+  SymBlockStage<K, 64, CPW> blk;
+  cellface_columns<K, PhysB, AMR>(m, ph, in, sh.cf, [&]() { blk.load(Ainv + (long)cell0 * SB::SIZE, nc); });
   OutRows<K> R = out_rows<K>(threadIdx.x);
   for (int t = threadIdx.x; t < CPW * ND2; t += 64) {
     if (!rows_invariant<K>()) R = out_rows<K>(t);
@@ -55,8 +62,7 @@ __global__ __launch_bounds__(64) void k_b_ainv(MeshDev m, PhysB ph, const double
     s_t[1][st] = cellface_out<K, PhysB, AMR>(sh.cf, 1, t, R);
   }
   __syncthreads();
-  const int nc = min(CPW, m.ncell - cell0);
-  stage_sym_blocks<K, 64, CPW>(sh.A, Ainv + (long)cell0 * SB::SIZE, nc);
+  blk.store(sh.A, nc);
   __syncthreads();
   for (int t = threadIdx.x; t < CPW * ND2; t += 64) {
     const int cw = t / ND2, a = t - cw * ND2;
@@ -91,7 +97,9 @@ __global__ __launch_bounds__(256) void k_ainv_apply(int ncell, const double* __r
     s_b[0][cw * CS + a] = b[gi];
     s_b[1][cw * CS + a] = b[Ns + gi];
   }
-  stage_sym_blocks<K, 256, CB>(s_A, Ainv + cell0 * SB::SIZE, nc);
+  SymBlockStage<K, 256, CB> blk;
+  blk.load(Ainv + cell0 * SB::SIZE, nc);
+  blk.store(s_A, nc);
   __syncthreads();
   if (t >= CB * ND2 || cw >= nc) return;
   const double* Ac = s_A + cw * SB::LDS_STRIDE;
