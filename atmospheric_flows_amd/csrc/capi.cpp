@@ -99,6 +99,7 @@ static void check_params(const atmo_params& p) {
   if (p.max_iterations < 1 || p.schur_inner_max < 1 || p.fp_max_iter < 1 || p.cheb_degree < 1 ||
       p.cheb_eig_its < 1)
     throw std::invalid_argument("params: iteration limits must be >= 1");
+  if (p.cheb_eig_its > MAX_EIG_ITS) throw std::invalid_argument("params: cheb_eig_its must be <= 256");
   if (p.gmres_restart < 1 || p.gmres_restart > MAX_GMRES - 2)
     throw std::invalid_argument("params: gmres_restart must be in [1, 62]");
   if (p.precision != 0 && p.precision != 1) throw std::invalid_argument("params: precision must be 0 or 1");

@@ -144,6 +144,7 @@ class EngineK : public Engine {
     rb.bind(red_partials.p);
     cgst.alloc(4);
     cgst.zero(s_);
+    eig_hist.alloc(2 * MAX_EIG_ITS);
     scal.alloc(256);
     scal.zero(s_);
     coarse_its.alloc(1);
@@ -154,6 +155,7 @@ class EngineK : public Engine {
     ATMO_HIP_CHECK(hipHostMalloc((void**)&hscal, sizeof(double) * 256, hipHostMallocDefault));
     ATMO_HIP_CHECK(hipHostMalloc((void**)&hint, sizeof(int) * 4, hipHostMallocDefault));
     ATMO_HIP_CHECK(hipHostMalloc((void**)&hgm, sizeof(double) * GM_HOST, hipHostMallocDefault));
+    ATMO_HIP_CHECK(hipHostMalloc((void**)&heig, sizeof(double) * 2 * MAX_EIG_ITS, hipHostMallocDefault));
     for (auto& e : gm_ev) ATMO_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     rho_fix.alloc(Ns);
     pres_fix.alloc(Ns);
@@ -190,6 +192,7 @@ class EngineK : public Engine {
     if (hint) (void)hipHostFree(hint);
     (void)hipStreamSynchronize(s_);
     if (hgm) (void)hipHostFree(hgm);
+    if (heig) (void)hipHostFree(heig);
     for (auto& e : gm_ev)
       if (e) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(s_);
@@ -438,6 +441,8 @@ class EngineK : public Engine {
   // update coefficients, and one event per iteration
   static constexpr int GM_HOST = MAX_GMRES * GM_REC + MAX_GMRES + 2;
   double* hgm = nullptr;
+  double* heig = nullptr;  // pinned copy of eig_hist
+  DBuf<double> eig_hist;   // CG coefficients of the eigenvalue estimate: alpha_it, beta_it
   hipEvent_t gm_ev[MAX_GMRES] = {};
   DBuf<double> gm_rec, gm_sig;
   int hyper = 1, nsst = 1;
@@ -617,6 +622,12 @@ class EngineK : public Engine {
   // Chebyshev eigenvalue estimate: <= cheb_eig_its CG iterations with the
   // Jacobi preconditioner from b = (i % 11) - mean, Lanczos tridiagonal,
   // lmax = 1.2 * estimate, lmin = lmax / range.
+  //
+  // Every iteration is queued without a host round trip: the CG kernels
+  // are no-ops once the device-side state says the solve has finished, and
+  // each iteration's alpha and beta are recorded on the device (eig_hist);
+  // the host reads them back once. The iterations that ran are the first
+  // st.it (the one that finished the solve included).
   void estimate_eig(int l, int blk) {
     Level& L = *lv[l];
     const long n = (long)ncomp(blk) * L.Ns;
@@ -624,31 +635,30 @@ class EngineK : public Engine {
     CGState* st = cgst.p + 2;
     double *g = L.x64.p, *d = L.b64.p, *h = L.t64.p, *x = L.d64.p;
     const double* Pd = L.inv64[blk].p;
-    hipLaunchKernelGGL(k_set_cg, dim3(1), dim3(1), 0, s_, st, 1e-2, P.cheb_eig_its, 0.0);
+    double* hist = eig_hist.p;
+    const int nit = P.cheb_eig_its;
+    hipLaunchKernelGGL(k_set_cg, dim3(1), dim3(1), 0, s_, st, 1e-2, nit, 0.0);
     cg_residual(n, g, b, false, s_);
     vec_fill(n, 0.0, x, s_);
     vec_mul(n, g, Pd, h, s_);
     vec_axpby(n, -1.0, h, 0.0, d, s_);
     cg_dot(n, g, h, st, DOT_GH_INIT, rb, s_);
-    std::vector<double> alphas, betas;
-    for (int it = 0; it < P.cheb_eig_its; ++it) {
+    for (int it = 0; it < nit; ++it) {
       level_apply<double>(L, blk, d, h);
-      cg_dot(n, d, h, st, DOT_ALPHA, rb, s_);
+      cg_dot(n, d, h, st, DOT_ALPHA, rb, s_, hist + 2 * it);
       cg_update(n, x, g, d, h, st, rb, s_);
-      read_state(2);
-      alphas.push_back(hst[2].alpha);
-      if (hst[2].done) break;
+      if (it + 1 == nit) break;  // the last beta is never used
       vec_mul(n, g, Pd, h, s_);
-      cg_dot(n, g, h, st, DOT_BETA, rb, s_);
+      cg_dot(n, g, h, st, DOT_BETA, rb, s_, hist + 2 * it + 1);
       cg_dir(n, d, h, st, s_);
-      read_state(2);
-      betas.push_back(hst[2].beta);
     }
-    const int m = (int)alphas.size();
+    ATMO_HIP_CHECK(hipMemcpyAsync(heig, hist, sizeof(double) * 2 * nit, hipMemcpyDeviceToHost, s_));
+    read_state(2);
+    const int m = std::min(nit, hst[2].it);
     std::vector<double> a(m), o(m > 0 ? m - 1 : 0);
     for (int j = 0; j < m; ++j) {
-      a[j] = 1.0 / alphas[j] + (j > 0 ? betas[j - 1] / alphas[j - 1] : 0.0);
-      if (j + 1 < m) o[j] = std::sqrt(betas[j]) / alphas[j];
+      a[j] = 1.0 / heig[2 * j] + (j > 0 ? heig[2 * (j - 1) + 1] / heig[2 * (j - 1)] : 0.0);
+      if (j + 1 < m) o[j] = std::sqrt(heig[2 * j + 1]) / heig[2 * j];
     }
     double lmin, lmax;
     tridiag_extreme(a, o, lmin, lmax);
@@ -1101,6 +1111,7 @@ size_t engine_device_bytes(int k, const MeshDesc& mesh, const EngineParams& p) {
   }
   const size_t Ns = (size_t)mesh.nx * mesh.nz * nd2;
   b += RedBuf::doubles() * D + 4 * sizeof(CGState) + 256 * D + sizeof(int) + DIAG_PARTIALS * D + DG_SLOTS * D;
+  b += 2 * MAX_EIG_ITS * D;               // CG coefficients of the eigenvalue estimates
   b += 2 * Ns * D;                        // rho_fix, pres_fix
   b += 8 * 2 * Ns * D + Ns * D;           // CG / Schur / inner work
   if (p.precision == 1) b += (size_t)mesh.nx * mesh.nz * nd2 * (nd2 + 1) / 2 * D;  // packed FP64 cell-block inverses of A

@@ -20,6 +20,9 @@ constexpr int RED_MAX_SLOTS = MAX_GMRES + 2;
 // GMRES record of iteration j: w.V_0 .. w.V_{j+1}, then |w| after the
 // projection in the last slot
 constexpr int GM_REC = MAX_GMRES + 3;
+// iterations of the Chebyshev eigenvalue estimate (their CG coefficients are
+// recorded on the device)
+constexpr int MAX_EIG_ITS = 256;
 
 // Partial buffers: gen[RED_MAX_SLOTS][RED_MAX_BLOCKS] for host-driven
 // reductions, pa[RED_MAX_BLOCKS] and pb[2][2][RED_MAX_BLOCKS] for the fused
@@ -62,7 +65,10 @@ void vec_linf(long n, const double* x, const double* y, double* out, RedBuf& rb,
 
 // --- generic preconditioned CG pieces (host drives matvec/preconditioner)
 enum DotMode { DOT_ALPHA = 0, DOT_GH_INIT = 1, DOT_BETA = 2 };
-void cg_dot(long n, const double* x, const double* y, CGState* st, int mode, RedBuf& rb, hipStream_t s);
+// record (optional, device): receives alpha (DOT_ALPHA) or beta (DOT_BETA)
+// of this iteration when the solve has not finished
+void cg_dot(long n, const double* x, const double* y, CGState* st, int mode, RedBuf& rb, hipStream_t s,
+            double* record = nullptr);
 // x += alpha d ; g += alpha h ; res = |g| ; it++ ; convergence flag
 void cg_update(long n, double* x, double* g, const double* d, const double* h, CGState* st, RedBuf& rb,
                hipStream_t s);

@@ -70,10 +70,19 @@ __device__ inline void write_partials(double (&v)[NS], double* part) {
     for (int k = 0; k < NS; ++k) part[(long)k * RED_MAX_BLOCKS + blockIdx.x] = v[k];
 }
 
-// sum of G partials (one slot), identical in every block (fixed order)
+// sum of G partials (one slot), identical in every block (fixed order);
+// blockDim.x = 256, every load issued before the first add
 __device__ inline double sum_slot(const double* part, int G) {
-  double v[1] = {0.0};
-  for (int b = threadIdx.x; b < G; b += blockDim.x) v[0] += part[b];
+  constexpr int NB = RED_MAX_BLOCKS / RED_THREADS;
+  double p[NB];
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    const int b = threadIdx.x + RED_THREADS * i;
+    p[i] = b < G ? part[b] : 0.0;
+  }
+  double v[1] = {p[0]};
+#pragma unroll
+  for (int i = 1; i < NB; ++i) v[0] += p[i];
   block_reduce<1, false>(v);
   return v[0];
 }
@@ -97,13 +106,18 @@ enum FinMode {
 __global__ void k_fin(const double* __restrict__ part, int G, int nslot, int mode, double* out, CGState* st,
                       double* out2) {
   __shared__ double s_sum[RED_MAX_SLOTS];
+  constexpr int NB = RED_MAX_BLOCKS / 64;  // partials per lane (fixed count: all loads in flight at once)
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63, nw = blockDim.x >> 6;
   for (int k = w; k < nslot; k += nw) {
-    double v = 0.0;
-    for (int b = l; b < G; b += 64) {
-      const double p = part[(long)k * RED_MAX_BLOCKS + b];
-      v = mode == FIN_MAX ? fmax(v, p) : v + p;
+    double p[NB];
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      const int b = l + 64 * i;
+      p[i] = b < G ? part[(long)k * RED_MAX_BLOCKS + b] : 0.0;
     }
+    double v = p[0];
+#pragma unroll
+    for (int i = 1; i < NB; ++i) v = mode == FIN_MAX ? fmax(v, p[i]) : v + p[i];
     v = mode == FIN_MAX ? wave_max(v) : wave_sum(v);
     if (l == 0) s_sum[k] = v;
   }
@@ -121,13 +135,21 @@ __global__ void k_fin(const double* __restrict__ part, int G, int nslot, int mod
     }
     case FIN_MAX: *out = s0; break;
     case FIN_CG_ALPHA:
-      if (!st->done) { st->dh = s0; st->alpha = st->gh / s0; }
+      if (!st->done) {
+        st->dh = s0;
+        st->alpha = st->gh / s0;
+        if (out) *out = st->alpha;
+      }
       break;
     case FIN_CG_GH:
       if (!st->done) st->gh = s0;
       break;
     case FIN_CG_BETA:
-      if (!st->done) { st->beta = s0 / st->gh; st->gh = s0; }
+      if (!st->done) {
+        st->beta = s0 / st->gh;
+        st->gh = s0;
+        if (out) *out = st->beta;
+      }
       break;
     case FIN_CG_UPDATE:
       if (!st->done) { st->it += 1; cg_check(st, sqrt(s0)); }
@@ -259,12 +281,13 @@ void vec_linf(long n, const double* x, const double* y, double* out, RedBuf& rb,
 }
 
 // ------------------------------------------------------------ generic CG
-void cg_dot(long n, const double* x, const double* y, CGState* st, int mode, RedBuf& rb, hipStream_t s) {
+void cg_dot(long n, const double* x, const double* y, CGState* st, int mode, RedBuf& rb, hipStream_t s,
+            double* record) {
   const int G = red_blocks(n);
   hipLaunchKernelGGL(k_dot_part, dim3(G), dim3(RED_THREADS), 0, s, n, x, y, rb.gen, (const CGState*)st);
   ATMO_HIP_CHECK(hipGetLastError());
   const int fm = mode == DOT_ALPHA ? FIN_CG_ALPHA : (mode == DOT_GH_INIT ? FIN_CG_GH : FIN_CG_BETA);
-  fin(rb.gen, G, 1, fm, nullptr, st, s);
+  fin(rb.gen, G, 1, fm, record, st, s);
 }
 
 __global__ void k_cg_update(long n, double* __restrict__ x, double* __restrict__ g, const double* __restrict__ d,
