@@ -164,7 +164,7 @@ class EngineK : public Engine {
     for (auto* b : {&cg_g, &cg_d, &cg_h, &s_t1, &s_t2, &in_g, &in_d, &in_h}) b->alloc(2 * Ns);
     s_t3.alloc(Ns);
     if (P.precision == 1) ainv.alloc((size_t)fine().ncell * SymBlock<K>::SIZE);
-    gm_V.alloc((size_t)(P.gmres_restart + 2) * Ns);
+    gm_V.alloc((size_t)(P.gmres_restart + 2) * gm_ld());
     gm_p.alloc(Ns);
     gm_coef.alloc(MAX_GMRES + 2);
     gm_rec.alloc((size_t)MAX_GMRES * GM_REC);
@@ -222,7 +222,7 @@ class EngineK : public Engine {
     const bool rebuild = p.gmres_restart != P.gmres_restart;
     P = p;
     oc_ = consts();
-    if (rebuild) gm_V.alloc((size_t)(P.gmres_restart + 2) * fine().Ns);
+    if (rebuild) gm_V.alloc((size_t)(P.gmres_restart + 2) * gm_ld());
     if (prec_change) {
       if (P.precision == 1 && !ainv.p) ainv.alloc((size_t)fine().ncell * SymBlock<K>::SIZE);
       refresh_levels_rho();
@@ -360,6 +360,18 @@ class EngineK : public Engine {
           launch_d_minus_c<K>(m, oc_, bc_coeff(), pres_fix.p, u_old.p, p_old.p, fine().inv64[BLK_D].p, gm_p.p, Ns,
                               s_);
           fields = 6;
+          break;
+        case 10:  // GMRES dot products: w = V_13 against V_0 .. V_13 (14 columns, w included)
+        case 11:  // GMRES projection + norm: V_13 -= sum of 13 columns
+          if (P.gmres_restart < 13) throw std::invalid_argument("time_kernel 10/11 need gmres_restart >= 13");
+          if (which == 10) {
+            gm_mdot(Ns, gm_V.p + 13 * gm_ld(), gm_V.p, gm_ld(), 14, gm_rec.p, rb, s_);
+            fields = 15;
+          } else {
+            gm_maxpy_norm(Ns, gm_V.p + 13 * gm_ld(), gm_V.p, gm_ld(), 13, gm_rec.p, gm_sig.p, gm_rec.p + GM_REC - 1,
+                          gm_sig.p + 14, rb, s_);
+            fields = 15;
+          }
           break;
         default:
           throw std::invalid_argument("time_kernel: unknown kernel id");
@@ -876,14 +888,21 @@ class EngineK : public Engine {
   // from there.
   static constexpr int GM_LOOKAHEAD = 2;
 
+  // leading dimension of the basis: the columns are N_s doubles (a power-of-two
+  // byte count on the usual meshes) plus 256 bytes apart, so the streams of
+  // the multi-vector kernels do not all start on the same HBM channel
+  static constexpr long GM_PAD = 32;
+  long gm_ld() const { return fine().Ns + GM_PAD; }
+
   void gm_enqueue(int j, const double* PD) {
     const long Ns = fine().Ns;
+    const long ld = gm_ld();
     double* V = gm_V.p;
-    double* W = V + (long)(j + 1) * Ns;
+    double* W = V + (long)(j + 1) * ld;
     double* rec = gm_rec.p + (long)j * GM_REC;
-    schur_vmult(V + (long)j * Ns, W, PD);
-    gm_mdot(Ns, W, V, Ns, j + 2, rec, rb, s_);  // W.V_i (i <= j) and W.W
-    gm_maxpy_norm(Ns, W, V, Ns, j + 1, rec, gm_sig.p, rec + GM_REC - 1, gm_sig.p + j + 1, rb, s_);
+    schur_vmult(V + (long)j * ld, W, PD);
+    gm_mdot(Ns, W, V, ld, j + 2, rec, rb, s_);  // W.V_i (i <= j) and W.W
+    gm_maxpy_norm(Ns, W, V, ld, j + 1, rec, gm_sig.p, rec + GM_REC - 1, gm_sig.p + j + 1, rb, s_);
     ATMO_HIP_CHECK(hipMemcpyAsync(hgm + (long)j * GM_REC, rec, sizeof(double) * GM_REC, hipMemcpyDeviceToHost, s_));
     ATMO_HIP_CHECK(hipEventRecord(gm_ev[j], s_));
   }
@@ -892,6 +911,7 @@ class EngineK : public Engine {
     const long Ns = fine().Ns;
     const int m = P.gmres_restart;
     const double* PD = fine().inv64[BLK_D].p;
+    const long ld = gm_ld();
     double* V = gm_V.p;
     double* p = gm_p.p;
     double* sig = gm_sig.p;
@@ -935,10 +955,10 @@ class EngineK : public Engine {
         if (nraw < reorth_fac * std::sqrt(r[j + 1])) {
           // second projection of W = V_{j+1}; iterations queued after j used the first one
           sync();
-          double* W = V + (long)(j + 1) * Ns;
+          double* W = V + (long)(j + 1) * ld;
           double* c2 = gm_coef.p;
-          gm_mdot(Ns, W, V, Ns, j + 1, c2, rb, s_);
-          gm_maxpy_norm(Ns, W, V, Ns, j + 1, c2, sig, scal.p + 1, sig + j + 1, rb, s_);
+          gm_mdot(Ns, W, V, ld, j + 1, c2, rb, s_);
+          gm_maxpy_norm(Ns, W, V, ld, j + 1, c2, sig, scal.p + 1, sig + j + 1, rb, s_);
           ATMO_HIP_CHECK(hipMemcpyAsync(hscal + 2, c2, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, s_));
           ATMO_HIP_CHECK(hipMemcpyAsync(hscal + 1, scal.p + 1, sizeof(double), hipMemcpyDeviceToHost, s_));
           sync();
@@ -972,7 +992,7 @@ class EngineK : public Engine {
       // not rewritten before the next drain of the stream
       for (int i = 0; i < dim; ++i) hy[i] = y[i] * hs[i];
       ATMO_HIP_CHECK(hipMemcpyAsync(gm_coef.p, hy, sizeof(double) * dim, hipMemcpyHostToDevice, s_));
-      gm_update(Ns, x, V, Ns, dim, gm_coef.p, s_);
+      gm_update(Ns, x, V, ld, dim, gm_coef.p, s_);
       if (state != 0) break;
     }
     if (state == -1) {
@@ -1115,7 +1135,7 @@ size_t engine_device_bytes(int k, const MeshDesc& mesh, const EngineParams& p) {
   b += 2 * Ns * D;                        // rho_fix, pres_fix
   b += 8 * 2 * Ns * D + Ns * D;           // CG / Schur / inner work
   if (p.precision == 1) b += (size_t)mesh.nx * mesh.nz * nd2 * (nd2 + 1) / 2 * D;  // packed FP64 cell-block inverses of A
-  b += (size_t)(p.gmres_restart + 2) * Ns * D + Ns * D + (MAX_GMRES + 2) * D;
+  b += (size_t)(p.gmres_restart + 2) * (Ns + 32) * D + Ns * D + (MAX_GMRES + 2) * D;  // basis: leading dim N_s + 32
   b += (size_t)MAX_GMRES * GM_REC * D + (MAX_GMRES + 2) * D;  // GMRES records and basis scales
   b += 10 * Ns * D + 7 * 2 * Ns * D;      // IMEX state and stage fields
   return b;

@@ -200,7 +200,10 @@ atmo_status atmo_init_hill_state(const atmo_mesh_desc* mesh, double theta0, doub
  * 5 stage-3 pressure residual, 6 fused inner-CG update (k_jcg_upd),
  * 7 plain device-to-device copy of one velocity field (calibration),
  * 8 B with the cellwise A^{-1} (k_b_ainv, mixed form only), 9 scaled D - C
- * (k_cellface<PhysDminusC>). Also reports the kernel's algorithmic HBM
+ * (k_cellface<PhysDminusC>), 10 GMRES dot products of one vector with 14
+ * basis vectors (k_mdot_part + finalize), 11 GMRES projection on 13 basis
+ * vectors with the norm of the result (k_maxpy_norm + finalize; needs
+ * gmres_restart >= 13 for both). Also reports the kernel's algorithmic HBM
  * bytes per launch. */
 atmo_status atmo_time_kernel(atmo_ctx* ctx, int which, int nrep, double* ms_per_launch, double* bytes_per_launch);
 

@@ -18,8 +18,8 @@ def test_footprint_scales_with_the_mesh_and_precision():
     # N (N + 1) / 2 doubles per cell for N = (k+1)^2
     bm = af.device_bytes(3, 128, 64, precision=1)
     assert bm - b2 == 128 * 64 * 136 * 8
-    # a longer GMRES basis adds exactly its vectors
-    assert af.device_bytes(3, 128, 64, gmres_restart=30) - b2 == 2 * ns2 * 8
+    # a longer GMRES basis adds exactly its vectors (leading dimension N_s + 32)
+    assert af.device_bytes(3, 128, 64, gmres_restart=30) - b2 == 2 * (ns2 + 32) * 8
 
 
 def test_footprint_of_the_headline_mesh_fits_hbm_many_times():
