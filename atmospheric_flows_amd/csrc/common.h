@@ -163,6 +163,18 @@ __device__ inline FaceGeo face_geo(const CellVerts& v, int f) {
 }
 
 // Lane -> (cell, column) for the lane-per-column layout.
+// Block of cells a one-wave workgroup of a cell kernel handles. Workgroups
+// are dispatched round-robin over the chip's 8 XCDs, each with its own L2;
+// this numbering gives each XCD one contiguous range of blocks, so the
+// cells above and below a cell (one mesh row away) are mostly handled by
+// the same XCD shortly before or after it and their face traces are L2 hits.
+// A bijection of [0, gridDim.x).
+__device__ inline int cell_block() {
+  const int G = gridDim.x, b = blockIdx.x, x = b & 7, r = b >> 3;
+  const int q = G >> 3, rem = G & 7;
+  return x * q + (x < rem ? x : rem) + r;
+}
+
 template <int K>
 struct LaneMap {
   int lane, cw, c, cell;
@@ -171,7 +183,7 @@ struct LaneMap {
     lane = threadIdx.x;
     cw = lane / Cfg<K>::NQ;
     c = lane - cw * Cfg<K>::NQ;
-    cell = blockIdx.x * Cfg<K>::CPW + cw;
+    cell = cell_block() * Cfg<K>::CPW + cw;
     active = (cw < Cfg<K>::CPW) && (cell < ncell);
   }
 };
@@ -185,7 +197,7 @@ template <int K, int NF, typename T, typename S, int STRIDE = Cfg<K>::ND2>
 __device__ inline void stage_fields(T* __restrict__ lds, const S* const (&src)[NF], int ncell) {
   constexpr int ND2 = Cfg<K>::ND2, N = Cfg<K>::CPW * ND2, NPASS = (N + 63) / 64, FS = Cfg<K>::CPW * STRIDE;
   constexpr int G = 4;
-  const long base = (long)blockIdx.x * N;
+  const long base = (long)cell_block() * N;
   const long lim = (long)ncell * ND2;
 #pragma unroll
   for (int f0 = 0; f0 < NF; f0 += G) {

@@ -49,9 +49,9 @@ __global__ __launch_bounds__(64) void k_mass(MeshDev m, const T* __restrict__ sr
   if constexpr (W) srcs[NS - 1] = rho;
   stage_fields<K, NS, T, T>(&s_u[0][0], srcs, m.ncell);
   __syncthreads();
-  wave_mass_core<K, T, NC, W>(m, blockIdx.x * CPW, &s_u[0][0], &s_u[NS - 1][0], coeff, &s_red[0][0][0][0]);
+  wave_mass_core<K, T, NC, W>(m, cell_block() * CPW, &s_u[0][0], &s_u[NS - 1][0], coeff, &s_red[0][0][0][0]);
   __syncthreads();
-  const long base = (long)blockIdx.x * CPW * ND2;
+  const long base = (long)cell_block() * CPW * ND2;
   const long lim = (long)m.ncell * ND2;
   T Si[C::NQ];
   lane_cols_S<K, T>(threadIdx.x % C::ND, Si);
@@ -88,9 +88,9 @@ __global__ __launch_bounds__(64) void k_mass_cheb(MeshDev m, T* __restrict__ x, 
   if constexpr (W) srcs[NS - 1] = rho;
   stage_fields<K, NS, T, T>(&s_u[0][0], srcs, m.ncell);
   __syncthreads();
-  wave_mass_core<K, T, NC, W>(m, blockIdx.x * CPW, &s_u[0][0], &s_u[NS - 1][0], coeff, &s_red[0][0][0][0]);
+  wave_mass_core<K, T, NC, W>(m, cell_block() * CPW, &s_u[0][0], &s_u[NS - 1][0], coeff, &s_red[0][0][0][0]);
   __syncthreads();
-  const long base = (long)blockIdx.x * CPW * ND2;
+  const long base = (long)cell_block() * CPW * ND2;
   const long lim = (long)m.ncell * ND2;
   T Si[C::NQ];
   lane_cols_S<K, T>(threadIdx.x % C::ND, Si);
@@ -158,7 +158,7 @@ __global__ __launch_bounds__(64) void k_inv_diag(MeshDev m, int ncomp, const T* 
     }
   }
   __syncthreads();
-  const long base = (long)blockIdx.x * CPW * ND2;
+  const long base = (long)cell_block() * CPW * ND2;
   const long lim = (long)m.ncell * ND2;
   for (int t = threadIdx.x; t < CPW * ND2; t += 64) {
     const long gi = base + t;
@@ -412,7 +412,7 @@ __global__ __launch_bounds__(64) void k_cellface(MeshDev m, Ph ph, FieldPtrs<Ph:
   constexpr int ND2 = Cfg<K>::ND2, CPW = Cfg<K>::CPW;
   __shared__ CFSmem<K, Ph, AMR> sh;
   cellface_columns<K, Ph, AMR>(m, ph, in, sh);
-  const long base = (long)blockIdx.x * CPW * ND2;
+  const long base = (long)cell_block() * CPW * ND2;
   const long lim = (long)m.ncell * ND2;
   OutRows<K> R = out_rows<K>(threadIdx.x);
   for (int t = threadIdx.x; t < CPW * ND2; t += 64) {

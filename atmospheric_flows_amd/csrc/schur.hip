@@ -49,7 +49,7 @@ __global__ __launch_bounds__(64) void k_b_ainv(MeshDev m, PhysB ph, const double
   } sh;
   __shared__ double s_t[2][CPW * CS];
   FieldPtrs<1> in{{p}};
-  const int cell0 = blockIdx.x * CPW;
+  const int cell0 = cell_block() * CPW;
   const int nc = min(CPW, m.ncell - cell0);
   // the inverse blocks are loaded while B p is formed
   SymBlockStage<K, 64, CPW> blk;
