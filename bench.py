@@ -43,8 +43,10 @@ def run_gpu(args):
     h = af.witch_of_agnesi(nx)
     res = {}
     states = {}
-    forms = [("mixed", dict(precision=1)), ("fp64", dict(precision=0))]
-    if not args.no_tight:
+    forms = [("mixed", dict(precision=1))]
+    if not args.mixed_only:
+        forms.append(("fp64", dict(precision=0)))
+    if not args.no_tight and not args.mixed_only:
         # the FP64 reference form solved to convergence: the yardstick both forms are measured against
         forms.append(("tight", dict(precision=0, eps=1e-13, a_solve_tol=1e-14, schur_inner_tol=1e-14)))
     for name, prm in forms:
@@ -65,7 +67,7 @@ def run_gpu(args):
         states[name] = ctx.download_state()
         res[name] = dict(seconds=el, warmup_seconds=tw, stats=st)
         print(f"[bench] {name}: {args.steps} steps in {el:.3f} s  stats={st}", file=sys.stderr, flush=True)
-        if name == "mixed":
+        if name == "mixed" and not args.mixed_only:
             kern = {}
             for kid, kname in KERNELS:
                 ms, b = ctx.time_kernel(kid, nrep=50)
@@ -76,7 +78,8 @@ def run_gpu(args):
     def rel_errs(a, b):
         return {nm: float(np.linalg.norm(x - y) / np.linalg.norm(y)) for nm, x, y in zip(("rho", "u", "p"), a, b)}
 
-    res["rel_err_mixed_vs_fp64"] = rel_errs(states["mixed"], states["fp64"])
+    if "fp64" in states:
+        res["rel_err_mixed_vs_fp64"] = rel_errs(states["mixed"], states["fp64"])
     if "tight" in states:
         res["rel_err_vs_converged_fp64"] = {"mixed": rel_errs(states["mixed"], states["tight"]),
                                             "fp64": rel_errs(states["fp64"], states["tight"])}
@@ -311,6 +314,8 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-tight", action="store_true")
     ap.add_argument("--no-amr", action="store_true")
+    ap.add_argument("--mixed-only", action="store_true",
+                    help="profiling runs: the mixed form's time loop only (no FP64 form, no kernel table, no roofline)")
     ap.add_argument("--amr-steps", type=int, default=50)
     ap.add_argument("--serve", action="store_true", help="also run the Poisson-arrival serving stream")
     ap.add_argument("--serve-rate", type=float, default=2.0)
@@ -330,6 +335,8 @@ def main():
         kernels_only(args)
         return
     traffic = None
+    if args.mixed_only:
+        args.no_traffic = args.no_ensemble = args.no_amr = args.no_cpu = True
     if not args.no_traffic:
         # counter passes run in child processes before this process touches the GPU
         nx, nz = (args.nx or WORKLOAD["nx"]), (args.nz or WORKLOAD["nz"])
@@ -338,11 +345,13 @@ def main():
     r = run_gpu(args)
     dofs = r["dofs"]
     mixed = dofs * args.steps / r["mixed"]["seconds"]
-    fp64 = dofs * args.steps / r["fp64"]["seconds"]
-    dom = r["kernels"][DOMINANT[1]]
-    roof = dict(bound="hbm", achieved=dom["gbs"], peak=HBM_PEAK_GBS, unit="GB/s", frac=dom["gbs"] / HBM_PEAK_GBS,
-                traffic=traffic["bytes"] if traffic else None, kernel=DOMINANT[1], bytes_per_launch=dom["bytes"],
-                ms_per_launch=dom["ms"], traffic_detail=traffic)
+    fp64 = dofs * args.steps / r["fp64"]["seconds"] if "fp64" in r else None
+    roof = None
+    if "kernels" in r:
+        dom = r["kernels"][DOMINANT[1]]
+        roof = dict(bound="hbm", achieved=dom["gbs"], peak=HBM_PEAK_GBS, unit="GB/s", frac=dom["gbs"] / HBM_PEAK_GBS,
+                    traffic=traffic["bytes"] if traffic else None, kernel=DOMINANT[1], bytes_per_launch=dom["bytes"],
+                    ms_per_launch=dom["ms"], traffic_detail=traffic)
     ensemble = None if args.no_ensemble else run_ensemble(args)
     amr = None if args.no_amr else run_amr(args)
     serve = {"mixed": run_serve(args, 1), "fp64": run_serve(args, 0)} if args.serve else None
@@ -355,12 +364,12 @@ def main():
         "config": {"workload": WORKLOAD["workload"] if not args.nx else f"hill_{r['nx']}x{r['nz']}_Q3_dt1",
                    "nx": r["nx"], "nz": r["nz"], "degree": 3, "dt": 1.0, "dofs": dofs, "gmres_rel_tol": 1e-10,
                    "parallelism": "single_gpu"},
-        "fp64_value": fp64, "speedup_mixed_vs_fp64": mixed / fp64,
-        "rel_err_mixed_vs_fp64": r["rel_err_mixed_vs_fp64"],
+        "fp64_value": fp64, "speedup_mixed_vs_fp64": mixed / fp64 if fp64 else None,
+        "rel_err_mixed_vs_fp64": r.get("rel_err_mixed_vs_fp64"),
         "rel_err_vs_converged_fp64": r.get("rel_err_vs_converged_fp64"),
-        "solver_stats": {"mixed": r["mixed"]["stats"], "fp64": r["fp64"]["stats"]},
+        "solver_stats": {f: r[f]["stats"] for f in ("mixed", "fp64") if f in r},
         "converged_fp64_seconds": r["tight"]["seconds"] if "tight" in r else None,
-        "kernels": r["kernels"], "roofline": roof, "cpu_baseline": cpu, "ensemble": ensemble,
+        "kernels": r.get("kernels"), "roofline": roof, "cpu_baseline": cpu, "ensemble": ensemble,
         "serve": serve, "amr": amr,
     }
     print(json.dumps(out))
