@@ -168,6 +168,7 @@ class EngineK : public Engine {
     gm_p.alloc(Ns);
     gm_coef.alloc(MAX_GMRES + 2);
     gm_rec.alloc((size_t)MAX_GMRES * GM_REC);
+    gm_part.alloc((size_t)RED_MAX_SLOTS * cell_blocks<K>(fine().ncell));
     gm_sig.alloc(MAX_GMRES + 2);
     for (auto* b : {&rho_old, &rho2, &rho3, &rho_curr, &p_old, &p2, &p3, &p_fixed, &p_fixed_old, &rhs_p}) b->alloc(Ns);
     for (auto* b : {&u_old, &u2, &u3, &u_curr, &u_fixed, &tmp1, &rhs_u}) b->alloc(2 * Ns);
@@ -457,6 +458,7 @@ class EngineK : public Engine {
   DBuf<double> eig_hist;   // CG coefficients of the eigenvalue estimate: alpha_it, beta_it
   hipEvent_t gm_ev[MAX_GMRES] = {};
   DBuf<double> gm_rec, gm_sig;
+  DBuf<double> gm_part;  // per-block partial dot products of the fused Arnoldi step, [slot][block]
   int hyper = 1, nsst = 1;
   DBuf<double> rho_fix, pres_fix;
   DBuf<double> cg_g, cg_d, cg_h, s_t1, s_t2, s_t3, in_g, in_d, in_h, gm_V, gm_p, gm_coef;
@@ -851,6 +853,12 @@ class EngineK : public Engine {
   // one pass. Mixed form: B and the cellwise-direct A^{-1} in one pass (A is
   // block diagonal; its FP64 cell inverses are rebuilt with every density).
   void schur_vmult(const double* src, double* dst, const double* scale = nullptr) {
+    schur_velocity(src);
+    launch_d_minus_c<K>(fine().m, oc_, bc_coeff(), pres_fix.p, s_t2.p, src, scale, dst, fine().Ns, s_);
+  }
+
+  // s_t2 = A^{-1} B src, the velocity part of the Schur product
+  void schur_velocity(const double* src) {
     ++st_.schur_matvecs;
     const long Ns = fine().Ns;
     if (P.precision == 1) {
@@ -859,7 +867,6 @@ class EngineK : public Engine {
       launch_B<K>(fine().m, oc_, bc_coeff(), src, s_t1.p, Ns, s_);
       inner_solve(s_t2.p, s_t1.p);
     }
-    launch_d_minus_c<K>(fine().m, oc_, bc_coeff(), pres_fix.p, s_t2.p, src, scale, dst, Ns, s_);
   }
 
   // The A solve the reference runs to the near-machine-precision tolerance
@@ -900,8 +907,12 @@ class EngineK : public Engine {
     double* V = gm_V.p;
     double* W = V + (long)(j + 1) * ld;
     double* rec = gm_rec.p + (long)j * GM_REC;
-    schur_vmult(V + (long)j * ld, W, PD);
-    gm_mdot(Ns, W, V, ld, j + 2, rec, rb, s_);  // W.V_i (i <= j) and W.W
+    // W = P S V_j with the dot products W.V_i (i <= j) and W.W of the same pass
+    schur_velocity(V + (long)j * ld);
+    const int G = cell_blocks<K>(fine().ncell);
+    launch_d_minus_c_dots<K>(fine().m, oc_, bc_coeff(), pres_fix.p, s_t2.p, V + (long)j * ld, PD, W, Ns, V, ld, j + 1,
+                             gm_part.p, G, s_);
+    gm_sum_slots(gm_part.p, G, G, j + 2, rec, s_);
     gm_maxpy_norm(Ns, W, V, ld, j + 1, rec, gm_sig.p, rec + GM_REC - 1, gm_sig.p + j + 1, rb, s_);
     ATMO_HIP_CHECK(hipMemcpyAsync(hgm + (long)j * GM_REC, rec, sizeof(double) * GM_REC, hipMemcpyDeviceToHost, s_));
     ATMO_HIP_CHECK(hipEventRecord(gm_ev[j], s_));
@@ -1137,6 +1148,7 @@ size_t engine_device_bytes(int k, const MeshDesc& mesh, const EngineParams& p) {
   if (p.precision == 1) b += (size_t)mesh.nx * mesh.nz * nd2 * (nd2 + 1) / 2 * D;  // packed FP64 cell-block inverses of A
   b += (size_t)(p.gmres_restart + 2) * (Ns + 32) * D + Ns * D + (MAX_GMRES + 2) * D;  // basis: leading dim N_s + 32
   b += (size_t)MAX_GMRES * GM_REC * D + (MAX_GMRES + 2) * D;  // GMRES records and basis scales
+  b += (size_t)RED_MAX_SLOTS * ((mesh.nx * mesh.nz + 64 / (2 * k + 1) - 1) / (64 / (2 * k + 1))) * D;  // Arnoldi partials
   b += 10 * Ns * D + 7 * 2 * Ns * D;      // IMEX state and stage fields
   return b;
 }

@@ -103,6 +103,8 @@ void gm_mdot(long n, const double* w, const double* V, long ldv, int nv, double*
 // result, *inv_norm = 1/|w| (0 for w = 0) (device scalars)
 void gm_maxpy_norm(long n, double* w, const double* V, long ldv, int nv, const double* dots, const double* sig,
                    double* norm, double* inv_norm, RedBuf& rb, hipStream_t s);
+// out[s] = sum_{b < G} part[s*pstride + b] for s < nslot (fixed order)
+void gm_sum_slots(const double* part, long pstride, int G, int nslot, double* out, hipStream_t s);
 // *norm = |v|, *inv_norm = 1/|v| (0 for v = 0) (device scalars)
 void gm_norm(long n, const double* v, double* norm, double* inv_norm, RedBuf& rb, hipStream_t s);
 // x += sum_i y[i] V_i

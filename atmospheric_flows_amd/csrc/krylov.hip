@@ -741,6 +741,33 @@ void gm_maxpy_norm(long n, double* w, const double* V, long ldv, int nv, const d
   fin(rb.gen, G, 1, FIN_GM_NORM, norm, nullptr, s, inv_norm);
 }
 
+// out[s] = sum over b < G of part[s*pstride + b] for s < nslot: one block
+// of 1024 threads per slot, fixed order (each thread's strided terms, then
+// the block tree), every load of a round issued before the first add.
+__global__ __launch_bounds__(1024) void k_sum_slots(const double* __restrict__ part, long pstride, int G,
+                                                    double* __restrict__ out) {
+  constexpr int U = 16;
+  const double* p = part + (long)blockIdx.x * pstride;
+  double acc[1] = {0.0};
+  for (int b0 = threadIdx.x; b0 < G; b0 += 1024 * U) {
+    double t[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int b = b0 + 1024 * u;
+      t[u] = b < G ? p[b] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) acc[0] += t[u];
+  }
+  block_reduce<1, false>(acc);
+  if (threadIdx.x == 0) out[blockIdx.x] = acc[0];
+}
+
+void gm_sum_slots(const double* part, long pstride, int G, int nslot, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_sum_slots, dim3(nslot), dim3(1024), 0, s, part, pstride, G, out);
+  ATMO_HIP_CHECK(hipGetLastError());
+}
+
 void gm_norm(long n, const double* v, double* norm, double* inv_norm, RedBuf& rb, hipStream_t s) {
   const int G = red_blocks(n);
   hipLaunchKernelGGL(k_dot_part, dim3(G), dim3(RED_THREADS), 0, s, n, v, v, rb.gen, (const CGState*)nullptr);

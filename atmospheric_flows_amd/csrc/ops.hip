@@ -468,6 +468,107 @@ void launch_d_minus_c(const MeshDev& m, const OpConsts& oc, double coeff, const 
   launch_cf<K>(m, ph, in, dst, Ns, s, scale);
 }
 
+// Eight per-lane values summed over the wave by a transposing butterfly
+// (10 shuffles for the eight sums, fixed order): every lane returns the sum
+// of slot (lane >> 3) & 7.
+__device__ inline double wave_sum8(const double (&v)[8]) {
+  const int l = threadIdx.x & 63;
+  const bool b5 = l & 32, b4 = l & 16, b3 = l & 8;
+  double u[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const double keep = b5 ? v[4 + i] : v[i], give = b5 ? v[i] : v[4 + i];
+    u[i] = keep + __shfl_xor(give, 32, 64);
+  }
+  double w[2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const double keep = b4 ? u[2 + i] : u[i], give = b4 ? u[i] : u[2 + i];
+    w[i] = keep + __shfl_xor(give, 16, 64);
+  }
+  double x = (b3 ? w[1] : w[0]) + __shfl_xor(b3 ? w[0] : w[1], 8, 64);
+  x += __shfl_xor(x, 4, 64);
+  x += __shfl_xor(x, 2, 64);
+  x += __shfl_xor(x, 1, 64);
+  return x;
+}
+
+// w = scale * (D p - C u) (as k_cellface<PhysDminusC>) and, in the same pass,
+// the partial dot products of w with V_0 .. V_{nv-1} (column k at V + k*ldv)
+// and w.w: slot k of block b at part[k*pstride + b], slot nv is w.w. The
+// Arnoldi step of GMRES: its Schur product and the classical Gram-Schmidt dot
+// products without a second pass over w. Chunks of 8 slots keep every load
+// of a chunk in flight at once (addresses past nv are clamped to column 0 and
+// their values dropped).
+template <int K, bool AMR>
+__global__ __launch_bounds__(64) void k_dmc_dots(MeshDev m, PhysDminusC ph, FieldPtrs<4> in, double* __restrict__ dst,
+                                                 const double* __restrict__ scale, long Ns,
+                                                 const double* __restrict__ V, long ldv, int nv,
+                                                 double* __restrict__ part, long pstride) {
+  constexpr int ND2 = Cfg<K>::ND2, CPW = Cfg<K>::CPW, NPASS = (CPW * ND2 + 63) / 64;
+  __shared__ CFSmem<K, PhysDminusC, AMR> sh;
+  cellface_columns<K, PhysDminusC, AMR>(m, ph, in, sh);
+  const int blk = cell_block();
+  const long base = (long)blk * CPW * ND2;
+  const long lim = (long)m.ncell * ND2;
+  OutRows<K> R = out_rows<K>(threadIdx.x);
+  double wv[NPASS];
+  long gi[NPASS];
+#pragma unroll
+  for (int ps = 0; ps < NPASS; ++ps) {
+    const int t = threadIdx.x + 64 * ps;
+    gi[ps] = base + t;
+    wv[ps] = 0.0;
+    if (t < CPW * ND2 && gi[ps] < lim) {
+      if (!rows_invariant<K>()) R = out_rows<K>(t);
+      const double acc = cellface_out<K, PhysDminusC, AMR>(sh, 0, t, R) * scale[gi[ps]];
+      dst[gi[ps]] = acc;
+      wv[ps] = acc;
+    } else {
+      gi[ps] = base;  // in bounds (the block's first dof); its product is zero
+    }
+  }
+  const int l = threadIdx.x & 63;
+  for (int k0 = 0; k0 <= nv; k0 += 8) {
+    double v[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      const int k = k0 + s;
+      const long col = (long)(k < nv ? k : 0) * ldv;
+      double acc = 0.0;
+#pragma unroll
+      for (int ps = 0; ps < NPASS; ++ps) {
+        const double y = V[col + gi[ps]];
+        acc += wv[ps] * (k < nv ? y : (k == nv ? wv[ps] : 0.0));
+      }
+      v[s] = acc;
+    }
+    const double sum = wave_sum8(v);
+    const int slot = k0 + (l >> 3);
+    if ((l & 7) == 0 && slot <= nv) part[(long)slot * pstride + blk] = sum;
+  }
+}
+
+template <int K>
+void launch_d_minus_c_dots(const MeshDev& m, const OpConsts& oc, double coeff, const double* pfix, const double* u,
+                           const double* p, const double* scale, double* dst, long Ns, const double* V, long ldv,
+                           int nv, double* part, long pstride, hipStream_t s) {
+  const int grid = ceil_div(m.ncell, Cfg<K>::CPW);
+  if (grid == 0) return;
+  if (grid > pstride) throw std::invalid_argument("d_minus_c_dots: partial buffer too small");
+  PhysDminusC ph;
+  ph.c.set(coeff * oc.dt, oc.gamma_gas);
+  ph.cD = 1.0 / (oc.gamma_gas - 1.0);
+  FieldPtrs<4> in{{pfix, u, u + Ns, p}};
+  if (m.cface)
+    hipLaunchKernelGGL((k_dmc_dots<K, true>), dim3(grid), dim3(64), 0, s, m, ph, in, dst, scale, Ns, V, ldv, nv, part,
+                       pstride);
+  else
+    hipLaunchKernelGGL((k_dmc_dots<K, false>), dim3(grid), dim3(64), 0, s, m, ph, in, dst, scale, Ns, V, ldv, nv, part,
+                       pstride);
+  ATMO_HIP_CHECK(hipGetLastError());
+}
+
 template <int K>
 void launch_rhs_rho(const MeshDev& m, const OpConsts& oc, const Tableau& tb, int stage, const double* const* src,
                     double* dst, long Ns, hipStream_t s) {
@@ -592,6 +693,9 @@ void launch_rhs_velocity(const MeshDev& m, const OpConsts& oc, const Tableau& tb
                             hipStream_t);                                                                       \
   template void launch_d_minus_c<K>(const MeshDev&, const OpConsts&, double, const double*, const double*,         \
                                     const double*, const double*, double*, long, hipStream_t);                   \
+  template void launch_d_minus_c_dots<K>(const MeshDev&, const OpConsts&, double, const double*, const double*,    \
+                                         const double*, const double*, double*, long, const double*, long, int,  \
+                                         double*, long, hipStream_t);                                            \
   template void launch_rhs_rho<K>(const MeshDev&, const OpConsts&, const Tableau&, int, const double* const*,     \
                                   double*, long, hipStream_t);                                                  \
   template void launch_rhs_pressure<K>(const MeshDev&, const OpConsts&, const Tableau&, int,                     \
