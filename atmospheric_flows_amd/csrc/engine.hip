@@ -374,6 +374,17 @@ class EngineK : public Engine {
             fields = 15;
           }
           break;
+        case 12:  // the Arnoldi step's scaled D - C with the dot products against 14 basis vectors
+        case 13:  // the same kernel with no basis vector (D - C and w.w only)
+        {
+          if (P.gmres_restart < 13) throw std::invalid_argument("time_kernel 12/13 need gmres_restart >= 13");
+          const int nv = which == 12 ? 14 : 0;
+          const int G = cell_blocks<K>(fine().ncell);
+          launch_d_minus_c_dots<K>(m, oc_, bc_coeff(), pres_fix.p, u_old.p, p_old.p, fine().inv64[BLK_D].p,
+                                   gm_V.p + 14 * gm_ld(), Ns, gm_V.p, gm_ld(), nv, gm_part.p, G, s_);
+          fields = 6 + nv;
+          break;
+        }
         default:
           throw std::invalid_argument("time_kernel: unknown kernel id");
       }
@@ -707,28 +718,25 @@ class EngineK : public Engine {
     if constexpr (std::is_same<T, double>::value) return L.inv64[blk].p; else return L.inv32[blk].p;
   }
 
-  template <typename T>
-  void cheb(int l, int blk, T* x, const T* b, bool zero_guess) {
-    Level& L = *lv[l];
-    const long n = (long)ncomp(blk) * L.Ns;
+  // Coefficients of one PreconditionChebyshev sweep of degree cheb_degree on
+  // [lmin, lmax] of level l: step 0 is d = P (b - A x) / theta, step k the
+  // three-term recurrence.
+  ChebSteps cheb_steps(int l, int blk) const {
+    const Level& L = *lv[l];
     const double theta = 0.5 * (L.lmax[blk] + L.lmin[blk]), delta = 0.5 * (L.lmax[blk] - L.lmin[blk]);
     const double sigma = theta / delta;
     double rho = 1.0 / sigma;
-    T* d = LD<T>(L);
-    const T* Pd = LP<T>(L, blk);
-    const T* rw = nullptr;
-    if (blk == BLK_A) rw = std::is_same<T, double>::value ? (const T*)L.rho64.p : (const T*)L.rho32.p;
-    // one smoothing step: x += d with d = f1 d + f2 P (b - A x), operator and update in one kernel
-    auto step = [&](double f1, double f2) {
-      launch_mass_cheb<K, T>(L.m, ncomp(blk), blk == BLK_A, x, rw, (T)coeff(blk), b, Pd, d, (T)f1, (T)f2, L.Ns, s_);
-    };
-    if (zero_guess) cheb_first<T>(n, b, Pd, x, d, (T)(1.0 / theta), s_);
-    else step(0.0, 1.0 / theta);
-    for (int k = 0; k < P.cheb_degree - 1; ++k) {
+    ChebSteps cs;
+    cs.n = P.cheb_degree;
+    cs.f1[0] = 0.0;
+    cs.f2[0] = 1.0 / theta;
+    for (int k = 1; k < cs.n; ++k) {
       const double rn = 1.0 / (2.0 * sigma - rho);
-      step(rn * rho, 2.0 * rn / delta);
+      cs.f1[k] = rn * rho;
+      cs.f2[k] = 2.0 * rn / delta;
       rho = rn;
     }
+    return cs;
   }
 
   // V-cycle on level l: x = MG(b); x, b are level buffers of type T
@@ -752,14 +760,19 @@ class EngineK : public Engine {
       if constexpr (!std::is_same<T, double>::value) vec_cast<T, double>((long)nc * L.Ns, x64, x, s_);
       return;
     }
-    cheb<T>(l, blk, x, b, true);
+    // pre-smoothing from a zero guess and the residual t = b - A x: one kernel
     T* t = LT<T>(L);
-    level_apply<T>(L, blk, x, t, (T)-1, b, (T)1);  // t = b - A x
+    const T* Pd = LP<T>(L, blk);
+    const T* rw = nullptr;
+    if (blk == BLK_A) rw = std::is_same<T, double>::value ? (const T*)L.rho64.p : (const T*)L.rho32.p;
+    const ChebSteps cs = cheb_steps(l, blk);
+    launch_presmooth_residual<K, T>(L.m, nc, blk == BLK_A, x, rw, (T)coeff(blk), b, Pd, cs, t, L.Ns, s_);
     Level& C = *lv[l - 1];
     launch_restrict<K, T>(L.m, C.m, t, LB<T>(C), nc, L.Ns, C.Ns, s_);
     vcycle<T>(l - 1, blk, LX<T>(C), LB<T>(C), tol, max_its);
-    launch_prolongate_add<K, T>(L.m, C.m, LX<T>(C), x, nc, L.Ns, C.Ns, s_);
-    cheb<T>(l, blk, x, b, false);
+    // coarse-grid correction and post-smoothing: one kernel
+    launch_prolongate_postsmooth<K, T>(L.m, C.m, nc, blk == BLK_A, x, LX<T>(C), rw, (T)coeff(blk), b, Pd, cs, L.Ns,
+                                       C.Ns, s_);
   }
 
   // PreconditionMG::vmult: dst = V-cycle(src) on the fine level

@@ -14,19 +14,29 @@ namespace atmo {
 template <int K, typename T>
 void launch_restrict(const MeshDev& mf, const MeshDev& mc, const T* fine, T* coarse, int ncomp, long Nf, long Nc,
                      hipStream_t s);
-// fine += P coarse
-template <int K, typename T>
-void launch_prolongate_add(const MeshDev& mf, const MeshDev& mc, const T* coarse, T* fine, int ncomp, long Nf,
-                           long Nc, hipStream_t s);
 // coarse = L2 projection of fine (density)
 template <int K>
 void launch_project_down(const MeshDev& mf, const MeshDev& mc, const double* fine, double* coarse, hipStream_t s);
 
-// Chebyshev updates (P = inverse diagonal): the first step from a zero
-// guess, d = P b / theta ; x = d (the later steps need A x and are fused
-// with it: launch_mass_cheb)
-template <typename T>
-void cheb_first(long n, const T* b, const T* P, T* x, T* d, T inv_theta, hipStream_t s);
+// Coefficients of one Chebyshev sweep: step s is d = f1[s] d + f2[s] P (b - A x), x += d.
+struct ChebSteps {
+  static constexpr int MAX = 16;
+  int n = 0;
+  double f1[MAX] = {}, f2[MAX] = {};
+};
+
+// One Chebyshev sweep of a level from a zero guess (the V-cycle's
+// pre-smoothing) fused with the residual it leaves: x = S(b), res = b - A x.
+// The mass-type operators are cell-local, so every step runs in one kernel.
+template <int K, typename T>
+void launch_presmooth_residual(const MeshDev& m, int ncomp, bool weighted, T* x, const T* rho, T coeff, const T* b,
+                               const T* P, const ChebSteps& cs, T* res, long Ns, hipStream_t s);
+// x += prolongation of the coarse level's xc, then one Chebyshev sweep from
+// that guess (the V-cycle's coarse-grid correction and post-smoothing).
+template <int K, typename T>
+void launch_prolongate_postsmooth(const MeshDev& m, const MeshDev& mc, int ncomp, bool weighted, T* x, const T* xc,
+                                  const T* rho, T coeff, const T* b, const T* P, const ChebSteps& cs, long Ns,
+                                  long Nc, hipStream_t s);
 
 // Single-workgroup CG on the coarsest level (FP64, identity preconditioner,
 // absolute tolerance read from *tol, at most max_its iterations). Work

@@ -68,44 +68,6 @@ __global__ void k_restrict(MeshDev mf, MeshDev mc, const T* __restrict__ fine, T
   }
 }
 
-// one thread per fine dof
-template <int K, typename T>
-__global__ void k_prolongate_add(MeshDev mf, MeshDev mc, const T* __restrict__ coarse, T* __restrict__ fine,
-                                 int ncomp, long Nf, long Nc) {
-  constexpr int ND = K + 1, ND2 = ND * ND;
-  const long tot = (long)ncomp * Nf;
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < tot; t += (long)gridDim.x * blockDim.x) {
-    const int comp = (int)(t / Nf);
-    const long r = t - comp * Nf;
-    const int cf = (int)(r / ND2), o = (int)(r - (long)cf * ND2), jj = o / ND, ii = o - jj * ND;
-    int cc, a, b;
-    if (mf.parent) {
-      cc = mf.parent[2 * cf];
-      const int code = mf.parent[2 * cf + 1];
-      if (code < 0) {
-        fine[t] += coarse[comp * Nc + (long)cc * ND2 + o];
-        continue;
-      }
-      a = code & 1;
-      b = code >> 1;
-    } else {
-      const int i = cf % mf.nx, j = cf / mf.nx;
-      cc = (j >> 1) * mc.nx + (i >> 1);
-      a = i & 1;
-      b = j & 1;
-    }
-    const T* u = coarse + comp * Nc + (long)cc * ND2;
-    double acc = 0;
-#pragma unroll
-    for (int m = 0; m < ND; ++m) {
-      const double pb = trP<K>(b, jj, m);
-#pragma unroll
-      for (int n = 0; n < ND; ++n) acc += pb * trP<K>(a, ii, n) * (double)u[m * ND + n];
-    }
-    fine[t] += (T)acc;
-  }
-}
-
 template <int K>
 __global__ void k_project_down(MeshDev mf, MeshDev mc, const double* __restrict__ fine, double* __restrict__ coarse) {
   constexpr int ND = K + 1, ND2 = ND * ND;
@@ -150,14 +112,6 @@ void launch_restrict(const MeshDev& mf, const MeshDev& mc, const T* fine, T* coa
   ATMO_HIP_CHECK(hipGetLastError());
 }
 
-template <int K, typename T>
-void launch_prolongate_add(const MeshDev& mf, const MeshDev& mc, const T* coarse, T* fine, int ncomp, long Nf,
-                           long Nc, hipStream_t s) {
-  hipLaunchKernelGGL((k_prolongate_add<K, T>), dim3(grid_for(ncomp * Nf)), dim3(256), 0, s, mf, mc, coarse, fine,
-                     ncomp, Nf, Nc);
-  ATMO_HIP_CHECK(hipGetLastError());
-}
-
 template <int K>
 void launch_project_down(const MeshDev& mf, const MeshDev& mc, const double* fine, double* coarse, hipStream_t s) {
   hipLaunchKernelGGL((k_project_down<K>), dim3(grid_for((long)mc.ncell * (K + 1) * (K + 1))), dim3(256), 0, s, mf,
@@ -165,21 +119,238 @@ void launch_project_down(const MeshDev& mf, const MeshDev& mc, const double* fin
   ATMO_HIP_CHECK(hipGetLastError());
 }
 
-// ------------------------------------------------------------ Chebyshev
-template <typename T>
-__global__ void k_cheb_first(long n, const T* __restrict__ b, const T* __restrict__ P, T* __restrict__ x,
-                             T* __restrict__ d, T inv_theta) {
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    const T v = P[i] * b[i] * inv_theta;
-    d[i] = v;
-    x[i] = v;
+// ------------------------------------------------------ fused smoothing
+// One Chebyshev sweep of a level (all its steps) for the CPW cells of a
+// one-wave block; the mass-type operators are cell-local, so x stays in LDS
+// and d, b, P in registers between the steps. Step s: d = f1[s] d +
+// f2[s] P (b - A x), x += d; with ZG (zero initial guess) step 0 is
+// d = x = P b f2[0] without an operator application. PRO: the sweep starts
+// from x + prolongation of the coarse level's xc (the V-cycle's coarse-grid
+// correction). RES: the sweep ends with res = b - A x (the V-cycle's
+// residual for the restriction). Same expressions, in the same order, as
+// the first-step update, the fused operator-and-update step, the DG
+// embedding prolongation and the residual of the two-kernel forms, so the
+// values are those of those kernels.
+template <int K, typename T, int NC, bool W, bool ZG, bool PRO, bool RES>
+__global__ __launch_bounds__(64) void k_cheb_sweep(MeshDev m, MeshDev mc, T* __restrict__ x, const T* __restrict__ xc,
+                                                   const T* __restrict__ rho, T coeff, const T* __restrict__ b,
+                                                   const T* __restrict__ P, ChebSteps cs, T* __restrict__ res,
+                                                   long Ns, long Nc) {
+  using C = Cfg<K>;
+  constexpr int ND = C::ND, NQ = C::NQ, ND2 = C::ND2, CPW = C::CPW, NPASS = (CPW * ND2 + 63) / 64;
+  constexpr auto& tb = basis::kT<K>;
+  constexpr int NS = NC + (W ? 1 : 0);
+  __shared__ T s_u[NS][CPW * ND2];  // x (components), then rho when W
+  __shared__ T s_red[NC][CPW][ND][NQ];
+  const int blk = cell_block();
+  const long base = (long)blk * CPW * ND2;
+  const long lim = (long)m.ncell * ND2;
+  // stage x (unless the guess is zero) and rho
+  {
+    const T* srcs[NS];
+    int nsrc = 0;
+#pragma unroll
+    for (int comp = 0; comp < NC; ++comp) srcs[comp] = x + comp * Ns;
+    if constexpr (W) srcs[NS - 1] = rho;
+    if constexpr (!ZG) {
+      stage_fields<K, NS, T, T>(&s_u[0][0], srcs, m.ncell);
+    } else if constexpr (W) {
+      const T* s1[1] = {rho};
+      stage_fields<K, 1, T, T>(&s_u[NS - 1][0], s1, m.ncell);
+    }
+    (void)nsrc;
+  }
+  // per-dof b and P in registers
+  T bv[NPASS][NC], pv[NPASS][NC], dv[NPASS][NC];
+  bool ok[NPASS];
+#pragma unroll
+  for (int ps = 0; ps < NPASS; ++ps) {
+    const int t = threadIdx.x + 64 * ps;
+    ok[ps] = t < CPW * ND2 && base + t < lim;
+#pragma unroll
+    for (int comp = 0; comp < NC; ++comp) {
+      const long k = comp * Ns + (ok[ps] ? base + t : 0);
+      bv[ps][comp] = ok[ps] ? b[k] : (T)0;
+      pv[ps][comp] = ok[ps] ? P[k] : (T)0;
+      dv[ps][comp] = (T)0;
+    }
+  }
+  __syncthreads();
+  if constexpr (PRO) {
+    // x += prolongation of the parent cell's coarse values (DG embedding), or the identity
+#pragma unroll
+    for (int ps = 0; ps < NPASS; ++ps) {
+      const int t = threadIdx.x + 64 * ps;
+      if (!ok[ps]) continue;
+      const long r = base + t;
+      const int cf = (int)(r / ND2), o = (int)(r - (long)cf * ND2), jj = o / ND, ii = o - jj * ND;
+      int cc, a = 0, bb = 0;
+      bool ident = false;
+      if (m.parent) {
+        cc = m.parent[2 * cf];
+        const int code = m.parent[2 * cf + 1];
+        if (code < 0) ident = true;
+        else {
+          a = code & 1;
+          bb = code >> 1;
+        }
+      } else {
+        const int i = cf % m.nx, j = cf / m.nx;
+        cc = (j >> 1) * mc.nx + (i >> 1);
+        a = i & 1;
+        bb = j & 1;
+      }
+#pragma unroll
+      for (int comp = 0; comp < NC; ++comp) {
+        const T* u = xc + comp * Nc + (long)cc * ND2;
+        if (ident) {
+          s_u[comp][t] += u[o];
+        } else {
+          double acc = 0;
+#pragma unroll
+          for (int mm = 0; mm < ND; ++mm) {
+            const double pb = trP<K>(bb, jj, mm);
+#pragma unroll
+            for (int n = 0; n < ND; ++n) acc += pb * trP<K>(a, ii, n) * (double)u[mm * ND + n];
+          }
+          s_u[comp][t] += (T)acc;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // column weights once per lane
+  const int lane = threadIdx.x & 63;
+  const int cw = lane / NQ, c = lane - cw * NQ;
+  const int cell = blk * CPW + cw;
+  const bool col_active = cw < CPW && cell < m.ncell;
+  T Sc[ND], Dc[ND], wq[NQ];
+  if (col_active) {
+    lane_rows<K, T>(c, Sc, Dc);
+    column_weights<K, T, W>(m, cell, c, cw, &s_u[NS - 1][0], coeff, wq, Sc);
+  }
+  T Si[NPASS][NQ];
+#pragma unroll
+  for (int ps = 0; ps < NPASS; ++ps) lane_cols_S<K, T>((threadIdx.x + 64 * ps) % ND, Si[ps]);
+  // A x of the staged x into s_red (the column phase of wave_mass_core with the cached weights)
+  auto apply = [&]() {
+    if (col_active) {
+#pragma unroll
+      for (int comp = 0; comp < NC; ++comp) {
+        const T* u = &s_u[comp][cw * ND2];
+        T tv[ND];
+#pragma unroll
+        for (int j = 0; j < ND; ++j) {
+          T s = 0;
+#pragma unroll
+          for (int i = 0; i < ND; ++i) s += Sc[i] * u[j * ND + i];
+          tv[j] = s;
+        }
+        T Tv[ND];
+#pragma unroll
+        for (int j = 0; j < ND; ++j) Tv[j] = 0;
+#pragma unroll
+        for (int r = 0; r < NQ; ++r) {
+          T val = 0;
+#pragma unroll
+          for (int j = 0; j < ND; ++j) val += (T)tb.S[r][j] * tv[j];
+          val *= wq[r];
+#pragma unroll
+          for (int j = 0; j < ND; ++j) Tv[j] += (T)tb.S[r][j] * val;
+        }
+#pragma unroll
+        for (int j = 0; j < ND; ++j) s_red[comp][cw][j][c] = Tv[j];
+      }
+    }
+    __syncthreads();
+  };
+  for (int s = 0; s < cs.n; ++s) {
+    const T f1 = (T)cs.f1[s], f2 = (T)cs.f2[s];
+    if (ZG && s == 0) {
+#pragma unroll
+      for (int ps = 0; ps < NPASS; ++ps) {
+        const int t = threadIdx.x + 64 * ps;
+        if (!ok[ps]) continue;
+#pragma unroll
+        for (int comp = 0; comp < NC; ++comp) {
+          const T v = pv[ps][comp] * bv[ps][comp] * f2;
+          dv[ps][comp] = v;
+          s_u[comp][t] = v;
+        }
+      }
+      __syncthreads();
+      continue;
+    }
+    apply();
+#pragma unroll
+    for (int ps = 0; ps < NPASS; ++ps) {
+      const int t = threadIdx.x + 64 * ps;
+      if (!ok[ps]) continue;
+#pragma unroll
+      for (int comp = 0; comp < NC; ++comp) {
+        const T ax = wave_mass_out<K, T>(&s_red[comp][0][0][0], t, Si[ps]);
+        const T v = f1 * dv[ps][comp] + f2 * pv[ps][comp] * (bv[ps][comp] - ax);
+        dv[ps][comp] = v;
+        s_u[comp][t] = s_u[comp][t] + v;
+      }
+    }
+    __syncthreads();
+  }
+  if constexpr (RES) {
+    apply();
+#pragma unroll
+    for (int ps = 0; ps < NPASS; ++ps) {
+      const int t = threadIdx.x + 64 * ps;
+      if (!ok[ps]) continue;
+#pragma unroll
+      for (int comp = 0; comp < NC; ++comp) {
+        const T ax = wave_mass_out<K, T>(&s_red[comp][0][0][0], t, Si[ps]);
+        res[comp * Ns + base + t] = -ax + bv[ps][comp];
+      }
+    }
+  }
+#pragma unroll
+  for (int ps = 0; ps < NPASS; ++ps) {
+    const int t = threadIdx.x + 64 * ps;
+    if (!ok[ps]) continue;
+#pragma unroll
+    for (int comp = 0; comp < NC; ++comp) x[comp * Ns + base + t] = s_u[comp][t];
   }
 }
 
-template <typename T>
-void cheb_first(long n, const T* b, const T* P, T* x, T* d, T inv_theta, hipStream_t s) {
-  hipLaunchKernelGGL(k_cheb_first<T>, dim3(grid_for(n)), dim3(256), 0, s, n, b, P, x, d, inv_theta);
+template <int K, typename T, bool ZG, bool PRO, bool RES>
+static void sweep_launch(const MeshDev& m, const MeshDev& mc, int ncomp, bool weighted, T* x, const T* xc,
+                         const T* rho, T coeff, const T* b, const T* P, const ChebSteps& cs, T* res, long Ns, long Nc,
+                         hipStream_t s) {
+  const int grid = (m.ncell + Cfg<K>::CPW - 1) / Cfg<K>::CPW;
+  if (grid == 0) return;
+  if (weighted) {
+    if (ncomp != 2) throw std::invalid_argument("weighted mass operator expects 2 components");
+    hipLaunchKernelGGL((k_cheb_sweep<K, T, 2, true, ZG, PRO, RES>), dim3(grid), dim3(64), 0, s, m, mc, x, xc, rho,
+                       coeff, b, P, cs, res, Ns, Nc);
+  } else if (ncomp == 1) {
+    hipLaunchKernelGGL((k_cheb_sweep<K, T, 1, false, ZG, PRO, RES>), dim3(grid), dim3(64), 0, s, m, mc, x, xc, rho,
+                       coeff, b, P, cs, res, Ns, Nc);
+  } else {
+    hipLaunchKernelGGL((k_cheb_sweep<K, T, 2, false, ZG, PRO, RES>), dim3(grid), dim3(64), 0, s, m, mc, x, xc, rho,
+                       coeff, b, P, cs, res, Ns, Nc);
+  }
   ATMO_HIP_CHECK(hipGetLastError());
+}
+
+template <int K, typename T>
+void launch_presmooth_residual(const MeshDev& m, int ncomp, bool weighted, T* x, const T* rho, T coeff, const T* b,
+                               const T* P, const ChebSteps& cs, T* res, long Ns, hipStream_t s) {
+  if (cs.n < 1 || cs.n > ChebSteps::MAX) throw std::invalid_argument("Chebyshev degree out of range");
+  sweep_launch<K, T, true, false, true>(m, m, ncomp, weighted, x, nullptr, rho, coeff, b, P, cs, res, Ns, 0, s);
+}
+
+template <int K, typename T>
+void launch_prolongate_postsmooth(const MeshDev& m, const MeshDev& mc, int ncomp, bool weighted, T* x, const T* xc,
+                                  const T* rho, T coeff, const T* b, const T* P, const ChebSteps& cs, long Ns,
+                                  long Nc, hipStream_t s) {
+  if (cs.n < 1 || cs.n > ChebSteps::MAX) throw std::invalid_argument("Chebyshev degree out of range");
+  sweep_launch<K, T, false, true, false>(m, mc, ncomp, weighted, x, xc, rho, coeff, b, P, cs, nullptr, Ns, Nc, s);
 }
 
 // ------------------------------------------------------------ coarse CG
@@ -296,11 +467,19 @@ void launch_coarse_cg(const MeshDev& m, int ncomp, bool weighted, const double* 
                                            hipStream_t);                                                        \
   template void launch_restrict<K, float>(const MeshDev&, const MeshDev&, const float*, float*, int, long, long,   \
                                           hipStream_t);                                                         \
-  template void launch_prolongate_add<K, double>(const MeshDev&, const MeshDev&, const double*, double*, int, long, \
-                                                 long, hipStream_t);                                            \
-  template void launch_prolongate_add<K, float>(const MeshDev&, const MeshDev&, const float*, float*, int, long,    \
-                                                long, hipStream_t);                                             \
   template void launch_project_down<K>(const MeshDev&, const MeshDev&, const double*, double*, hipStream_t);      \
+  template void launch_presmooth_residual<K, float>(const MeshDev&, int, bool, float*, const float*, float,          \
+                                                    const float*, const float*, const ChebSteps&, float*, long,      \
+                                                    hipStream_t);                                                    \
+  template void launch_presmooth_residual<K, double>(const MeshDev&, int, bool, double*, const double*, double,      \
+                                                     const double*, const double*, const ChebSteps&, double*, long,  \
+                                                     hipStream_t);                                                   \
+  template void launch_prolongate_postsmooth<K, float>(const MeshDev&, const MeshDev&, int, bool, float*,            \
+                                                       const float*, const float*, float, const float*, const float*, \
+                                                       const ChebSteps&, long, long, hipStream_t);                   \
+  template void launch_prolongate_postsmooth<K, double>(const MeshDev&, const MeshDev&, int, bool, double*,          \
+                                                        const double*, const double*, double, const double*,         \
+                                                        const double*, const ChebSteps&, long, long, hipStream_t);   \
   template void launch_coarse_cg<K>(const MeshDev&, int, bool, const double*, double, const double*, double*,     \
                                     double*, double*, double*, const double*, int, int*, long, hipStream_t);
 
@@ -308,7 +487,5 @@ ATMO_MG_INST(1)
 ATMO_MG_INST(2)
 ATMO_MG_INST(3)
 
-template void cheb_first<double>(long, const double*, const double*, double*, double*, double, hipStream_t);
-template void cheb_first<float>(long, const float*, const float*, float*, float*, float, hipStream_t);
 
 }  // namespace atmo

@@ -23,12 +23,6 @@ template <int K, typename T>
 void launch_mass(const MeshDev& m, int ncomp, bool weighted, const T* src, const T* rho, T* dst, T coeff,
                  const T* add, T add_scale, long Ns, hipStream_t s);
 
-// One Chebyshev smoothing step fused with its operator application:
-// t = coeff M x (A x when weighted); d = f1 d + f2 P (b - t); x += d.
-template <int K, typename T>
-void launch_mass_cheb(const MeshDev& m, int ncomp, bool weighted, T* x, const T* rho, T coeff, const T* b,
-                      const T* P, T* d, T f1, T f2, long Ns, hipStream_t s);
-
 // inverse diagonal of coeff*M (weighted: of A, written to both components)
 template <int K, typename T>
 void launch_inv_diag(const MeshDev& m, int ncomp, bool weighted, const T* rho, T* dst, T coeff, long Ns,

@@ -68,47 +68,6 @@ __global__ __launch_bounds__(64) void k_mass(MeshDev m, const T* __restrict__ sr
   }
 }
 
-// One Chebyshev smoothing step with the operator application it needs:
-// t = coeff M x (A x when W), d = f1 d + f2 P (b - t), x += d. The mass-type
-// operators are cell-local, so each block updates exactly the cells it
-// staged (the old x comes from LDS). Same values as launch_mass followed by
-// the separate update.
-template <int K, typename T, int NC, bool W>
-__global__ __launch_bounds__(64) void k_mass_cheb(MeshDev m, T* __restrict__ x, const T* __restrict__ rho, T coeff,
-                                                  const T* __restrict__ b, const T* __restrict__ P,
-                                                  T* __restrict__ d, T f1, T f2, long Ns) {
-  using C = Cfg<K>;
-  constexpr int ND2 = C::ND2, CPW = C::CPW;
-  constexpr int NS = NC + (W ? 1 : 0);
-  __shared__ T s_u[NS][CPW * ND2];  // the components of x, then rho when W
-  __shared__ T s_red[NC][CPW][C::ND][C::NQ];
-  const T* srcs[NS];
-#pragma unroll
-  for (int comp = 0; comp < NC; ++comp) srcs[comp] = x + comp * Ns;
-  if constexpr (W) srcs[NS - 1] = rho;
-  stage_fields<K, NS, T, T>(&s_u[0][0], srcs, m.ncell);
-  __syncthreads();
-  wave_mass_core<K, T, NC, W>(m, cell_block() * CPW, &s_u[0][0], &s_u[NS - 1][0], coeff, &s_red[0][0][0][0]);
-  __syncthreads();
-  const long base = (long)cell_block() * CPW * ND2;
-  const long lim = (long)m.ncell * ND2;
-  T Si[C::NQ];
-  lane_cols_S<K, T>(threadIdx.x % C::ND, Si);
-  for (int t = threadIdx.x; t < CPW * ND2; t += 64) {
-    const long gi = base + t;
-    if (gi >= lim) break;
-    if (64 % C::ND != 0) lane_cols_S<K, T>(t % C::ND, Si);
-#pragma unroll
-    for (int comp = 0; comp < NC; ++comp) {
-      const T ax = wave_mass_out<K, T>(&s_red[comp][0][0][0], t, Si);
-      const long k = comp * Ns + gi;
-      const T v = f1 * d[k] + f2 * P[k] * (b[k] - ax);
-      d[k] = v;
-      x[k] = s_u[comp][t] + v;
-    }
-  }
-}
-
 template <int K, typename T, bool W>
 __global__ __launch_bounds__(64) void k_inv_diag(MeshDev m, int ncomp, const T* __restrict__ rho, T* __restrict__ dst,
                                                  T coeff, long Ns) {
@@ -189,25 +148,6 @@ void launch_mass(const MeshDev& m, int ncomp, bool weighted, const T* src, const
   } else {
     hipLaunchKernelGGL((k_mass<K, T, 2, false>), dim3(grid), dim3(64), 0, s, m, src, rho, dst, coeff, add,
                        add_scale, Ns);
-  }
-  ATMO_HIP_CHECK(hipGetLastError());
-}
-
-template <int K, typename T>
-void launch_mass_cheb(const MeshDev& m, int ncomp, bool weighted, T* x, const T* rho, T coeff, const T* b,
-                      const T* P, T* d, T f1, T f2, long Ns, hipStream_t s) {
-  const int grid = ceil_div(m.ncell, Cfg<K>::CPW);
-  if (grid == 0) return;
-  if (weighted) {
-    if (ncomp != 2) throw std::invalid_argument("weighted mass operator expects 2 components");
-    hipLaunchKernelGGL((k_mass_cheb<K, T, 2, true>), dim3(grid), dim3(64), 0, s, m, x, rho, coeff, b, P, d, f1, f2,
-                       Ns);
-  } else if (ncomp == 1) {
-    hipLaunchKernelGGL((k_mass_cheb<K, T, 1, false>), dim3(grid), dim3(64), 0, s, m, x, rho, coeff, b, P, d, f1, f2,
-                       Ns);
-  } else {
-    hipLaunchKernelGGL((k_mass_cheb<K, T, 2, false>), dim3(grid), dim3(64), 0, s, m, x, rho, coeff, b, P, d, f1, f2,
-                       Ns);
   }
   ATMO_HIP_CHECK(hipGetLastError());
 }
@@ -679,11 +619,6 @@ void launch_rhs_velocity(const MeshDev& m, const OpConsts& oc, const Tableau& tb
                                        const double*, double, long, hipStream_t);                               \
   template void launch_mass<K, float>(const MeshDev&, int, bool, const float*, const float*, float*, float,      \
                                       const float*, float, long, hipStream_t);                                  \
-  template void launch_mass_cheb<K, double>(const MeshDev&, int, bool, double*, const double*, double,          \
-                                            const double*, const double*, double*, double, double, long,        \
-                                            hipStream_t);                                                       \
-  template void launch_mass_cheb<K, float>(const MeshDev&, int, bool, float*, const float*, float, const float*, \
-                                           const float*, float*, float, float, long, hipStream_t);              \
   template void launch_inv_diag<K, double>(const MeshDev&, int, bool, const double*, double*, double, long,      \
                                            hipStream_t);                                                        \
   template void launch_inv_diag<K, float>(const MeshDev&, int, bool, const float*, float*, float, long,          \

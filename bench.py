@@ -88,7 +88,8 @@ def run_gpu(args):
     return res
 
 
-KERNELS = ((8, "k_b_ainv"), (9, "k_d_minus_c"), (10, "gm_mdot_14"), (11, "gm_maxpy_norm_13"), (1, "k_jcg_apply"),
+KERNELS = ((8, "k_b_ainv"), (9, "k_d_minus_c"), (12, "k_dmc_dots_14"), (13, "k_dmc_dots_0"), (10, "gm_mdot_14"),
+           (11, "gm_maxpy_norm_13"), (1, "k_jcg_apply"),
            (6, "k_jcg_upd"), (0, "k_mass_A"), (2, "k_mass_M"), (3, "k_cellface_B"), (4, "k_cellface_C"),
            (5, "k_cellface_rhs_p3"), (7, "copy_2field"))
 # dominant kernel of the mixed form (the measured `value`): B with the cellwise A^{-1}

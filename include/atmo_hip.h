@@ -203,8 +203,10 @@ atmo_status atmo_init_hill_state(const atmo_mesh_desc* mesh, double theta0, doub
  * (k_cellface<PhysDminusC>), 10 GMRES dot products of one vector with 14
  * basis vectors (k_mdot_part + finalize), 11 GMRES projection on 13 basis
  * vectors with the norm of the result (k_maxpy_norm + finalize; needs
- * gmres_restart >= 13 for both). Also reports the kernel's algorithmic HBM
- * bytes per launch. */
+ * gmres_restart >= 13 for both), 12 the GMRES Arnoldi step's scaled D - C
+ * with the dot products of its result against 14 basis vectors
+ * (k_dmc_dots, partial sums only), 13 the same kernel with no basis vector.
+ * Also reports the kernel's algorithmic HBM bytes per launch. */
 atmo_status atmo_time_kernel(atmo_ctx* ctx, int which, int nrep, double* ms_per_launch, double* bytes_per_launch);
 
 /* ------------------------------------------------------------ device footprint
