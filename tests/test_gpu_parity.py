@@ -202,3 +202,26 @@ def test_mixed_schur_vmult_parity():
         op.vmult(out, dev.vector(af.BLOCK_PRESSURE, p))
         assert rel(out.download(), orc.vmult(p, 2)) < 1e-11, f"stage {hs}"
     assert dev.stats()["inner_cg_its"] == 0
+
+
+def test_c1_hundred_steps_against_oracle_fixture():
+    """40x20 Q2 hill, dt = 5 s, U = 10 m/s, N = 0.01: 100 IMEX steps of the
+    FP64 form in parity mode against the oracle's states after 2 and 100 steps
+    (tests/golden/c1_hill_40x20_Q2.npz, same tolerances, same locked
+    fixed-point count)."""
+    import os
+    fx = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c1_hill_40x20_Q2.npz"))
+    k, nx, nz = int(fx["k"]), int(fx["nx"]), int(fx["nz"])
+    dev = af.DeviceContext(k, nx, nz, h=af.witch_of_agnesi(nx), Ma=1.0, Fr=FR, dt=float(fx["dt"]), eps=1e-13,
+                           schur_inner_tol=1e-14, a_solve_tol=1e-14, fp_lock=3)
+    dev.upload_state(fx["rho0"], fx["u0"], fx["p0"])
+    done = 0
+    for n in (2, 100):
+        dev.imex_step(n - done)
+        done = n
+        st = dev.stats()
+        for name, a in zip(("rho", "u", "p"), dev.download_state()):
+            e = rel(a, fx[f"{name}_{n}"])
+            print(f"step {n} {name}: rel L2 {e:.3e}  (device GMRES its {st['gmres_its']}, "
+                  f"oracle {int(fx[f'gmres_its_{n}'])})", flush=True)
+            assert e < 1e-12, (n, name)

@@ -102,3 +102,27 @@ def test_C_equals_scaled_BT_for_constant_pfix():
     B = fx["B"]
     expect = -(gm * P * float(fx["Ma"]) ** 2 / (gm - 1)) * (B.T @ u)
     assert rel(o.vmult_enthalpy(u), expect) < 1e-12
+
+
+def test_c1_fixture_is_the_oracles_run():
+    """tests/golden/c1_hill_40x20_Q2.npz (the 100-step 40x20 Q2 run in parity
+    mode, written by tests/golden/make_c1_fixture.py): the first two steps are
+    rerun here, and the mass integral (the sum of M rho over all dofs) of every
+    stored state equals the initial one, the property the IMEX scheme
+    conserves for any step count (the density residual telescopes)."""
+    fx = np.load(os.path.join(HERE, "golden", "c1_hill_40x20_Q2.npz"))
+    k, nx, nz = int(fx["k"]), int(fx["nx"]), int(fx["nz"])
+    x = 40000.0 * np.arange(nx + 1) / nx
+    o = Oracle(k, nx, nz, hx=400.0 / (1.0 + ((x - 20000.0) / 1000.0) ** 2), periodic=True)
+    o.set_params(Ma=1.0, Fr=1.0 / np.sqrt(9.81), dt=float(fx["dt"]), eps=1e-13, schur_inner_tol=1e-14,
+                 a_solve_tol=1e-14, fp_lock=3)
+    r0, u0, p0 = o.init_hill(seed=int(fx["seed"]))
+    for a, b in ((r0, fx["rho0"]), (u0, fx["u0"]), (p0, fx["p0"])):
+        assert np.array_equal(a, b)
+    r2, u2, p2, _ = o.run(r0, u0, p0, 2)
+    for a, name in ((r2, "rho_2"), (u2, "u_2"), (p2, "p_2")):
+        assert rel(a, fx[name]) < 1e-12, name
+    m0 = o.vmult(fx["rho0"], 1).sum()
+    for n in (2, 100):
+        assert abs(o.vmult(fx[f"rho_{n}"], 1).sum() - m0) < 1e-12 * abs(m0), n
+    assert np.all(np.isfinite(fx["p_100"])) and rel(fx["p_100"], fx["p0"]) > 1e-6  # the flow has evolved
