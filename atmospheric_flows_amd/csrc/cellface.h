@@ -219,7 +219,67 @@ __device__ inline void cellface_columns(const MeshDev& m, const Ph& ph, const Fi
         }
       }
     }
-    if constexpr (!AMR) {
+    if constexpr (!AMR && NF == 1) {
+      const int cj = L.cell / m.nx, ci = L.cell - cj * m.nx;
+      // neighbours of the faces f = 0 left i-1, 1 right i+1, 2 bottom j-1, 3 top j+1; a wall side reads its
+      // own cell instead (the value is not used), so the loads need no branch
+      bool inner[4];
+      long nbase[4];
+#pragma unroll
+      for (int side = 0; side < 2; ++side) {
+        const int nj = side ? cj + 1 : cj - 1;
+        inner[2 + side] = nj >= 0 && nj < m.nz;
+        nbase[2 + side] = (long)((inner[2 + side] ? nj : cj) * m.nx + ci) * ND2;
+        int ni = side ? ci + 1 : ci - 1;
+        bool in_x = true;
+        if (ni < 0 || ni >= m.nx) {
+          if (m.periodic) ni = (ni + m.nx) % m.nx;
+          else in_x = false;
+        }
+        inner[side] = in_x;
+        nbase[side] = (long)(cj * m.nx + (in_x ? ni : ci)) * ND2;
+      }
+      // one field: the neighbour loads of all four faces are issued together (one memory latency for the
+      // face phase); operators of more fields take the faces one at a time below, as four faces' loads
+      // would not fit the registers their cell phase leaves
+      constexpr int FG = 4;
+#pragma unroll
+      for (int g0 = 0; g0 < 4; g0 += FG) {
+        double nb[FG][NF];
+#pragma unroll
+        for (int i = 0; i < FG; ++i) {
+          const int f = (g0 + i + 2) & 3;
+#pragma unroll
+          for (int q = 0; q < NF; ++q) nb[i][q] = face_trace<K>(in.p[q] + nbase[f], f ^ 1, Sc);
+        }
+#pragma unroll
+        for (int i = 0; i < FG; ++i) {
+          const int f = (g0 + i + 2) & 3;
+          const FaceGeo fg = face_geo(v, f);
+          const double wl = fg.len * g.wc;
+          double own[NF], F[NO];
+          // own trace from LDS (this lane is face point c of the bottom/top faces, s = c of the left/right
+          // ones): keeps tv dead after the cell integral
+#pragma unroll
+          for (int q = 0; q < NF; ++q) own[q] = face_trace<K>(&sh.in[q][off], f, Sc);
+          if (inner[f]) {
+            ph.face(own, nb[i], fg.n0, fg.n1, F);
+          } else if (Ph::BDRY) {
+            ph.bdry(own, fg.n0, fg.n1, F);
+          } else {
+#pragma unroll
+            for (int o = 0; o < NO; ++o) F[o] = 0;
+          }
+          if (f >= 2) {
+#pragma unroll
+            for (int o = 0; o < NO; ++o) Tv[o][f == 3 ? K : 0] += F[o] * wl;
+          } else {
+#pragma unroll
+            for (int o = 0; o < NO; ++o) sh.F[o][f][0][L.cw][L.c] = F[o] * wl;
+          }
+        }
+      }
+    } else if constexpr (!AMR) {
       const int cj = L.cell / m.nx, ci = L.cell - cj * m.nx;
       // bottom (j-1) and top (j+1) faces: this lane is face point c
 #pragma unroll

@@ -7,33 +7,11 @@
 
 namespace atmo {
 
+// The 1D transfer matrices in constant memory: a lane's rows are read with
+// lane-dependent indices. (Selected from literals by a runtime index, the 32
+// FP64 entries per matrix become SGPR pairs and spill.)
 template <int K>
-__device__ inline double trP(int a, int i, int n) {
-  constexpr auto& tr = basis::kTr<K>;
-  double v = 0;
-#pragma unroll
-  for (int aa = 0; aa < 2; ++aa)
-#pragma unroll
-    for (int ii = 0; ii < K + 1; ++ii)
-#pragma unroll
-      for (int nn = 0; nn < K + 1; ++nn)
-        if (aa == a && ii == i && nn == n) v = tr.P[aa][ii][nn];
-  return v;
-}
-
-template <int K>
-__device__ inline double trR(int a, int mm, int i) {
-  constexpr auto& tr = basis::kTr<K>;
-  double v = 0;
-#pragma unroll
-  for (int aa = 0; aa < 2; ++aa)
-#pragma unroll
-    for (int m2 = 0; m2 < K + 1; ++m2)
-#pragma unroll
-      for (int ii = 0; ii < K + 1; ++ii)
-        if (aa == a && m2 == mm && ii == i) v = tr.R[aa][m2][ii];
-  return v;
-}
+__constant__ basis::Transfer<K> cTr = basis::make_transfer<K>();
 
 // one thread per coarse dof
 template <int K, typename T>
@@ -51,7 +29,8 @@ __global__ void k_restrict(MeshDev mf, MeshDev mc, const T* __restrict__ fine, T
     }
     const int I = mc.child ? 0 : cc % mc.nx, J = mc.child ? 0 : cc / mc.nx;
     double acc = 0;
-#pragma unroll
+    // FP64: one row of children at a time keeps the loads in flight within the register budget
+#pragma unroll(sizeof(T) == 8 ? 1 : 2)
     for (int b = 0; b < 2; ++b)
 #pragma unroll
       for (int a = 0; a < 2; ++a) {
@@ -59,9 +38,9 @@ __global__ void k_restrict(MeshDev mf, MeshDev mc, const T* __restrict__ fine, T
         const T* u = fine + comp * Nf + cf * ND2;
 #pragma unroll
         for (int jj = 0; jj < ND; ++jj) {
-          const double pb = trP<K>(b, jj, m);
+          const double pb = cTr<K>.P[b][jj][m];
 #pragma unroll
-          for (int ii = 0; ii < ND; ++ii) acc += pb * trP<K>(a, ii, n) * (double)u[jj * ND + ii];
+          for (int ii = 0; ii < ND; ++ii) acc += pb * cTr<K>.P[a][ii][n] * (double)u[jj * ND + ii];
         }
       }
     coarse[t] = (T)acc;
@@ -80,7 +59,7 @@ __global__ void k_project_down(MeshDev mf, MeshDev mc, const double* __restrict_
     }
     const int I = mc.child ? 0 : cc % mc.nx, J = mc.child ? 0 : cc / mc.nx;
     double acc = 0;
-#pragma unroll
+#pragma unroll 1
     for (int b = 0; b < 2; ++b)
 #pragma unroll
       for (int a = 0; a < 2; ++a) {
@@ -88,9 +67,9 @@ __global__ void k_project_down(MeshDev mf, MeshDev mc, const double* __restrict_
         const double* u = fine + cf * ND2;
 #pragma unroll
         for (int jj = 0; jj < ND; ++jj) {
-          const double rb = trR<K>(b, m, jj);
+          const double rb = cTr<K>.R[b][m][jj];
 #pragma unroll
-          for (int ii = 0; ii < ND; ++ii) acc += rb * trR<K>(a, n, ii) * u[jj * ND + ii];
+          for (int ii = 0; ii < ND; ++ii) acc += rb * cTr<K>.R[a][n][ii] * u[jj * ND + ii];
         }
       }
     coarse[t] = acc;
@@ -206,12 +185,14 @@ __global__ __launch_bounds__(64) void k_cheb_sweep(MeshDev m, MeshDev mc, T* __r
         if (ident) {
           s_u[comp][t] += u[o];
         } else {
+          const double* Pa = cTr<K>.P[a][ii];
+          const double* Pb = cTr<K>.P[bb][jj];
           double acc = 0;
 #pragma unroll
           for (int mm = 0; mm < ND; ++mm) {
-            const double pb = trP<K>(bb, jj, mm);
+            const double pb = Pb[mm];
 #pragma unroll
-            for (int n = 0; n < ND; ++n) acc += pb * trP<K>(a, ii, n) * (double)u[mm * ND + n];
+            for (int n = 0; n < ND; ++n) acc += pb * Pa[n] * (double)u[mm * ND + n];
           }
           s_u[comp][t] += (T)acc;
         }

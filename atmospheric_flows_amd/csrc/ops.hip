@@ -437,8 +437,8 @@ __device__ inline double wave_sum8(const double (&v)[8]) {
 // the partial dot products of w with V_0 .. V_{nv-1} (column k at V + k*ldv)
 // and w.w: slot k of block b at part[k*pstride + b], slot nv is w.w. The
 // Arnoldi step of GMRES: its Schur product and the classical Gram-Schmidt dot
-// products without a second pass over w. Chunks of 8 slots keep every load
-// of a chunk in flight at once (addresses past nv are clamped to column 0 and
+// products without a second pass over w. Slots go in chunks of 8 whose loads
+// are all in flight at once (addresses past nv are clamped to column 0 and
 // their values dropped).
 template <int K, bool AMR>
 __global__ __launch_bounds__(64) void k_dmc_dots(MeshDev m, PhysDminusC ph, FieldPtrs<4> in, double* __restrict__ dst,
@@ -452,6 +452,8 @@ __global__ __launch_bounds__(64) void k_dmc_dots(MeshDev m, PhysDminusC ph, Fiel
   const long base = (long)blk * CPW * ND2;
   const long lim = (long)m.ncell * ND2;
   OutRows<K> R = out_rows<K>(threadIdx.x);
+  // the block's dofs: full passes of 64 lanes, then a tail of TL < 64 dofs
+  constexpr int NFULL = (CPW * ND2) / 64, TL = CPW * ND2 - 64 * NFULL;
   double wv[NPASS];
   long gi[NPASS];
 #pragma unroll
@@ -469,23 +471,65 @@ __global__ __launch_bounds__(64) void k_dmc_dots(MeshDev m, PhysDminusC ph, Fiel
     }
   }
   const int l = threadIdx.x & 63;
-  for (int k0 = 0; k0 <= nv; k0 += 8) {
+  // Dot products in chunks of 8 slots (slot k: column k of V, slot nv: w itself). Full passes: every lane
+  // multiplies its own dofs. Tail (TL dofs, TL/8 per slot when TL divides into 8 lanes' pairs): lanes 8s ..
+  // 8s+7 take slot s's tail as one 16-byte load of two dofs, with w's tail values fetched from the lanes
+  // that formed them, so a chunk takes 8*NFULL + 1 loads per lane instead of 8*NPASS.
+  constexpr bool PAIRS = TL == 16;
+  constexpr int NL = PAIRS ? NFULL : NPASS;  // per-lane single loads per slot
+  double wt0 = 0.0, wt1 = 0.0;
+  long gt = base;
+  if constexpr (PAIRS) {
+    const int src = 2 * (l & 7);
+    wt0 = __shfl(wv[NFULL], src, 64);
+    wt1 = __shfl(wv[NFULL], src + 1, 64);
+    gt = base + 64 * NFULL + src;
+    if (gt + 1 >= lim) gt = base;  // past the last cell: w's values there are zero
+  }
+  struct Chunk {
+    double y[8][NL];
+    double2 t;
+  };
+  auto load = [&](int k0, Chunk& c) {
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      const long col = (long)(k0 + s < nv ? k0 + s : 0) * ldv;
+#pragma unroll
+      for (int ps = 0; ps < NL; ++ps) c.y[s][ps] = V[col + gi[ps]];
+    }
+    if constexpr (PAIRS) {
+      const int s = l >> 3;
+      const long col = (long)(k0 + s < nv ? k0 + s : 0) * ldv;
+      c.t = *reinterpret_cast<const double2*>(V + col + gt);
+    }
+  };
+  auto reduce = [&](int k0, const Chunk& c) {
     double v[8];
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
       const int k = k0 + s;
-      const long col = (long)(k < nv ? k : 0) * ldv;
       double acc = 0.0;
 #pragma unroll
-      for (int ps = 0; ps < NPASS; ++ps) {
-        const double y = V[col + gi[ps]];
-        acc += wv[ps] * (k < nv ? y : (k == nv ? wv[ps] : 0.0));
-      }
+      for (int ps = 0; ps < NL; ++ps) acc += wv[ps] * (k < nv ? c.y[s][ps] : (k == nv ? wv[ps] : 0.0));
       v[s] = acc;
+    }
+    if constexpr (PAIRS) {
+      const int k = k0 + (l >> 3);
+      const double t0 = k < nv ? c.t.x : (k == nv ? wt0 : 0.0);
+      const double t1 = k < nv ? c.t.y : (k == nv ? wt1 : 0.0);
+      const double tail = wt0 * t0 + wt1 * t1;
+#pragma unroll
+      for (int s = 0; s < 8; ++s)
+        if (s == (l >> 3)) v[s] += tail;
     }
     const double sum = wave_sum8(v);
     const int slot = k0 + (l >> 3);
     if ((l & 7) == 0 && slot <= nv) part[(long)slot * pstride + blk] = sum;
+  };
+  for (int k0 = 0; k0 <= nv; k0 += 8) {
+    Chunk ca;
+    load(k0, ca);
+    reduce(k0, ca);
   }
 }
 
