@@ -157,14 +157,16 @@ __device__ inline void cellface_columns(const MeshDev& m, const Ph& ph, const Fi
   constexpr int ND = C::ND, NQ = C::NQ, ND2 = C::ND2;
   constexpr int NF = Ph::NF, NO = Ph::NO;
   LaneMap<K> L(m.ncell);
+  // the cell's vertices and the lane's table rows are loaded before the staged fields, so the three share
+  // one memory latency (an inactive lane reads cell 0's)
+  const CellVerts v = cell_verts(m, L.active ? L.cell : 0);
+  double Sc[ND], Dc[ND];
+  lane_rows<K, double>(L.c, Sc, Dc);
   stage_fields<K, NF, double, double, CFSmem<K, Ph, AMR>::CS>(&sh.in[0][0], in.p, m.ncell);
   pre();
   __syncthreads();
   if (L.active) {
-    const CellVerts v = cell_verts(m, L.cell);
     const ColGeo<double> g = col_geo<K, double>(v, L.c);
-    double Sc[ND], Dc[ND];
-    lane_rows<K, double>(L.c, Sc, Dc);
     const int off = L.cw * CFSmem<K, Ph, AMR>::CS;
     double tv[NF][ND];
 #pragma unroll

@@ -447,10 +447,18 @@ __global__ __launch_bounds__(64) void k_dmc_dots(MeshDev m, PhysDminusC ph, Fiel
                                                  double* __restrict__ part, long pstride) {
   constexpr int ND2 = Cfg<K>::ND2, CPW = Cfg<K>::CPW, NPASS = (CPW * ND2 + 63) / 64;
   __shared__ CFSmem<K, PhysDminusC, AMR> sh;
-  cellface_columns<K, PhysDminusC, AMR>(m, ph, in, sh);
   const int blk = cell_block();
   const long base = (long)blk * CPW * ND2;
   const long lim = (long)m.ncell * ND2;
+  // the Jacobi scale of the block's dofs is loaded with the staged fields (in flight during the column phase)
+  double sc[NPASS];
+  cellface_columns<K, PhysDminusC, AMR>(m, ph, in, sh, [&]() {
+#pragma unroll
+    for (int ps = 0; ps < NPASS; ++ps) {
+      const int t = threadIdx.x + 64 * ps;
+      sc[ps] = (t < CPW * ND2 && base + t < lim) ? scale[base + t] : 0.0;
+    }
+  });
   OutRows<K> R = out_rows<K>(threadIdx.x);
   // the block's dofs: full passes of 64 lanes, then a tail of TL < 64 dofs
   constexpr int NFULL = (CPW * ND2) / 64, TL = CPW * ND2 - 64 * NFULL;
@@ -463,7 +471,7 @@ __global__ __launch_bounds__(64) void k_dmc_dots(MeshDev m, PhysDminusC ph, Fiel
     wv[ps] = 0.0;
     if (t < CPW * ND2 && gi[ps] < lim) {
       if (!rows_invariant<K>()) R = out_rows<K>(t);
-      const double acc = cellface_out<K, PhysDminusC, AMR>(sh, 0, t, R) * scale[gi[ps]];
+      const double acc = cellface_out<K, PhysDminusC, AMR>(sh, 0, t, R) * sc[ps];
       dst[gi[ps]] = acc;
       wv[ps] = acc;
     } else {
