@@ -114,6 +114,9 @@ __global__ void k_set_cg(CGState* st, double tol, int max_its, double tol_factor
 // st->tol = factor * |b| (deal.II: SolverControl(max_its, eps*rhs.l2_norm()))
 __global__ void k_tol_from_norm(CGState* st, const double* nrm, double factor) { st->tol = factor * nrm[0]; }
 
+// extra streams of an engine for the concurrent eigenvalue estimates of its multigrid levels
+constexpr int ENGINE_EIG_STREAMS = 3;
+
 template <int K>
 class EngineK : public Engine {
  public:
@@ -144,7 +147,20 @@ class EngineK : public Engine {
     rb.bind(red_partials.p);
     cgst.alloc(4);
     cgst.zero(s_);
-    eig_hist.alloc(2 * MAX_EIG_ITS);
+    // eigenvalue estimates: one CG state and coefficient record per level, one reduction buffer per extra stream
+    const int nl = (int)lv.size();
+    eig_st.alloc(nl);
+    eig_st.zero(s_);
+    eig_hist.alloc((size_t)2 * MAX_EIG_ITS * nl);
+    // (the CG of an estimate reduces one slot at a time: RED_MAX_BLOCKS partials, no ping-pong buffers)
+    eig_red.alloc((size_t)EIG_STREAMS * RED_MAX_BLOCKS);
+    eig_red.zero(s_);
+    for (int e = 0; e < EIG_STREAMS; ++e) {
+      eig_rb[e].gen = eig_red.p + (size_t)e * RED_MAX_BLOCKS;
+      ATMO_HIP_CHECK(hipStreamCreateWithFlags(&eig_s[e], hipStreamNonBlocking));
+      ATMO_HIP_CHECK(hipEventCreateWithFlags(&eig_join[e], hipEventDisableTiming));
+    }
+    ATMO_HIP_CHECK(hipEventCreateWithFlags(&eig_fork, hipEventDisableTiming));
     scal.alloc(256);
     scal.zero(s_);
     coarse_its.alloc(1);
@@ -155,7 +171,8 @@ class EngineK : public Engine {
     ATMO_HIP_CHECK(hipHostMalloc((void**)&hscal, sizeof(double) * 256, hipHostMallocDefault));
     ATMO_HIP_CHECK(hipHostMalloc((void**)&hint, sizeof(int) * 4, hipHostMallocDefault));
     ATMO_HIP_CHECK(hipHostMalloc((void**)&hgm, sizeof(double) * GM_HOST, hipHostMallocDefault));
-    ATMO_HIP_CHECK(hipHostMalloc((void**)&heig, sizeof(double) * 2 * MAX_EIG_ITS, hipHostMallocDefault));
+    ATMO_HIP_CHECK(hipHostMalloc((void**)&heig, sizeof(double) * 2 * MAX_EIG_ITS * nl, hipHostMallocDefault));
+    ATMO_HIP_CHECK(hipHostMalloc((void**)&heig_st, sizeof(CGState) * nl, hipHostMallocDefault));
     for (auto& e : gm_ev) ATMO_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     rho_fix.alloc(Ns);
     pres_fix.alloc(Ns);
@@ -180,10 +197,8 @@ class EngineK : public Engine {
       level_diag(*L, BLK_RHO);
       level_diag(*L, BLK_D);
     }
-    for (int l = 1; l < (int)lv.size(); ++l) {
-      estimate_eig(l, BLK_RHO);
-      estimate_eig(l, BLK_D);
-    }
+    estimate_eig_levels(BLK_RHO);
+    estimate_eig_levels(BLK_D);
     sync();
   }
 
@@ -194,8 +209,18 @@ class EngineK : public Engine {
     (void)hipStreamSynchronize(s_);
     if (hgm) (void)hipHostFree(hgm);
     if (heig) (void)hipHostFree(heig);
+    if (heig_st) (void)hipHostFree(heig_st);
     for (auto& e : gm_ev)
       if (e) (void)hipEventDestroy(e);
+    for (int e = 0; e < EIG_STREAMS; ++e) {
+      if (eig_s[e]) {
+        (void)hipStreamSynchronize(eig_s[e]);
+        (void)hipStreamDestroy(eig_s[e]);
+      }
+      if (eig_join[e]) (void)hipEventDestroy(eig_join[e]);
+    }
+    if (eig_fork) (void)hipEventDestroy(eig_fork);
+    drop_eig_graph();
     (void)hipStreamDestroy(s_);
   }
 
@@ -223,6 +248,7 @@ class EngineK : public Engine {
     const bool rebuild = p.gmres_restart != P.gmres_restart;
     P = p;
     oc_ = consts();
+    drop_eig_graph();  // its kernels hold the previous parameters
     if (rebuild) gm_V.alloc((size_t)(P.gmres_restart + 2) * gm_ld());
     if (prec_change) {
       if (P.precision == 1 && !ainv.p) ainv.alloc((size_t)fine().ncell * SymBlock<K>::SIZE);
@@ -465,8 +491,18 @@ class EngineK : public Engine {
   // update coefficients, and one event per iteration
   static constexpr int GM_HOST = MAX_GMRES * GM_REC + MAX_GMRES + 2;
   double* hgm = nullptr;
-  double* heig = nullptr;  // pinned copy of eig_hist
-  DBuf<double> eig_hist;   // CG coefficients of the eigenvalue estimate: alpha_it, beta_it
+  // Chebyshev eigenvalue estimates of all levels run concurrently: the finest level on s_, the others
+  // spread over EIG_STREAMS extra streams (default priority), each with its own reduction buffer
+  static constexpr int EIG_STREAMS = ENGINE_EIG_STREAMS;
+  hipStream_t eig_s[EIG_STREAMS] = {};
+  hipEvent_t eig_fork = nullptr, eig_join[EIG_STREAMS] = {};
+  RedBuf eig_rb[EIG_STREAMS];
+  DBuf<double> eig_red;
+  DBuf<CGState> eig_st;      // per level
+  CGState* heig_st = nullptr;  // pinned copy of eig_st
+  hipGraphExec_t eig_graph_a = nullptr;  // the A block's estimates, captured
+  double* heig = nullptr;      // pinned copy of eig_hist
+  DBuf<double> eig_hist;       // per level: CG coefficients of the eigenvalue estimate, alpha_it, beta_it
   hipEvent_t gm_ev[MAX_GMRES] = {};
   DBuf<double> gm_rec, gm_sig;
   DBuf<double> gm_part;  // per-block partial dot products of the fused Arnoldi step, [slot][block]
@@ -622,7 +658,8 @@ class EngineK : public Engine {
 
   // level operator in precision T: dst = Op_blk src (+ add_scale*add)
   template <typename T>
-  void level_apply(Level& L, int blk, const T* src, T* dst, T cf = 1, const T* add = nullptr, T add_scale = 0);
+  void level_apply(Level& L, int blk, const T* src, T* dst, T cf = 1, const T* add = nullptr, T add_scale = 0,
+                   hipStream_t s = nullptr);
 
   void fine_apply(int blk, const double* src, double* dst) { level_apply<double>(fine(), blk, src, dst); }
 
@@ -640,7 +677,7 @@ class EngineK : public Engine {
       if (P.precision == 1) vec_cast<float, double>(L->Ns, L->rho64.p, L->rho32.p, s_);
       level_diag(*L, BLK_A);
     }
-    for (int l = 1; l < (int)lv.size(); ++l) estimate_eig(l, BLK_A);
+    estimate_eig_levels(BLK_A);
     if (P.precision == 1) build_block_inverse<K>(F.m, rho_fix.p, ainv.p, s_);
   }
 
@@ -653,42 +690,102 @@ class EngineK : public Engine {
   // each iteration's alpha and beta are recorded on the device (eig_hist);
   // the host reads them back once. The iterations that ran are the first
   // st.it (the one that finished the solve included).
-  void estimate_eig(int l, int blk) {
+  void eig_enqueue(int l, int blk, hipStream_t s, RedBuf& r) {
     Level& L = *lv[l];
     const long n = (long)ncomp(blk) * L.Ns;
     const double* b = ncomp(blk) == 1 ? L.eigb1.p : L.eigb2.p;
-    CGState* st = cgst.p + 2;
+    CGState* st = eig_st.p + l;
     double *g = L.x64.p, *d = L.b64.p, *h = L.t64.p, *x = L.d64.p;
     const double* Pd = L.inv64[blk].p;
-    double* hist = eig_hist.p;
+    double* hist = eig_hist.p + (size_t)2 * MAX_EIG_ITS * l;
     const int nit = P.cheb_eig_its;
-    hipLaunchKernelGGL(k_set_cg, dim3(1), dim3(1), 0, s_, st, 1e-2, nit, 0.0);
-    cg_residual(n, g, b, false, s_);
-    vec_fill(n, 0.0, x, s_);
-    vec_mul(n, g, Pd, h, s_);
-    vec_axpby(n, -1.0, h, 0.0, d, s_);
-    cg_dot(n, g, h, st, DOT_GH_INIT, rb, s_);
+    hipLaunchKernelGGL(k_set_cg, dim3(1), dim3(1), 0, s, st, 1e-2, nit, 0.0);
+    cg_residual(n, g, b, false, s);
+    vec_fill(n, 0.0, x, s);
+    vec_mul(n, g, Pd, h, s);
+    vec_axpby(n, -1.0, h, 0.0, d, s);
+    cg_dot(n, g, h, st, DOT_GH_INIT, r, s);
     for (int it = 0; it < nit; ++it) {
-      level_apply<double>(L, blk, d, h);
-      cg_dot(n, d, h, st, DOT_ALPHA, rb, s_, hist + 2 * it);
-      cg_update(n, x, g, d, h, st, rb, s_);
+      level_apply<double>(L, blk, d, h, 1.0, nullptr, 0.0, s);
+      cg_dot(n, d, h, st, DOT_ALPHA, r, s, hist + 2 * it);
+      cg_update(n, x, g, d, h, st, r, s);
       if (it + 1 == nit) break;  // the last beta is never used
-      vec_mul(n, g, Pd, h, s_);
-      cg_dot(n, g, h, st, DOT_BETA, rb, s_, hist + 2 * it + 1);
-      cg_dir(n, d, h, st, s_);
+      vec_mul(n, g, Pd, h, s);
+      cg_dot(n, g, h, st, DOT_BETA, r, s, hist + 2 * it + 1);
+      cg_dir(n, d, h, st, s);
     }
-    ATMO_HIP_CHECK(hipMemcpyAsync(heig, hist, sizeof(double) * 2 * nit, hipMemcpyDeviceToHost, s_));
-    read_state(2);
-    const int m = std::min(nit, hst[2].it);
+  }
+
+  void eig_bounds(int l, int blk) {
+    Level& L = *lv[l];
+    const int nit = P.cheb_eig_its;
+    const double* he = heig + (size_t)2 * MAX_EIG_ITS * l;
+    const int m = std::min(nit, heig_st[l].it);
     std::vector<double> a(m), o(m > 0 ? m - 1 : 0);
     for (int j = 0; j < m; ++j) {
-      a[j] = 1.0 / heig[2 * j] + (j > 0 ? heig[2 * (j - 1) + 1] / heig[2 * (j - 1)] : 0.0);
-      if (j + 1 < m) o[j] = std::sqrt(heig[2 * j + 1]) / heig[2 * j];
+      a[j] = 1.0 / he[2 * j] + (j > 0 ? he[2 * (j - 1) + 1] / he[2 * (j - 1)] : 0.0);
+      if (j + 1 < m) o[j] = std::sqrt(he[2 * j + 1]) / he[2 * j];
     }
     double lmin, lmax;
     tridiag_extreme(a, o, lmin, lmax);
     L.lmax[blk] = 1.2 * lmax;
     L.lmin[blk] = L.lmax[blk] / P.cheb_range;
+  }
+
+  // The estimates of levels 1 .. finest, each on its own level's vectors: the finest on s_, the others
+  // round-robin on the extra streams, which start after the work queued on s_ so far and which s_ waits for.
+  void eig_enqueue_levels(int blk) {
+    const int nl = (int)lv.size();
+    ATMO_HIP_CHECK(hipEventRecord(eig_fork, s_));
+    for (int e = 0; e < EIG_STREAMS; ++e) ATMO_HIP_CHECK(hipStreamWaitEvent(eig_s[e], eig_fork, 0));
+    for (int l = 1; l < nl; ++l) {
+      if (l == nl - 1) eig_enqueue(l, blk, s_, rb);
+      else eig_enqueue(l, blk, eig_s[(l - 1) % EIG_STREAMS], eig_rb[(l - 1) % EIG_STREAMS]);
+    }
+    for (int e = 0; e < EIG_STREAMS; ++e) {
+      ATMO_HIP_CHECK(hipEventRecord(eig_join[e], eig_s[e]));
+      ATMO_HIP_CHECK(hipStreamWaitEvent(s_, eig_join[e], 0));
+    }
+  }
+
+  // Bounds of every level for block blk; one host synchronization reads every level's record. The A block's
+  // estimates (rerun whenever the density of the velocity block changes) are captured once into a HIP graph
+  // (about 90 kernels per level) and replayed; the captured kernels take only this engine's fixed buffers and
+  // parameters, and set_params drops the graph.
+  void estimate_eig_levels(int blk) {
+    const int nl = (int)lv.size();
+    if (nl < 2) return;
+    if (blk == BLK_A) {
+      if (!eig_graph_a) {
+        hipGraph_t g = nullptr;
+        ATMO_HIP_CHECK(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
+        try {
+          eig_enqueue_levels(blk);
+        } catch (...) {
+          (void)hipStreamEndCapture(s_, &g);
+          if (g) (void)hipGraphDestroy(g);
+          throw;
+        }
+        ATMO_HIP_CHECK(hipStreamEndCapture(s_, &g));
+        const hipError_t e = hipGraphInstantiate(&eig_graph_a, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        ATMO_HIP_CHECK(e);
+      }
+      ATMO_HIP_CHECK(hipGraphLaunch(eig_graph_a, s_));
+    } else {
+      eig_enqueue_levels(blk);
+    }
+    ATMO_HIP_CHECK(hipMemcpyAsync(heig, eig_hist.p, sizeof(double) * 2 * MAX_EIG_ITS * nl, hipMemcpyDeviceToHost, s_));
+    ATMO_HIP_CHECK(hipMemcpyAsync(heig_st, eig_st.p, sizeof(CGState) * nl, hipMemcpyDeviceToHost, s_));
+    sync();
+    for (int l = 1; l < nl; ++l) eig_bounds(l, blk);
+  }
+
+  void drop_eig_graph() {
+    if (eig_graph_a) {
+      (void)hipGraphExecDestroy(eig_graph_a);
+      eig_graph_a = nullptr;
+    }
   }
 
   void read_state(int slot) {
@@ -1125,10 +1222,12 @@ class EngineK : public Engine {
 
 template <int K>
 template <typename T>
-void EngineK<K>::level_apply(Level& L, int blk, const T* src, T* dst, T cf, const T* add, T add_scale) {
+void EngineK<K>::level_apply(Level& L, int blk, const T* src, T* dst, T cf, const T* add, T add_scale,
+                             hipStream_t s) {
   const T* rho = nullptr;
   if (blk == BLK_A) rho = std::is_same<T, double>::value ? (const T*)L.rho64.p : (const T*)L.rho32.p;
-  launch_mass<K, T>(L.m, ncomp(blk), blk == BLK_A, src, rho, dst, (T)(cf * coeff(blk)), add, add_scale, L.Ns, s_);
+  launch_mass<K, T>(L.m, ncomp(blk), blk == BLK_A, src, rho, dst, (T)(cf * coeff(blk)), add, add_scale, L.Ns,
+                    s ? s : s_);
 }
 
 // Mirrors the allocations of EngineK (constructor, build_levels): device
@@ -1155,7 +1254,8 @@ size_t engine_device_bytes(int k, const MeshDesc& mesh, const EngineParams& p) {
   }
   const size_t Ns = (size_t)mesh.nx * mesh.nz * nd2;
   b += RedBuf::doubles() * D + 4 * sizeof(CGState) + 256 * D + sizeof(int) + DIAG_PARTIALS * D + DG_SLOTS * D;
-  b += 2 * MAX_EIG_ITS * D;               // CG coefficients of the eigenvalue estimates
+  b += lev.size() * (2 * MAX_EIG_ITS * D + sizeof(CGState));  // eigenvalue estimates: records and CG states
+  b += ENGINE_EIG_STREAMS * RED_MAX_BLOCKS * D;               // their reduction partials on the extra streams
   b += 2 * Ns * D;                        // rho_fix, pres_fix
   b += 8 * 2 * Ns * D + Ns * D;           // CG / Schur / inner work
   if (p.precision == 1) b += (size_t)mesh.nx * mesh.nz * nd2 * (nd2 + 1) / 2 * D;  // packed FP64 cell-block inverses of A
