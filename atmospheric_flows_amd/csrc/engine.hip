@@ -155,12 +155,14 @@ class EngineK : public Engine {
     // (the CG of an estimate reduces one slot at a time: RED_MAX_BLOCKS partials, no ping-pong buffers)
     eig_red.alloc((size_t)EIG_STREAMS * RED_MAX_BLOCKS);
     eig_red.zero(s_);
+    eig_concurrent = fine().ncell >= EIG_CONCURRENT_CELLS;
     for (int e = 0; e < EIG_STREAMS; ++e) {
       eig_rb[e].gen = eig_red.p + (size_t)e * RED_MAX_BLOCKS;
+      if (!eig_concurrent) continue;
       ATMO_HIP_CHECK(hipStreamCreateWithFlags(&eig_s[e], hipStreamNonBlocking));
       ATMO_HIP_CHECK(hipEventCreateWithFlags(&eig_join[e], hipEventDisableTiming));
     }
-    ATMO_HIP_CHECK(hipEventCreateWithFlags(&eig_fork, hipEventDisableTiming));
+    if (eig_concurrent) ATMO_HIP_CHECK(hipEventCreateWithFlags(&eig_fork, hipEventDisableTiming));
     scal.alloc(256);
     scal.zero(s_);
     coarse_its.alloc(1);
@@ -220,7 +222,6 @@ class EngineK : public Engine {
       if (eig_join[e]) (void)hipEventDestroy(eig_join[e]);
     }
     if (eig_fork) (void)hipEventDestroy(eig_fork);
-    drop_eig_graph();
     (void)hipStreamDestroy(s_);
   }
 
@@ -248,7 +249,6 @@ class EngineK : public Engine {
     const bool rebuild = p.gmres_restart != P.gmres_restart;
     P = p;
     oc_ = consts();
-    drop_eig_graph();  // its kernels hold the previous parameters
     if (rebuild) gm_V.alloc((size_t)(P.gmres_restart + 2) * gm_ld());
     if (prec_change) {
       if (P.precision == 1 && !ainv.p) ainv.alloc((size_t)fine().ncell * SymBlock<K>::SIZE);
@@ -494,13 +494,16 @@ class EngineK : public Engine {
   // Chebyshev eigenvalue estimates of all levels run concurrently: the finest level on s_, the others
   // spread over EIG_STREAMS extra streams (default priority), each with its own reduction buffer
   static constexpr int EIG_STREAMS = ENGINE_EIG_STREAMS;
+  // below this many fine cells the estimates are launch-bound and stay on s_: with many engines on one
+  // GPU (ensembles) the extra streams of each would share the few hardware queues
+  static constexpr int EIG_CONCURRENT_CELLS = 65536;
+  bool eig_concurrent = false;
   hipStream_t eig_s[EIG_STREAMS] = {};
   hipEvent_t eig_fork = nullptr, eig_join[EIG_STREAMS] = {};
   RedBuf eig_rb[EIG_STREAMS];
   DBuf<double> eig_red;
   DBuf<CGState> eig_st;      // per level
   CGState* heig_st = nullptr;  // pinned copy of eig_st
-  hipGraphExec_t eig_graph_a = nullptr;  // the A block's estimates, captured
   double* heig = nullptr;      // pinned copy of eig_hist
   DBuf<double> eig_hist;       // per level: CG coefficients of the eigenvalue estimate, alpha_it, beta_it
   hipEvent_t gm_ev[MAX_GMRES] = {};
@@ -736,6 +739,10 @@ class EngineK : public Engine {
   // round-robin on the extra streams, which start after the work queued on s_ so far and which s_ waits for.
   void eig_enqueue_levels(int blk) {
     const int nl = (int)lv.size();
+    if (!eig_concurrent) {
+      for (int l = 1; l < nl; ++l) eig_enqueue(l, blk, s_, rb);
+      return;
+    }
     ATMO_HIP_CHECK(hipEventRecord(eig_fork, s_));
     for (int e = 0; e < EIG_STREAMS; ++e) ATMO_HIP_CHECK(hipStreamWaitEvent(eig_s[e], eig_fork, 0));
     for (int l = 1; l < nl; ++l) {
@@ -748,44 +755,17 @@ class EngineK : public Engine {
     }
   }
 
-  // Bounds of every level for block blk; one host synchronization reads every level's record. The A block's
-  // estimates (rerun whenever the density of the velocity block changes) are captured once into a HIP graph
-  // (about 90 kernels per level) and replayed; the captured kernels take only this engine's fixed buffers and
-  // parameters, and set_params drops the graph.
+  // Bounds of every level for block blk; one host synchronization reads every level's record. (Capturing
+  // the A block's estimates into a HIP graph saved 1.4 % of the mixed step but failed when several engines
+  // run in separate threads: another thread's allocation or free during a capture invalidates it.)
   void estimate_eig_levels(int blk) {
     const int nl = (int)lv.size();
     if (nl < 2) return;
-    if (blk == BLK_A) {
-      if (!eig_graph_a) {
-        hipGraph_t g = nullptr;
-        ATMO_HIP_CHECK(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
-        try {
-          eig_enqueue_levels(blk);
-        } catch (...) {
-          (void)hipStreamEndCapture(s_, &g);
-          if (g) (void)hipGraphDestroy(g);
-          throw;
-        }
-        ATMO_HIP_CHECK(hipStreamEndCapture(s_, &g));
-        const hipError_t e = hipGraphInstantiate(&eig_graph_a, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        ATMO_HIP_CHECK(e);
-      }
-      ATMO_HIP_CHECK(hipGraphLaunch(eig_graph_a, s_));
-    } else {
-      eig_enqueue_levels(blk);
-    }
+    eig_enqueue_levels(blk);
     ATMO_HIP_CHECK(hipMemcpyAsync(heig, eig_hist.p, sizeof(double) * 2 * MAX_EIG_ITS * nl, hipMemcpyDeviceToHost, s_));
     ATMO_HIP_CHECK(hipMemcpyAsync(heig_st, eig_st.p, sizeof(CGState) * nl, hipMemcpyDeviceToHost, s_));
     sync();
     for (int l = 1; l < nl; ++l) eig_bounds(l, blk);
-  }
-
-  void drop_eig_graph() {
-    if (eig_graph_a) {
-      (void)hipGraphExecDestroy(eig_graph_a);
-      eig_graph_a = nullptr;
-    }
   }
 
   void read_state(int slot) {

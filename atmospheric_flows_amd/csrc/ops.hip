@@ -482,7 +482,8 @@ __global__ __launch_bounds__(64) void k_dmc_dots(MeshDev m, PhysDminusC ph, Fiel
   // Dot products in chunks of 8 slots (slot k: column k of V, slot nv: w itself). Full passes: every lane
   // multiplies its own dofs. Tail (TL dofs, TL/8 per slot when TL divides into 8 lanes' pairs): lanes 8s ..
   // 8s+7 take slot s's tail as one 16-byte load of two dofs, with w's tail values fetched from the lanes
-  // that formed them, so a chunk takes 8*NFULL + 1 loads per lane instead of 8*NPASS.
+  // that formed them, so a chunk takes 8*NFULL + 1 loads per lane instead of 8*NPASS. The basis is read with
+  // non-temporal loads (streamed, not kept in the caches).
   constexpr bool PAIRS = TL == 16;
   constexpr int NL = PAIRS ? NFULL : NPASS;  // per-lane single loads per slot
   double wt0 = 0.0, wt1 = 0.0;
@@ -503,12 +504,16 @@ __global__ __launch_bounds__(64) void k_dmc_dots(MeshDev m, PhysDminusC ph, Fiel
     for (int s = 0; s < 8; ++s) {
       const long col = (long)(k0 + s < nv ? k0 + s : 0) * ldv;
 #pragma unroll
-      for (int ps = 0; ps < NL; ++ps) c.y[s][ps] = V[col + gi[ps]];
+      for (int ps = 0; ps < NL; ++ps) c.y[s][ps] = __builtin_nontemporal_load(V + col + gi[ps]);
     }
     if constexpr (PAIRS) {
       const int s = l >> 3;
       const long col = (long)(k0 + s < nv ? k0 + s : 0) * ldv;
-      c.t = *reinterpret_cast<const double2*>(V + col + gt);
+      {
+        typedef double nt_d2 __attribute__((ext_vector_type(2)));
+        const nt_d2 tv = __builtin_nontemporal_load(reinterpret_cast<const nt_d2*>(V + col + gt));
+        c.t = make_double2(tv.x, tv.y);
+      }
     }
   };
   auto reduce = [&](int k0, const Chunk& c) {
