@@ -190,9 +190,9 @@ class EngineK : public Engine {
     gm_part.alloc((size_t)RED_MAX_SLOTS * cell_blocks<K>(fine().ncell));
     gm_sig.alloc(MAX_GMRES + 2);
     for (auto* b : {&rho_old, &rho2, &rho3, &rho_curr, &p_old, &p2, &p3, &p_fixed, &p_fixed_old, &rhs_p}) b->alloc(Ns);
-    for (auto* b : {&u_old, &u2, &u3, &u_curr, &u_fixed, &tmp1, &rhs_u}) b->alloc(2 * Ns);
+    for (auto* b : {&u_old, &u2, &u3, &u_curr, &u_fixed, &tmp1, &rhs_u, &rhs_u0}) b->alloc(2 * Ns);
     for (auto* b : {&rho_old, &rho2, &rho3, &rho_curr, &p_old, &p2, &p3, &p_fixed, &p_fixed_old, &rhs_p, &u_old, &u2,
-                    &u3, &u_curr, &u_fixed, &tmp1, &rhs_u})
+                    &u3, &u_curr, &u_fixed, &tmp1, &rhs_u, &rhs_u0})
       b->zero(s_);
     // geometry-only diagonals and eigenvalue bounds (M_rho and D blocks)
     for (auto& L : lv) {
@@ -514,6 +514,7 @@ class EngineK : public Engine {
   DBuf<double> cg_g, cg_d, cg_h, s_t1, s_t2, s_t3, in_g, in_d, in_h, gm_V, gm_p, gm_coef;
   DBuf<double> rho_old, rho2, rho3, rho_curr, u_old, u2, u3, u_curr, u_fixed, p_old, p2, p3, p_fixed, p_fixed_old,
       tmp1, rhs_u, rhs_p;
+  DBuf<double> rhs_u0;  // rhs_u of the current stage before update_velocity subtracts B p_fixed
   EngineStats st_;
   int last_inner = 8;  // iterations of the last inner CG (first batch of the next)
   size_t bytes_ = 0;  // device bytes requested by this engine's buffers
@@ -1129,22 +1130,27 @@ class EngineK : public Engine {
   }
 
   // pressure_fixed_point (Atmospheric_Flow.cc:544-673)
-  void pressure_fixed_point(int hs) {
+  // rhs_u and tmp_1 = A^{-1} rhs_u do not depend on the fixed-point iterate: formed in the stage's first
+  // iteration only (the reference recomputes both every iteration, Atmospheric_Flow.cc:558-571; its A solve
+  // then restarts from the previous tmp_1, whose residual already meets the tolerance, and returns it
+  // unchanged, so the values are the same)
+  void pressure_fixed_point(int hs, bool first) {
     set_stages(hs, 2);
     set_pres_fixed(p_fixed_old.p);
+    const long Ns = fine().Ns;
     if (hs == 1) {
       const double* sp[5] = {rho_old.p, u_old.p, p_old.p, rho2.p, u_fixed.p};
       rhs_pressure(sp, 5, rhs_p.p);
       const double* su[4] = {rho_old.p, u_old.p, p_old.p, rho2.p};
-      rhs_velocity(su, 4, rhs_u.p);
+      if (first) rhs_velocity(su, 4, rhs_u0.p);
     } else {
       const double* sp[8] = {rho_old.p, u_old.p, p_old.p, rho2.p, u2.p, p2.p, rho3.p, u_fixed.p};
       rhs_pressure(sp, 8, rhs_p.p);
       const double* su[7] = {rho_old.p, u_old.p, p_old.p, rho2.p, u2.p, p2.p, rho3.p};
-      rhs_velocity(su, 7, rhs_u.p);
+      if (first) rhs_velocity(su, 7, rhs_u0.p);
     }
-    solve_A_tight(tmp1.p, rhs_u.p);
-    const long Ns = fine().Ns;
+    if (first) solve_A_tight(tmp1.p, rhs_u0.p);
+    vec_copy(2 * Ns, rhs_u0.p, rhs_u.p, s_);
     vmult_enthalpy(tmp1.p, s_t3.p);
     vec_axpby(Ns, -1.0, s_t3.p, 1.0, rhs_p.p, s_);
     vec_copy(Ns, p_fixed_old.p, p_fixed.p, s_);
@@ -1182,7 +1188,7 @@ class EngineK : public Engine {
   void fixed_point_loop(int hs) {
     const long Ns = fine().Ns;
     for (int iter = 0; iter < P.fp_max_iter; ++iter) {
-      pressure_fixed_point(hs);
+      pressure_fixed_point(hs, iter == 0);
       update_velocity(hs);
       ++st_.fp_its;
       vec_linf(Ns, p_fixed.p, nullptr, scal.p + 2, rb, s_);
@@ -1242,7 +1248,7 @@ size_t engine_device_bytes(int k, const MeshDesc& mesh, const EngineParams& p) {
   b += (size_t)(p.gmres_restart + 2) * (Ns + 32) * D + Ns * D + (MAX_GMRES + 2) * D;  // basis: leading dim N_s + 32
   b += (size_t)MAX_GMRES * GM_REC * D + (MAX_GMRES + 2) * D;  // GMRES records and basis scales
   b += (size_t)RED_MAX_SLOTS * ((mesh.nx * mesh.nz + 64 / (2 * k + 1) - 1) / (64 / (2 * k + 1))) * D;  // Arnoldi partials
-  b += 10 * Ns * D + 7 * 2 * Ns * D;      // IMEX state and stage fields
+  b += 10 * Ns * D + 8 * 2 * Ns * D;      // IMEX state and stage fields
   return b;
 }
 
