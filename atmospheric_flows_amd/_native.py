@@ -106,6 +106,8 @@ SIGNATURES = [
     ("atmo_step_diagnostics", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(StepDiag)]),
     ("atmo_init_hill_state", ctypes.c_int, [ctypes.POINTER(MeshDesc)] + [ctypes.c_double] * 8 +
      [ctypes.c_uint64, _dp, _dp, _dp]),
+    ("atmo_write_vtu", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]),
+    ("atmo_write_vtu_host", ctypes.c_int, [ctypes.POINTER(MeshDesc), _dp, _dp, _dp, ctypes.c_char_p, ctypes.c_int]),
     ("atmo_time_kernel", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp]),
     ("atmo_device_bytes", ctypes.c_int, [ctypes.POINTER(MeshDesc), ctypes.POINTER(Params),
                                          ctypes.POINTER(ctypes.c_size_t)]),

@@ -13,6 +13,7 @@
 #include "engine.h"
 #include "ensemble.h"
 #include "krylov.h"
+#include "vtu.h"
 
 using namespace atmo;
 
@@ -560,6 +561,32 @@ atmo_status atmo_init_hill_state(const atmo_mesh_desc* mesh, double theta0, doub
     m.H = mesh->H;
     m.h.assign(mesh->h, mesh->h + mesh->nx + 1);
     hill_initial_state(mesh->degree, m, theta0, p0, N, U, g, R, gamma_gas, perturbation, seed, rho, u, p);
+    return ATMO_OK;
+  });
+}
+
+atmo_status atmo_write_vtu(atmo_ctx* ctx, const char* path, int high_order) {
+  return guarded(ctx, [&]() {
+    if (!ctx || !path) throw std::invalid_argument("null argument");
+    const long Ns = ctx->eng->ns();
+    std::vector<double> rho(Ns), u(2 * Ns), p(Ns);
+    hipStream_t s = ctx->eng->stream();
+    ATMO_HIP_CHECK(hipMemcpyAsync(rho.data(), ctx->eng->state_ptr(0), Ns * sizeof(double), hipMemcpyDeviceToHost, s));
+    ATMO_HIP_CHECK(hipMemcpyAsync(u.data(), ctx->eng->state_ptr(1), 2 * Ns * sizeof(double), hipMemcpyDeviceToHost, s));
+    ATMO_HIP_CHECK(hipMemcpyAsync(p.data(), ctx->eng->state_ptr(2), Ns * sizeof(double), hipMemcpyDeviceToHost, s));
+    ATMO_HIP_CHECK(hipStreamSynchronize(s));
+    write_vtu(ctx->degree, ctx->mesh, ctx->eng->cell_geometry(), rho.data(), u.data(), p.data(), path, high_order != 0);
+    return ATMO_OK;
+  });
+}
+
+atmo_status atmo_write_vtu_host(const atmo_mesh_desc* mesh, const double* rho, const double* u, const double* p,
+                                const char* path, int high_order) {
+  return guarded(nullptr, [&]() {
+    if (!mesh || !mesh->h || !rho || !u || !p || !path) throw std::invalid_argument("null argument");
+    if (mesh->degree < 1 || mesh->degree > 3 || mesh->nx < 1 || mesh->nz < 1)
+      throw std::invalid_argument("bad mesh description");
+    write_vtu(mesh->degree, to_mesh(mesh), {}, rho, u, p, path, high_order != 0);
     return ATMO_OK;
   });
 }

@@ -9,12 +9,16 @@
 // after each density update, maximum celerity and velocity with the
 // acoustic and advective Courant numbers), clips the last time step to land
 // on final_time (:1243-1250) and stops with an error on a non-finite state.
-// Field output (VTU) is not written.
+// Field output as output_results (:886-918, 1104, 1237-1256): <dir>/solution-NNNNN.vtu
+// and <dir>/solution_high_order-NNNNN.vtu at step 0, every output_interval steps
+// and after the last step; written only when <dir> exists (the reference's
+// output directory must exist beforehand as well, runtime_parameters.h:46-48).
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <iostream>
 #include <string>
+#include <sys/stat.h>
 #include <vector>
 
 #include "../../include/atmo_operator.hpp"
@@ -44,6 +48,17 @@ int run(const atmo::DataStorage& d) {
   ctx.initial_state(rho, u, pr, 300.0, 1e5, d.brunt_vaisala, d.wind, d.gravity, 287.0, 1.4, 1e-4,
                     (uint64_t)d.seed);
   ctx.upload_state(rho, u, pr);
+  struct stat sb;
+  const bool out = stat(d.dir.c_str(), &sb) == 0 && S_ISDIR(sb.st_mode);
+  if (!out) std::printf("Output directory %s does not exist: no VTU output\n", d.dir.c_str());
+  auto output_results = [&](unsigned int step) {
+    if (!out) return;
+    char num[16];
+    std::snprintf(num, sizeof num, "%05u", step);
+    ctx.write_vtu(d.dir + "/solution-" + num + ".vtu", false);
+    ctx.write_vtu(d.dir + "/solution_high_order-" + num + ".vtu", true);
+  };
+  output_results(0);
 
   const double T = d.final_time;
   double time = d.initial_time, dt = d.dt;
@@ -66,6 +81,7 @@ int run(const atmo::DataStorage& d) {
                 g.max_celerity / d.Mach, g.cfl_c, g.max_velocity, g.cfl_u);
     if (g.nonfinite != 0) throw atmo::Error(ATMO_NONFINITE, "non-finite values in the state after step " +
                                                                 std::to_string(n));
+    if (n % d.output_interval == 0) output_results(n);
     if (T - time < dt && T - time > 1e-10) {
       dt = T - time;
       atmo_params q = ctx.params();
@@ -73,6 +89,7 @@ int run(const atmo::DataStorage& d) {
       ctx.set_params(q);
     }
   }
+  if (n % d.output_interval != 0) output_results(n);
   const atmo_stats st = ctx.stats();
   const double dofs = 4.0 * (double)ctx.scalar_dofs();
   std::printf("%u steps, %.3f s in steps, %.4g DoF-steps/s; CG %ld GMRES %ld inner CG %ld fixed-point %ld\n", n,

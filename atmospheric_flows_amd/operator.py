@@ -126,6 +126,10 @@ class DeviceContext:
         N.check(N.load().atmo_state_download(self.handle, _ptr(rho), _ptr(u), _ptr(p)), self.handle)
         return rho, u, p
 
+    def write_vtu(self, path, high_order=False):
+        """output_results (Atmospheric_Flow.cc:886-918): the current state as VTU."""
+        N.check(N.load().atmo_write_vtu(self.handle, str(path).encode(), 1 if high_order else 0), self.handle)
+
     def imex_step(self, nsteps=1):
         N.check(N.load().atmo_imex_step(self.handle, nsteps), self.handle)
 

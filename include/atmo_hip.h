@@ -193,6 +193,19 @@ atmo_status atmo_init_hill_state(const atmo_mesh_desc* mesh, double theta0, doub
                                  double R, double gamma_gas, double perturbation, uint64_t seed, double* rho,
                                  double* u, double* p);
 
+/* VTU output of the current state, as EulerSolver::output_results
+ * (Atmospheric_Flow.cc:886-918): point data "rho", "u" (3 components, the
+ * third zero) and "p" on DataOut's patches of `degree` subdivisions per cell
+ * (build_patches(mapping, degree_u), :903), each cell with its own points; with
+ * high_order != 0 one Lagrange quadrilateral of (k+1)^2 equispaced nodes per
+ * cell instead (the solution_high_order-*.vtu of :909-913). Data arrays are
+ * VTK inline binary (base64, UInt32 size header). Off the timed path: the state
+ * is copied to the host. */
+atmo_status atmo_write_vtu(atmo_ctx* ctx, const char* path, int high_order);
+/* The same for a state held on the host, on a structured mesh; needs no GPU. */
+atmo_status atmo_write_vtu_host(const atmo_mesh_desc* mesh, const double* rho, const double* u, const double* p,
+                                const char* path, int high_order);
+
 /* Measurement helper: launches kernel `which` nrep times back-to-back on the
  * context's stream on its current data and reports the mean duration from
  * HIP events recorded on that stream. which: 0 A apply (k_mass, weighted),

@@ -133,6 +133,10 @@ class Context {
   }
   // nsteps bodies of EulerSolver::run's time loop (Atmospheric_Flow.cc:1115-1222).
   void imex_step(int nsteps = 1) { check(atmo_imex_step(ctx_, nsteps), ctx_); }
+  // EulerSolver::output_results (Atmospheric_Flow.cc:886-918): the current state as VTU
+  void write_vtu(const std::string& path, bool high_order = false) const {
+    check(atmo_write_vtu(ctx_, path.c_str(), high_order ? 1 : 0), ctx_);
+  }
 
  private:
   atmo_params params_;

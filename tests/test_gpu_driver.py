@@ -35,3 +35,32 @@ def test_driver_runs_example_and_matches_python_path():
     d = ctx.step_diagnostics()
     assert cfl_c[-1] == pytest.approx(d["cfl_c"], rel=1e-9)
     assert d["nonfinite"] == 0 and 0 < d["cfl_c"] < 100
+
+
+def test_driver_writes_vtu_like_output_results(tmp_path):
+    """With an existing saving directory the driver writes solution-NNNNN.vtu
+    and solution_high_order-NNNNN.vtu at step 0, every output_interval steps
+    and after the last step (Atmospheric_Flow.cc:1104, 1237-1256); the last
+    high-order file holds the state the Python path reaches in the same steps."""
+    from test_vtu import read_vtu
+    text = open(PRM).read()
+    out = tmp_path / "out"
+    out.mkdir()
+    text = re.sub(r"set saving directory = \S+", f"set saving directory = {out}", text)
+    text = re.sub(r"set output_interval = \d+", "set output_interval = 4", text)
+    prm = tmp_path / "run.prm"
+    prm.write_text(text)
+    r = subprocess.run([DRIVER, str(prm)], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-1000:], r.stderr[-1000:])
+    assert r.returncode == 0
+    names = sorted(os.listdir(out))
+    want = [f"solution{s}-{n:05d}.vtu" for n in (0, 4, 8, 10) for s in ("", "_high_order")]
+    assert names == sorted(want)
+    ctx = af.DeviceContext(2, 40, 20, h=af.witch_of_agnesi(40), Ma=1.0, Fr=0.31927542840705515, dt=5.0, eps=1e-10)
+    rho, u, p = ctx.init_hill(N_bv=0.01, U=10.0, g=9.81, seed=20240601)
+    ctx.upload_state(rho, u, p)
+    ctx.imex_step(10)
+    ctx.write_vtu(tmp_path / "py.vtu", high_order=True)
+    a, b = read_vtu(out / "solution_high_order-00010.vtu"), read_vtu(tmp_path / "py.vtu")
+    for key in ("points", "rho", "u", "p", "connectivity"):
+        assert np.allclose(a[key], b[key], rtol=1e-9, atol=0), key
