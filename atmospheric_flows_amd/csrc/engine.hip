@@ -736,23 +736,57 @@ class EngineK : public Engine {
     L.lmin[blk] = L.lmax[blk] / P.cheb_range;
   }
 
-  // The estimates of levels 1 .. finest, each on its own level's vectors: the finest on s_, the others
-  // round-robin on the extra streams, which start after the work queued on s_ so far and which s_ waits for.
+  // The estimates of levels 1 .. finest, each on its own level's vectors. Levels of at most
+  // EIG_SMALL_CELLS cells run together in one launch (a workgroup each, all iterations inside: about 90
+  // launches of tiny kernels per level otherwise); the others kernel by kernel, the finest on s_. With
+  // concurrent estimates the batch and the other levels go to the extra streams, which start after the
+  // work queued on s_ so far and which s_ waits for.
+  static constexpr int EIG_SMALL_CELLS = 512;
   void eig_enqueue_levels(int blk) {
     const int nl = (int)lv.size();
+    EigBatch B{};
+    B.n = 0;
+    std::vector<int> big;
+    for (int l = 1; l < nl; ++l) {
+      Level& L = *lv[l];
+      if (L.ncell <= EIG_SMALL_CELLS && B.n < EIG_BATCH_MAX) {
+        EigLevel& e = B.L[B.n++];
+        e.m = L.m;
+        e.rho = blk == BLK_A ? L.rho64.p : nullptr;
+        e.P = L.inv64[blk].p;
+        e.b = ncomp(blk) == 1 ? L.eigb1.p : L.eigb2.p;
+        e.g = L.x64.p;
+        e.d = L.b64.p;
+        e.h = L.t64.p;
+        e.x = L.d64.p;
+        e.hist = eig_hist.p + (size_t)2 * MAX_EIG_ITS * l;
+        e.st = eig_st.p + l;
+        e.Ns = L.Ns;
+      } else {
+        big.push_back(l);
+      }
+    }
+    const double eig_tol = 1e-2;  // absolute, on |g|
     if (!eig_concurrent) {
-      for (int l = 1; l < nl; ++l) eig_enqueue(l, blk, s_, rb);
+      launch_eig_cg_batch<K>(B, ncomp(blk), blk == BLK_A, coeff(blk), P.cheb_eig_its, eig_tol, s_);
+      for (int l : big) eig_enqueue(l, blk, s_, rb);
       return;
     }
     ATMO_HIP_CHECK(hipEventRecord(eig_fork, s_));
     for (int e = 0; e < EIG_STREAMS; ++e) ATMO_HIP_CHECK(hipStreamWaitEvent(eig_s[e], eig_fork, 0));
-    for (int l = 1; l < nl; ++l) {
-      if (l == nl - 1) eig_enqueue(l, blk, s_, rb);
-      else eig_enqueue(l, blk, eig_s[(l - 1) % EIG_STREAMS], eig_rb[(l - 1) % EIG_STREAMS]);
+    launch_eig_cg_batch<K>(B, ncomp(blk), blk == BLK_A, coeff(blk), P.cheb_eig_its, eig_tol, eig_s[0]);
+    int e = 1;
+    for (int l : big) {
+      if (l == nl - 1) {
+        eig_enqueue(l, blk, s_, rb);
+      } else {
+        eig_enqueue(l, blk, eig_s[e], eig_rb[e]);
+        e = e % (EIG_STREAMS - 1) + 1;
+      }
     }
-    for (int e = 0; e < EIG_STREAMS; ++e) {
-      ATMO_HIP_CHECK(hipEventRecord(eig_join[e], eig_s[e]));
-      ATMO_HIP_CHECK(hipStreamWaitEvent(s_, eig_join[e], 0));
+    for (int e2 = 0; e2 < EIG_STREAMS; ++e2) {
+      ATMO_HIP_CHECK(hipEventRecord(eig_join[e2], eig_s[e2]));
+      ATMO_HIP_CHECK(hipStreamWaitEvent(s_, eig_join[e2], 0));
     }
   }
 

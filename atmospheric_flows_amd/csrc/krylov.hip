@@ -435,12 +435,6 @@ void jcg_update(int i, long Ns, const MeshDev& m, int K, double* x, double* g, c
   else launch_upd<3>(false, i, Ns, m, x, g, dd, h, nullptr, P, st, rb, s);
 }
 
-__device__ inline void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // Apply kernel of iteration i: (re-reducing the previous update's partials)
 // convergence test on |g_{i-1}|, beta = gz_{i-1}/gz_{i-2}; then per cell
 // d = -P g + beta d, h = A d, partial d.h -> pa.

@@ -47,4 +47,32 @@ void launch_coarse_cg(const MeshDev& m, int ncomp, bool weighted, const double* 
                       double* x, double* g, double* d, double* h, const double* tol, int max_its, int* its, long Ns,
                       hipStream_t s);
 
+// Chebyshev eigenvalue estimates of several small levels in one launch: one
+// workgroup of 16 waves per level runs all nit CG iterations (Jacobi
+// preconditioner P, start vector b, absolute tolerance tol on |g|), with the
+// recurrence of the multi-kernel estimate: g = -b, h = P g, d = -h; per
+// iteration h = A d, alpha = (g.h)/(d.h) recorded in hist[2 it], x += alpha d,
+// g += alpha h, convergence test on |g|, then h = P g, beta = (g.h)_new/(g.h)
+// recorded in hist[2 it + 1], d = -h + beta d (the last beta is not formed);
+// nothing is recorded once the test has decided. st->it, st->done as the
+// multi-kernel estimate leaves them. Work vectors x, g, d, h of the level.
+constexpr int EIG_BATCH_MAX = 8;
+struct EigLevel {
+  MeshDev m;
+  const double* rho;  // level density (weighted operator) or null
+  const double* P;
+  const double* b;
+  double *x, *g, *d, *h;
+  double* hist;
+  CGState* st;
+  long Ns;
+};
+struct EigBatch {
+  EigLevel L[EIG_BATCH_MAX];
+  int n;
+};
+template <int K>
+void launch_eig_cg_batch(const EigBatch& B, int ncomp, bool weighted, double coeff, int nit, double tol,
+                         hipStream_t s);
+
 }  // namespace atmo

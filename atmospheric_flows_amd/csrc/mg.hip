@@ -443,6 +443,127 @@ void launch_coarse_cg(const MeshDev& m, int ncomp, bool weighted, const double* 
   ATMO_HIP_CHECK(hipGetLastError());
 }
 
+template <int K, int NC, bool W>
+__global__ __launch_bounds__(1024) void k_eig_cg(EigBatch B, double coeff, int nit, double tol) {
+  using C = Cfg<K>;
+  constexpr int ND2 = C::ND2, CPW = C::CPW, NW = 16;
+  __shared__ double s_u[NW][NC * CPW * ND2];
+  __shared__ double s_r[NW][W ? CPW * ND2 : 1];
+  __shared__ double s_red[NW][NC * CPW * C::ND * C::NQ];
+  __shared__ double s_sum[NW];
+  const EigLevel& E = B.L[blockIdx.x];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long n = (long)NC * E.Ns;
+  // block sum in a fixed order: each wave's shuffle tree, then the waves in order
+  auto bsum = [&](double v) {
+    v = wsum(v);
+    if (lane == 0) s_sum[w] = v;
+    __syncthreads();
+    double t = 0;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) t += s_sum[i];
+    __syncthreads();
+    return t;
+  };
+  // dst = A src on the level: the waves take interleaved passes of CPW cells, each in its own LDS
+  auto apply = [&](const double* src, double* dst) {
+    const int passes = (E.m.ncell + CPW - 1) / CPW;
+    for (int p = w; p < passes; p += NW) {
+      const int cell0 = p * CPW;
+      const long base = (long)cell0 * ND2, lim = (long)E.m.ncell * ND2;
+      for (int t = lane; t < CPW * ND2; t += 64) {
+        const long gi = base + t;
+#pragma unroll
+        for (int comp = 0; comp < NC; ++comp) s_u[w][comp * CPW * ND2 + t] = gi < lim ? src[comp * E.Ns + gi] : 0.0;
+        if (W) s_r[w][t] = gi < lim ? E.rho[gi] : 0.0;
+      }
+      wave_sync();
+      wave_mass_core<K, double, NC, W>(E.m, cell0, s_u[w], s_r[w], coeff, s_red[w]);
+      wave_sync();
+      double Si[C::NQ];
+      lane_cols_S<K, double>(lane % C::ND, Si);
+      for (int t = lane; t < CPW * ND2; t += 64) {
+        const long gi = base + t;
+        if (gi >= lim) break;
+        if (64 % C::ND != 0) lane_cols_S<K, double>(t % C::ND, Si);
+#pragma unroll
+        for (int comp = 0; comp < NC; ++comp)
+          dst[comp * E.Ns + gi] = wave_mass_out<K, double>(s_red[w] + comp * CPW * C::ND * C::NQ, t, Si);
+      }
+      wave_sync();
+    }
+  };
+  double part = 0;
+  for (long i = threadIdx.x; i < n; i += blockDim.x) {
+    const double gi = -E.b[i];
+    const double hi = gi * E.P[i];
+    E.g[i] = gi;
+    E.x[i] = 0.0;
+    E.h[i] = hi;
+    E.d[i] = -hi;
+    part += gi * hi;
+  }
+  double gh = bsum(part);
+  int it = 0, done = 0;
+  for (int k = 0; k < nit; ++k) {
+    __syncthreads();
+    apply(E.d, E.h);
+    __syncthreads();
+    part = 0;
+    for (long i = threadIdx.x; i < n; i += blockDim.x) part += E.d[i] * E.h[i];
+    const double dh = bsum(part);
+    if (!done) {
+      const double alpha = gh / dh;
+      if (threadIdx.x == 0) E.hist[2 * k] = alpha;
+      part = 0;
+      for (long i = threadIdx.x; i < n; i += blockDim.x) {
+        E.x[i] += alpha * E.d[i];
+        const double gi = E.g[i] + alpha * E.h[i];
+        E.g[i] = gi;
+        part += gi * gi;
+      }
+      const double res = sqrt(bsum(part));
+      ++it;
+      if (res <= tol) done = 1;
+      else if (it >= nit || isnan(res)) done = -1;
+    }
+    if (k + 1 == nit) break;
+    part = 0;
+    for (long i = threadIdx.x; i < n; i += blockDim.x) {
+      const double hi = E.g[i] * E.P[i];
+      E.h[i] = hi;
+      part += E.g[i] * hi;
+    }
+    const double ggh = bsum(part);
+    if (!done) {
+      const double beta = ggh / gh;
+      gh = ggh;
+      if (threadIdx.x == 0) E.hist[2 * k + 1] = beta;
+      for (long i = threadIdx.x; i < n; i += blockDim.x) E.d[i] = -E.h[i] + beta * E.d[i];
+    }
+  }
+  if (threadIdx.x == 0) {
+    E.st->it = it;
+    E.st->done = done;
+  }
+}
+
+template <int K>
+void launch_eig_cg_batch(const EigBatch& B, int ncomp, bool weighted, double coeff, int nit, double tol,
+                         hipStream_t s) {
+  if (B.n < 1) return;
+  if (B.n > EIG_BATCH_MAX) throw std::invalid_argument("eig_cg_batch: too many levels");
+  if (weighted) {
+    if (ncomp != 2) throw std::invalid_argument("weighted mass operator expects 2 components");
+    hipLaunchKernelGGL((k_eig_cg<K, 2, true>), dim3(B.n), dim3(1024), 0, s, B, coeff, nit, tol);
+  } else if (ncomp == 1) {
+    hipLaunchKernelGGL((k_eig_cg<K, 1, false>), dim3(B.n), dim3(1024), 0, s, B, coeff, nit, tol);
+  } else {
+    hipLaunchKernelGGL((k_eig_cg<K, 2, false>), dim3(B.n), dim3(1024), 0, s, B, coeff, nit, tol);
+  }
+  ATMO_HIP_CHECK(hipGetLastError());
+}
+
 #define ATMO_MG_INST(K)                                                                                          \
   template void launch_restrict<K, double>(const MeshDev&, const MeshDev&, const double*, double*, int, long, long, \
                                            hipStream_t);                                                        \
@@ -462,7 +583,8 @@ void launch_coarse_cg(const MeshDev& m, int ncomp, bool weighted, const double* 
                                                         const double*, const double*, double, const double*,         \
                                                         const double*, const ChebSteps&, long, long, hipStream_t);   \
   template void launch_coarse_cg<K>(const MeshDev&, int, bool, const double*, double, const double*, double*,     \
-                                    double*, double*, double*, const double*, int, int*, long, hipStream_t);
+                                    double*, double*, double*, const double*, int, int*, long, hipStream_t);       \
+  template void launch_eig_cg_batch<K>(const EigBatch&, int, bool, double, int, double, hipStream_t);
 
 ATMO_MG_INST(1)
 ATMO_MG_INST(2)

@@ -5,6 +5,13 @@
 
 namespace atmo {
 
+// Barrier of one wavefront: its LDS writes are visible to all its lanes after it.
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Per-lane table columns for a lane-dependent dof index m: Si[c] = S[c][m],
 // Di[c] = D[c][m] (constant-memory loads of the transposed tables).
 template <int K, typename T>
