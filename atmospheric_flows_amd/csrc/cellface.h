@@ -159,12 +159,14 @@ __device__ inline void cellface_columns(const MeshDev& m, const Ph& ph, const Fi
   LaneMap<K> L(m.ncell);
   // the cell's vertices and the lane's table rows are loaded before the staged fields, so the three share
   // one memory latency (an inactive lane reads cell 0's)
-  const CellVerts v = cell_verts(m, L.active ? L.cell : 0);
+  const int vcell = L.active ? L.cell : 0;
+  const VertsRaw vr = cell_verts_load(m, vcell);
   double Sc[ND], Dc[ND];
   lane_rows<K, double>(L.c, Sc, Dc);
   stage_fields<K, NF, double, double, CFSmem<K, Ph, AMR>::CS>(&sh.in[0][0], in.p, m.ncell);
   pre();
   __syncthreads();
+  const CellVerts v = cell_verts_form(m, vcell, vr);
   if (L.active) {
     const ColGeo<double> g = col_geo<K, double>(v, L.c);
     const int off = L.cw * CFSmem<K, Ph, AMR>::CS;

@@ -87,10 +87,35 @@ struct CellVerts {
   double dx;
 };
 
-__device__ inline CellVerts cell_verts(const MeshDev& m, int cell) {
-  CellVerts v;
+// The loaded words a cell's vertices are formed from: the six of cgeo on a
+// locally refined mesh, else the heights h[i], h[i+1] of its column. Kernels
+// load them before their other loads (cell_verts_load) and form the vertices
+// after issuing those (cell_verts_form), so the loads share one memory
+// latency instead of waiting for the heights first.
+struct VertsRaw {
+  double g[6];
+};
+
+__device__ inline VertsRaw cell_verts_load(const MeshDev& m, int cell) {
+  VertsRaw r;
   if (m.cgeo) {
     const double* g = m.cgeo + 6 * (long)cell;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) r.g[q] = g[q];
+  } else {
+    const int i = cell % m.nx;
+    r.g[0] = m.h[i];
+    r.g[1] = m.h[i + 1];
+#pragma unroll
+    for (int q = 2; q < 6; ++q) r.g[q] = 0.0;
+  }
+  return r;
+}
+
+__device__ inline CellVerts cell_verts_form(const MeshDev& m, int cell, const VertsRaw& r) {
+  CellVerts v;
+  if (m.cgeo) {
+    const double* g = r.g;
     v.dx = g[1];
     v.x[0] = v.x[2] = g[0];
     v.x[1] = v.x[3] = g[0] + g[1];
@@ -102,7 +127,7 @@ __device__ inline CellVerts cell_verts(const MeshDev& m, int cell) {
   }
   const int j = cell / m.nx, i = cell - j * m.nx;
   v.dx = m.dx;
-  const double hl = m.h[i], hr = m.h[i + 1];
+  const double hl = r.g[0], hr = r.g[1];
   const double fj = j * m.inv_nz, fj1 = (j + 1) * m.inv_nz;
   v.x[0] = i * m.dx;
   v.x[1] = (i + 1) * m.dx;
@@ -113,6 +138,10 @@ __device__ inline CellVerts cell_verts(const MeshDev& m, int cell) {
   v.z[2] = hl + (m.H - hl) * fj1;
   v.z[3] = hr + (m.H - hr) * fj1;
   return v;
+}
+
+__device__ inline CellVerts cell_verts(const MeshDev& m, int cell) {
+  return cell_verts_form(m, cell, cell_verts_load(m, cell));
 }
 
 // Bilinear geometry of the lane's quadrature column xi_c on a vertical cell

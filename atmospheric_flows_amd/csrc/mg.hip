@@ -124,6 +124,8 @@ __global__ __launch_bounds__(64) void k_cheb_sweep(MeshDev m, MeshDev mc, T* __r
   const int blk = cell_block();
   const long base = (long)blk * CPW * ND2;
   const long lim = (long)m.ncell * ND2;
+  // the lane's table row and cell vertex words, loaded with the staged fields
+  const MassLane<K, T> lane_pre = wave_mass_lane<K, T>(m, blk * CPW);
   // stage x (unless the guess is zero) and rho
   {
     const T* srcs[NS];
@@ -205,11 +207,10 @@ __global__ __launch_bounds__(64) void k_cheb_sweep(MeshDev m, MeshDev mc, T* __r
   const int cw = lane / NQ, c = lane - cw * NQ;
   const int cell = blk * CPW + cw;
   const bool col_active = cw < CPW && cell < m.ncell;
-  T Sc[ND], Dc[ND], wq[NQ];
-  if (col_active) {
-    lane_rows<K, T>(c, Sc, Dc);
-    column_weights<K, T, W>(m, cell, c, cw, &s_u[NS - 1][0], coeff, wq, Sc);
-  }
+  T Sc[ND], wq[NQ];
+#pragma unroll
+  for (int i = 0; i < ND; ++i) Sc[i] = lane_pre.Sc[i];
+  if (col_active) column_weights<K, T, W>(cell_verts_form(m, cell, lane_pre.vr), c, cw, &s_u[NS - 1][0], coeff, wq, Sc);
   T Si[NPASS][NQ];
 #pragma unroll
   for (int ps = 0; ps < NPASS; ++ps) lane_cols_S<K, T>((threadIdx.x + 64 * ps) % ND, Si[ps]);

@@ -47,9 +47,10 @@ __global__ __launch_bounds__(64) void k_mass(MeshDev m, const T* __restrict__ sr
 #pragma unroll
   for (int comp = 0; comp < NC; ++comp) srcs[comp] = src + comp * Ns;
   if constexpr (W) srcs[NS - 1] = rho;
+  const MassLane<K, T> pre = wave_mass_lane<K, T>(m, cell_block() * CPW);
   stage_fields<K, NS, T, T>(&s_u[0][0], srcs, m.ncell);
   __syncthreads();
-  wave_mass_core<K, T, NC, W>(m, cell_block() * CPW, &s_u[0][0], &s_u[NS - 1][0], coeff, &s_red[0][0][0][0]);
+  wave_mass_core<K, T, NC, W>(m, cell_block() * CPW, &s_u[0][0], &s_u[NS - 1][0], coeff, &s_red[0][0][0][0], &pre);
   __syncthreads();
   const long base = (long)cell_block() * CPW * ND2;
   const long lim = (long)m.ncell * ND2;
@@ -438,8 +439,7 @@ __device__ inline double wave_sum8(const double (&v)[8]) {
 // and w.w: slot k of block b at part[k*pstride + b], slot nv is w.w. The
 // Arnoldi step of GMRES: its Schur product and the classical Gram-Schmidt dot
 // products without a second pass over w. Slots go in chunks of 8 whose loads
-// are all in flight at once (addresses past nv are clamped to column 0 and
-// their values dropped).
+// are all in flight at once (slots past nv load nothing).
 template <int K, bool AMR>
 __global__ __launch_bounds__(64) void k_dmc_dots(MeshDev m, PhysDminusC ph, FieldPtrs<4> in, double* __restrict__ dst,
                                                  const double* __restrict__ scale, long Ns,
@@ -502,16 +502,21 @@ __global__ __launch_bounds__(64) void k_dmc_dots(MeshDev m, PhysDminusC ph, Fiel
   auto load = [&](int k0, Chunk& c) {
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
-      const long col = (long)(k0 + s < nv ? k0 + s : 0) * ldv;
+      if (k0 + s < nv) {  // uniform: slots past the basis load nothing
+        const long col = (long)(k0 + s) * ldv;
 #pragma unroll
-      for (int ps = 0; ps < NL; ++ps) c.y[s][ps] = __builtin_nontemporal_load(V + col + gi[ps]);
+        for (int ps = 0; ps < NL; ++ps) c.y[s][ps] = __builtin_nontemporal_load(V + col + gi[ps]);
+      } else {
+#pragma unroll
+        for (int ps = 0; ps < NL; ++ps) c.y[s][ps] = 0.0;
+      }
     }
     if constexpr (PAIRS) {
       const int s = l >> 3;
-      const long col = (long)(k0 + s < nv ? k0 + s : 0) * ldv;
-      {
+      c.t = make_double2(0.0, 0.0);
+      if (k0 + s < nv) {
         typedef double nt_d2 __attribute__((ext_vector_type(2)));
-        const nt_d2 tv = __builtin_nontemporal_load(reinterpret_cast<const nt_d2*>(V + col + gt));
+        const nt_d2 tv = __builtin_nontemporal_load(reinterpret_cast<const nt_d2*>(V + (long)(k0 + s) * ldv + gt));
         c.t = make_double2(tv.x, tv.y);
       }
     }

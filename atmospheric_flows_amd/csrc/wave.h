@@ -29,13 +29,12 @@ __device__ inline void lane_cols_S(int m, T (&Si)[Cfg<K>::NQ]) {
 }
 
 // Quadrature weights of the lane's column: wq[r] = JxW(c, r) * coeff, times
-// the interpolated density rho(c, r) when W (the A operator).
+// the interpolated density rho(c, r) when W (the A operator); v the cell's vertices.
 template <int K, typename T, bool W>
-__device__ inline void column_weights(const MeshDev& m, int cell, int c, int cw, const T* s_r, T coeff,
+__device__ inline void column_weights(const CellVerts& v, int c, int cw, const T* s_r, T coeff,
                                       T (&wq)[Cfg<K>::NQ], const T (&Sc)[Cfg<K>::ND]) {
   constexpr int ND = Cfg<K>::ND, NQ = Cfg<K>::NQ, ND2 = Cfg<K>::ND2;
   constexpr auto& tb = basis::kT<K>;
-  const CellVerts v = cell_verts(m, cell);
   const ColGeo<T> g = col_geo<K, T>(v, c);
   const T jc = g.dx * g.ze * g.wc * coeff;
 #pragma unroll
@@ -59,11 +58,35 @@ __device__ inline void column_weights(const MeshDev& m, int cell, int c, int cw,
   }
 }
 
+// What a lane of the mass-type apply loads from memory besides the staged
+// fields: its table row Sc and its cell's vertex words. Loaded before the
+// staging (wave_mass_lane), they share the staging's memory latency.
+template <int K, typename T>
+struct MassLane {
+  T Sc[Cfg<K>::ND];
+  VertsRaw vr;
+};
+
+template <int K, typename T>
+__device__ inline MassLane<K, T> wave_mass_lane(const MeshDev& m, int cell0) {
+  constexpr int NQ = Cfg<K>::NQ, CPW = Cfg<K>::CPW;
+  const int lane = threadIdx.x & 63;
+  const int cw = lane / NQ, c = lane - cw * NQ;
+  const int cell = cell0 + cw;
+  MassLane<K, T> p;
+  T Dc[Cfg<K>::ND];
+  lane_rows<K, T>(c, p.Sc, Dc);
+  p.vr = cell_verts_load(m, cw < CPW && cell < m.ncell ? cell : 0);
+  return p;
+}
+
 // Column phase of the mass-type apply for the CPW cells starting at cell0:
 // s_u (LDS, [NC][CPW*ND2]) -> s_red (LDS, [NC][CPW][ND][NQ]) holding
 // Tv[j](c) = sum_r S[r][j] w(c,r) u(c,r). Callers barrier before reading s_red.
+// pre: the lane's loads issued earlier by wave_mass_lane (else loaded here).
 template <int K, typename T, int NC, bool W>
-__device__ inline void wave_mass_core(const MeshDev& m, int cell0, const T* s_u, const T* s_r, T coeff, T* s_red) {
+__device__ inline void wave_mass_core(const MeshDev& m, int cell0, const T* s_u, const T* s_r, T coeff, T* s_red,
+                                      const MassLane<K, T>* pre = nullptr) {
   using C = Cfg<K>;
   constexpr int ND = C::ND, NQ = C::NQ, ND2 = C::ND2, CPW = C::CPW;
   constexpr auto& tb = basis::kT<K>;
@@ -71,10 +94,19 @@ __device__ inline void wave_mass_core(const MeshDev& m, int cell0, const T* s_u,
   const int cw = lane / NQ, c = lane - cw * NQ;
   const int cell = cell0 + cw;
   if (cw >= CPW || cell >= m.ncell) return;
-  T Sc[ND], Dc[ND];
-  lane_rows<K, T>(c, Sc, Dc);
+  T Sc[ND];
+  CellVerts v;
+  if (pre) {
+#pragma unroll
+    for (int i = 0; i < ND; ++i) Sc[i] = pre->Sc[i];
+    v = cell_verts_form(m, cell, pre->vr);
+  } else {
+    T Dc[ND];
+    lane_rows<K, T>(c, Sc, Dc);
+    v = cell_verts(m, cell);
+  }
   T wq[NQ];
-  column_weights<K, T, W>(m, cell, c, cw, s_r, coeff, wq, Sc);
+  column_weights<K, T, W>(v, c, cw, s_r, coeff, wq, Sc);
 #pragma unroll
   for (int comp = 0; comp < NC; ++comp) {
     const T* u = s_u + comp * CPW * ND2 + cw * ND2;
