@@ -110,13 +110,20 @@ struct PhysDminusC {
 // mesh): the fluxes of all four faces, per half of a split face, go through
 // LDS with the face type of each cell; otherwise the bottom/top fluxes are
 // folded into the column partials and only left/right go through LDS.
+// Column partials: plane o holds output o's value/eta partials; an output
+// with an x-gradient (bit o of Ph::GX) also has an xi plane, NO + (its rank
+// among those outputs), so outputs without one take no LDS for it.
 template <int K, class Ph, bool AMR>
 struct CFSmem {
   static constexpr int ND = K + 1, NQ = 2 * K + 1, ND2 = ND * ND, CPW = 64 / NQ;
   static constexpr int NFACE = AMR ? 4 : 2, NHALF = AMR ? 2 : 1;
   static constexpr int CS = lds_cell_stride<K>();
+  static constexpr int popc(unsigned x) { return x ? (int)(x & 1u) + popc(x >> 1) : 0; }
+  static constexpr int NPLANE = Ph::NO + popc(Ph::GX & ((1u << Ph::NO) - 1u));
+  // plane of output o's value/eta (d = 0) or xi (d = 1) partials
+  static constexpr int plane(int o, int d) { return d == 0 ? o : Ph::NO + popc(Ph::GX & ((1u << o) - 1u)); }
   double in[Ph::NF][CPW * CS];               // staged inputs, cell w at w*CS
-  double T[Ph::NO][2][CPW][ND][NQ];          // column partials of the value/eta and xi test functions
+  double T[NPLANE][CPW][ND][NQ];             // column partials of the value/eta and xi test functions
   double F[Ph::NO][NFACE][NHALF][CPW][NQ];   // face fluxes times the face weight
   int ft[AMR ? CPW : 1][4];                  // AMR: 0 no flux, 1 full face, 2 split face
 };
@@ -399,8 +406,8 @@ __device__ inline void cellface_columns(const MeshDev& m, const Ph& ph, const Fi
     for (int o = 0; o < NO; ++o)
 #pragma unroll
       for (int j = 0; j < ND; ++j) {
-        sh.T[o][0][L.cw][j][L.c] = Tv[o][j];
-        if ((Ph::GX >> o) & 1) sh.T[o][1][L.cw][j][L.c] = Td[o][j];
+        sh.T[CFSmem<K, Ph, AMR>::plane(o, 0)][L.cw][j][L.c] = Tv[o][j];
+        if ((Ph::GX >> o) & 1) sh.T[CFSmem<K, Ph, AMR>::plane(o, 1)][L.cw][j][L.c] = Td[o][j];
       }
   }
   __syncthreads();
@@ -441,8 +448,8 @@ __device__ inline double cellface_out(const CFSmem<K, Ph, AMR>& sh, int o, int t
   double acc = 0;
 #pragma unroll
   for (int c = 0; c < NQ; ++c) {
-    acc += R.Si[c] * sh.T[o][0][cw][jj][c];
-    if (gx) acc += R.Di[c] * sh.T[o][1][cw][jj][c];
+    acc += R.Si[c] * sh.T[CFSmem<K, Ph, AMR>::plane(o, 0)][cw][jj][c];
+    if (gx) acc += R.Di[c] * sh.T[CFSmem<K, Ph, AMR>::plane(o, 1)][cw][jj][c];
   }
   if constexpr (!AMR) {
     if (ii == 0 || ii == K) {

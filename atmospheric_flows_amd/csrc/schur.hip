@@ -10,7 +10,8 @@
 namespace atmo {
 
 // The packed inverse blocks of nc consecutive cells, loaded coalesced into
-// registers (load) and written to LDS (store: cell w at w * LDS_STRIDE).
+// registers (load) and written to LDS (store: cell w at w * LDS_STRIDE; the
+// cells [c0, c1) only, cell w at (w - c0) * LDS_STRIDE).
 template <int K, int NT, int MAXC>
 struct SymBlockStage {
   using SB = SymBlock<K>;
@@ -24,60 +25,95 @@ struct SymBlockStage {
       v[it] = e < lim ? Ab[e] : 0.0;
     }
   }
-  __device__ void store(double* __restrict__ lds, int nc) const {
-    const int lim = nc * SB::SIZE;
+  __device__ void store(double* __restrict__ lds, int nc) const { store_cells(lds, 0, nc); }
+  __device__ void store_cells(double* __restrict__ lds, int c0, int c1) const {
 #pragma unroll
     for (int it = 0; it < NPASS; ++it) {
       const int e = threadIdx.x + it * NT;
-      if (e < lim) {
-        const int w = e / SB::SIZE;
-        lds[e + w * (SB::LDS_STRIDE - SB::SIZE)] = v[it];
-      }
+      const int w = e / SB::SIZE;
+      if (w >= c0 && w < c1) lds[e - c0 * SB::SIZE + (w - c0) * (SB::LDS_STRIDE - SB::SIZE)] = v[it];
     }
   }
 };
 
+// y = A^{-1} B p for the block's cells. The inverse blocks of the first half
+// of the cells are loaded while B p is formed, those of the second half
+// right after; B p waits in registers until the cell+face partials are
+// dead, and it and the inverse blocks of one half at a time take their LDS
+// (one 9-cell block's cell+face LDS, not that plus the 9 inverse blocks, so
+// more blocks fit a CU).
 template <int K, bool AMR>
-__global__ __launch_bounds__(64) void k_b_ainv(MeshDev m, PhysB ph, const double* __restrict__ p,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_b_ainv(MeshDev m, PhysB ph, const double* __restrict__ p,
                                                const double* __restrict__ Ainv, double* __restrict__ y, long Ns) {
   constexpr int ND2 = Cfg<K>::ND2, CPW = Cfg<K>::CPW, CS = lds_cell_stride<K>();
+  constexpr int NPASS = (CPW * ND2 + 63) / 64, HC = CPW / 2;  // cells of the first half
+  constexpr int HMAX = CPW - HC > HC ? CPW - HC : HC;
   using SB = SymBlock<K>;
-  // the inverse blocks take the place of the cell+face partials once t = B p is formed
   __shared__ union {
     CFSmem<K, PhysB, AMR> cf;
-    double A[CPW * SB::LDS_STRIDE];
+    struct {
+      double t[2][CPW * CS];        // B p
+      double A[HMAX * SB::LDS_STRIDE];  // the inverse blocks of one half
+    } ap;
   } sh;
-  __shared__ double s_t[2][CPW * CS];
   FieldPtrs<1> in{{p}};
   const int cell0 = cell_block() * CPW;
   const int nc = min(CPW, m.ncell - cell0);
-  // the inverse blocks are loaded while B p is formed
-  SymBlockStage<K, 64, CPW> blk;
-  cellface_columns<K, PhysB, AMR>(m, ph, in, sh.cf, [&]() { blk.load(Ainv + (long)cell0 * SB::SIZE, nc); });
+  // the first half's inverse blocks are loaded while B p is formed, the second half's after it
+  SymBlockStage<K, 64, HC> blk0;
+  SymBlockStage<K, 64, CPW - HC> blk1;
+  cellface_columns<K, PhysB, AMR>(m, ph, in, sh.cf,
+                                  [&]() { blk0.load(Ainv + (long)cell0 * SB::SIZE, min(HC, nc)); });
+  blk1.load(Ainv + (long)(cell0 + HC) * SB::SIZE, max(nc - HC, 0));
   OutRows<K> R = out_rows<K>(threadIdx.x);
-  for (int t = threadIdx.x; t < CPW * ND2; t += 64) {
-    if (!rows_invariant<K>()) R = out_rows<K>(t);
-    const int st = (t / ND2) * CS + t % ND2;
-    s_t[0][st] = cellface_out<K, PhysB, AMR>(sh.cf, 0, t, R);
-    s_t[1][st] = cellface_out<K, PhysB, AMR>(sh.cf, 1, t, R);
+  double t0[NPASS], t1[NPASS];
+#pragma unroll
+  for (int ps = 0; ps < NPASS; ++ps) {
+    const int t = threadIdx.x + 64 * ps;
+    t0[ps] = t1[ps] = 0.0;
+    if (t < CPW * ND2) {
+      if (!rows_invariant<K>()) R = out_rows<K>(t);
+      t0[ps] = cellface_out<K, PhysB, AMR>(sh.cf, 0, t, R);
+      t1[ps] = cellface_out<K, PhysB, AMR>(sh.cf, 1, t, R);
+    }
   }
   __syncthreads();
-  blk.store(sh.A, nc);
-  __syncthreads();
-  for (int t = threadIdx.x; t < CPW * ND2; t += 64) {
-    const int cw = t / ND2, a = t - cw * ND2;
-    if (cw >= nc) break;
-    const double* Ac = sh.A + cw * SB::LDS_STRIDE;
-    double y0 = 0.0, y1 = 0.0;
 #pragma unroll
-    for (int b = 0; b < ND2; ++b) {
-      const double v = Ac[SB::index(a, b)];
-      y0 += v * s_t[0][cw * CS + b];
-      y1 += v * s_t[1][cw * CS + b];
+  for (int ps = 0; ps < NPASS; ++ps) {
+    const int t = threadIdx.x + 64 * ps;
+    if (t < CPW * ND2) {
+      const int st = (t / ND2) * CS + t % ND2;
+      sh.ap.t[0][st] = t0[ps];
+      sh.ap.t[1][st] = t1[ps];
     }
-    const long gi = (long)(cell0 + cw) * ND2 + a;
-    y[gi] = y0;
-    y[Ns + gi] = y1;
+  }
+#pragma unroll
+  for (int half = 0; half < 2; ++half) {
+    const int c0 = half ? HC : 0, c1 = half ? nc : min(HC, nc);
+    if (half) {
+      __syncthreads();  // the first half's products have read their blocks
+      blk1.store_cells(sh.ap.A, 0, c1 - c0);
+    } else {
+      blk0.store_cells(sh.ap.A, 0, c1);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ps = 0; ps < NPASS; ++ps) {
+      const int t = threadIdx.x + 64 * ps;
+      const int cw = t / ND2, a = t - cw * ND2;
+      if (t >= CPW * ND2 || cw < c0 || cw >= c1) continue;
+      const double* Ac = sh.ap.A + (cw - c0) * SB::LDS_STRIDE;
+      double y0 = 0.0, y1 = 0.0;
+#pragma unroll
+      for (int b = 0; b < ND2; ++b) {
+        const double v = Ac[SB::index(a, b)];
+        y0 += v * sh.ap.t[0][cw * CS + b];
+        y1 += v * sh.ap.t[1][cw * CS + b];
+      }
+      const long gi = (long)(cell0 + cw) * ND2 + a;
+      y[gi] = y0;
+      y[Ns + gi] = y1;
+    }
   }
 }
 
