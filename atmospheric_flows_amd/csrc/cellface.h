@@ -122,8 +122,15 @@ struct CFSmem {
   static constexpr int NPLANE = Ph::NO + popc(Ph::GX & ((1u << Ph::NO) - 1u));
   // plane of output o's value/eta (d = 0) or xi (d = 1) partials
   static constexpr int plane(int o, int d) { return d == 0 ? o : Ph::NO + popc(Ph::GX & ((1u << o) - 1u)); }
-  double in[Ph::NF][CPW * CS];               // staged inputs, cell w at w*CS
-  double T[NPLANE][CPW][ND][NQ];             // column partials of the value/eta and xi test functions
+  // operators of several fields on the structured mesh also stage the cells before and after the block's
+  // (their left/right neighbour traces then come from LDS)
+  static constexpr int HALO = (!AMR && Ph::NF > 1) ? 1 : 0;
+  // the staged inputs are dead once the column phase has read them; the partials written at its end reuse
+  // their space (a one-wave block: its LDS reads are issued before those writes)
+  union {
+    double in[Ph::NF][(CPW + 2 * HALO) * CS];  // staged inputs, block cell w at (w + HALO)*CS
+    double T[NPLANE][CPW][ND][NQ];             // column partials of the value/eta and xi test functions
+  };
   double F[Ph::NO][NFACE][NHALF][CPW][NQ];   // face fluxes times the face weight
   int ft[AMR ? CPW : 1][4];                  // AMR: 0 no flux, 1 full face, 2 split face
 };
@@ -170,13 +177,14 @@ __device__ inline void cellface_columns(const MeshDev& m, const Ph& ph, const Fi
   const VertsRaw vr = cell_verts_load(m, vcell);
   double Sc[ND], Dc[ND];
   lane_rows<K, double>(L.c, Sc, Dc);
-  stage_fields<K, NF, double, double, CFSmem<K, Ph, AMR>::CS>(&sh.in[0][0], in.p, m.ncell);
+  constexpr int HALO = CFSmem<K, Ph, AMR>::HALO, CS = CFSmem<K, Ph, AMR>::CS;
+  stage_fields<K, NF, double, double, CS, HALO>(&sh.in[0][0], in.p, m.ncell);
   pre();
   __syncthreads();
   const CellVerts v = cell_verts_form(m, vcell, vr);
   if (L.active) {
     const ColGeo<double> g = col_geo<K, double>(v, L.c);
-    const int off = L.cw * CFSmem<K, Ph, AMR>::CS;
+    const int off = (L.cw + HALO) * CS;
     double tv[NF][ND];
 #pragma unroll
     for (int f = 0; f < NF; ++f)
@@ -327,15 +335,22 @@ __device__ inline void cellface_columns(const MeshDev& m, const Ph& ph, const Fi
 #pragma unroll
         for (int q = 0; q < NF; ++q) own[q] = face_trace<K>(&sh.in[q][off], f, Sc);
         int ni = side ? ci + 1 : ci - 1;
-        bool inner = true;
+        bool inner = true, staged = HALO == 1;
         if (ni < 0 || ni >= m.nx) {
+          staged = false;
           if (m.periodic) ni = (ni + m.nx) % m.nx;
           else inner = false;
         }
         if (inner) {
-          const long nbase = (long)(cj * m.nx + ni) * ND2;
+          if (staged) {
+            // the neighbour is the cell before or after this one, staged next to it
 #pragma unroll
-          for (int q = 0; q < NF; ++q) nb[q] = face_trace<K>(in.p[q] + nbase, f ^ 1, Sc);
+            for (int q = 0; q < NF; ++q) nb[q] = face_trace<K>(&sh.in[q][off + (side ? CS : -CS)], f ^ 1, Sc);
+          } else {
+            const long nbase = (long)(cj * m.nx + ni) * ND2;
+#pragma unroll
+            for (int q = 0; q < NF; ++q) nb[q] = face_trace<K>(in.p[q] + nbase, f ^ 1, Sc);
+          }
           ph.face(own, nb, fg.n0, fg.n1, F);
         } else if (Ph::BDRY) {
           ph.bdry(own, fg.n0, fg.n1, F);

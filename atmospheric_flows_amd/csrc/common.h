@@ -222,11 +222,15 @@ struct LaneMap {
 // ND2 pads the cells apart in the LDS banks). The passes have a fixed trip
 // count and every load of a group of up to four fields is issued before the
 // group's first LDS write, so the group costs one memory latency.
-template <int K, int NF, typename T, typename S, int STRIDE = Cfg<K>::ND2>
+// HALO = 1: the cells just before and after the block's are staged too (at
+// w = 0 and CPW + 1, the block's cells at w + 1; cells outside the mesh read
+// as zeros).
+template <int K, int NF, typename T, typename S, int STRIDE = Cfg<K>::ND2, int HALO = 0>
 __device__ inline void stage_fields(T* __restrict__ lds, const S* const (&src)[NF], int ncell) {
-  constexpr int ND2 = Cfg<K>::ND2, N = Cfg<K>::CPW * ND2, NPASS = (N + 63) / 64, FS = Cfg<K>::CPW * STRIDE;
+  constexpr int ND2 = Cfg<K>::ND2, NC = Cfg<K>::CPW + 2 * HALO, N = NC * ND2, NPASS = (N + 63) / 64;
+  constexpr int FS = NC * STRIDE;
   constexpr int G = 4;
-  const long base = (long)cell_block() * N;
+  const long base = (long)cell_block() * Cfg<K>::CPW * ND2 - HALO * ND2;
   const long lim = (long)ncell * ND2;
 #pragma unroll
   for (int f0 = 0; f0 < NF; f0 += G) {
@@ -237,7 +241,7 @@ __device__ inline void stage_fields(T* __restrict__ lds, const S* const (&src)[N
       for (int it = 0; it < NPASS; ++it) {
         const int t = threadIdx.x + it * 64;
         const long g = base + t;
-        v[f][it] = (f0 + f < NF && t < N && g < lim) ? (T)src[f0 + f < NF ? f0 + f : 0][g] : (T)0;
+        v[f][it] = (f0 + f < NF && t < N && g >= 0 && g < lim) ? (T)src[f0 + f < NF ? f0 + f : 0][g] : (T)0;
       }
 #pragma unroll
     for (int f = 0; f < G; ++f)
