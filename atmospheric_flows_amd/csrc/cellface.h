@@ -122,9 +122,9 @@ struct CFSmem {
   static constexpr int NPLANE = Ph::NO + popc(Ph::GX & ((1u << Ph::NO) - 1u));
   // plane of output o's value/eta (d = 0) or xi (d = 1) partials
   static constexpr int plane(int o, int d) { return d == 0 ? o : Ph::NO + popc(Ph::GX & ((1u << o) - 1u)); }
-  // operators of several fields on the structured mesh also stage the cells before and after the block's
-  // (their left/right neighbour traces then come from LDS)
-  static constexpr int HALO = (!AMR && Ph::NF > 1) ? 1 : 0;
+  // operators on the structured mesh also stage the cells before and after the block's (their left/right
+  // neighbour traces then come from LDS)
+  static constexpr int HALO = !AMR ? 1 : 0;
   // the staged inputs are dead once the column phase has read them; the partials written at its end reuse
   // their space (a one-wave block: its LDS reads are issued before those writes)
   union {
@@ -242,7 +242,7 @@ __device__ inline void cellface_columns(const MeshDev& m, const Ph& ph, const Fi
       const int cj = L.cell / m.nx, ci = L.cell - cj * m.nx;
       // neighbours of the faces f = 0 left i-1, 1 right i+1, 2 bottom j-1, 3 top j+1; a wall side reads its
       // own cell instead (the value is not used), so the loads need no branch
-      bool inner[4];
+      bool inner[4], staged[2];
       long nbase[4];
 #pragma unroll
       for (int side = 0; side < 2; ++side) {
@@ -251,7 +251,9 @@ __device__ inline void cellface_columns(const MeshDev& m, const Ph& ph, const Fi
         nbase[2 + side] = (long)((inner[2 + side] ? nj : cj) * m.nx + ci) * ND2;
         int ni = side ? ci + 1 : ci - 1;
         bool in_x = true;
+        staged[side] = true;  // the cell before or after this one, staged next to it
         if (ni < 0 || ni >= m.nx) {
+          staged[side] = false;
           if (m.periodic) ni = (ni + m.nx) % m.nx;
           else in_x = false;
         }
@@ -268,8 +270,13 @@ __device__ inline void cellface_columns(const MeshDev& m, const Ph& ph, const Fi
 #pragma unroll
         for (int i = 0; i < FG; ++i) {
           const int f = (g0 + i + 2) & 3;
+          if (f < 2 && staged[f & 1]) {
 #pragma unroll
-          for (int q = 0; q < NF; ++q) nb[i][q] = face_trace<K>(in.p[q] + nbase[f], f ^ 1, Sc);
+            for (int q = 0; q < NF; ++q) nb[i][q] = face_trace<K>(&sh.in[q][off + (f ? CS : -CS)], f ^ 1, Sc);
+          } else {
+#pragma unroll
+            for (int q = 0; q < NF; ++q) nb[i][q] = face_trace<K>(in.p[q] + nbase[f], f ^ 1, Sc);
+          }
         }
 #pragma unroll
         for (int i = 0; i < FG; ++i) {
