@@ -441,7 +441,7 @@ __device__ inline double wave_sum8(const double (&v)[8]) {
 // products without a second pass over w. Slots go in chunks of 8 whose loads
 // are all in flight at once (slots past nv load nothing).
 template <int K, bool AMR>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_dmc_dots(MeshDev m, PhysDminusC ph, FieldPtrs<4> in, double* __restrict__ dst,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(AMR ? 1 : 4))) void k_dmc_dots(MeshDev m, PhysDminusC ph, FieldPtrs<4> in, double* __restrict__ dst,
                                                  const double* __restrict__ scale, long Ns,
                                                  const double* __restrict__ V, long ldv, int nv,
                                                  double* __restrict__ part, long pstride) {

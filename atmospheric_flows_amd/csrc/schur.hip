@@ -43,7 +43,7 @@ struct SymBlockStage {
 // (one 9-cell block's cell+face LDS, not that plus the 9 inverse blocks, so
 // more blocks fit a CU).
 template <int K, bool AMR>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_b_ainv(MeshDev m, PhysB ph, const double* __restrict__ p,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(AMR ? 1 : 4))) void k_b_ainv(MeshDev m, PhysB ph, const double* __restrict__ p,
                                                const double* __restrict__ Ainv, double* __restrict__ y, long Ns) {
   constexpr int ND2 = Cfg<K>::ND2, CPW = Cfg<K>::CPW, CS = lds_cell_stride<K>();
   constexpr int NPASS = (CPW * ND2 + 63) / 64, HC = CPW / 2;  // cells of the first half
