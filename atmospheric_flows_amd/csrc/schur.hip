@@ -38,10 +38,10 @@ struct SymBlockStage {
 
 // y = A^{-1} B p for the block's cells. The inverse blocks of the first half
 // of the cells are loaded while B p is formed, those of the second half
-// right after; B p waits in registers until the cell+face partials are
-// dead, and it and the inverse blocks of one half at a time take their LDS
-// (one 9-cell block's cell+face LDS, not that plus the 9 inverse blocks, so
-// more blocks fit a CU).
+// once it is (in flight during the first half's products); B p waits in
+// registers until the cell+face partials are dead, and it and the inverse
+// blocks of one half at a time take their LDS (one 9-cell block's cell+face
+// LDS, not that plus the 9 inverse blocks, so more blocks fit a CU).
 template <int K, bool AMR>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(AMR ? 1 : 4))) void k_b_ainv(MeshDev m, PhysB ph, const double* __restrict__ p,
                                                const double* __restrict__ Ainv, double* __restrict__ y, long Ns) {
@@ -59,12 +59,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(AMR ? 1 : 4)
   FieldPtrs<1> in{{p}};
   const int cell0 = cell_block() * CPW;
   const int nc = min(CPW, m.ncell - cell0);
-  // the first half's inverse blocks are loaded while B p is formed, the second half's after it
+  // the first half's inverse blocks are loaded with the column phase, the second half's after B p
   SymBlockStage<K, 64, HC> blk0;
   SymBlockStage<K, 64, CPW - HC> blk1;
   cellface_columns<K, PhysB, AMR>(m, ph, in, sh.cf,
                                   [&]() { blk0.load(Ainv + (long)cell0 * SB::SIZE, min(HC, nc)); });
-  blk1.load(Ainv + (long)(cell0 + HC) * SB::SIZE, max(nc - HC, 0));
   OutRows<K> R = out_rows<K>(threadIdx.x);
   double t0[NPASS], t1[NPASS];
 #pragma unroll
@@ -77,6 +76,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(AMR ? 1 : 4)
       t1[ps] = cellface_out<K, PhysB, AMR>(sh.cf, 1, t, R);
     }
   }
+  blk1.load(Ainv + (long)(cell0 + HC) * SB::SIZE, max(nc - HC, 0));
   __syncthreads();
 #pragma unroll
   for (int ps = 0; ps < NPASS; ++ps) {
