@@ -110,8 +110,10 @@ void launch_project_down(const MeshDev& mf, const MeshDev& mc, const double* fin
 // the first-step update, the fused operator-and-update step, the DG
 // embedding prolongation and the residual of the two-kernel forms, so the
 // values are those of those kernels.
+// The FP32 sweeps are compiled for 5 waves per SIMD (they fit 102 VGPRs
+// without spills; the compiler's default choice, 118, allowed 4).
 template <int K, typename T, int NC, bool W, bool ZG, bool PRO, bool RES>
-__global__ __launch_bounds__(64) void k_cheb_sweep(MeshDev m, MeshDev mc, T* __restrict__ x, const T* __restrict__ xc,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 5 : 1))) void k_cheb_sweep(MeshDev m, MeshDev mc, T* __restrict__ x, const T* __restrict__ xc,
                                                    const T* __restrict__ rho, T coeff, const T* __restrict__ b,
                                                    const T* __restrict__ P, ChebSteps cs, T* __restrict__ res,
                                                    long Ns, long Nc) {
