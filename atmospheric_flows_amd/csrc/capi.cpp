@@ -103,7 +103,7 @@ static void check_params(const atmo_params& p) {
   if (p.cheb_eig_its > MAX_EIG_ITS) throw std::invalid_argument("params: cheb_eig_its must be <= 256");
   if (p.gmres_restart < 1 || p.gmres_restart > MAX_GMRES - 2)
     throw std::invalid_argument("params: gmres_restart must be in [1, 62]");
-  if (p.precision != 0 && p.precision != 1) throw std::invalid_argument("params: precision must be 0 or 1");
+  if (p.precision < 0 || p.precision > 2) throw std::invalid_argument("params: precision must be 0, 1 or 2");
   if (!(p.eps >= 0) || !(p.schur_inner_tol >= 0) || !(p.a_solve_tol >= 0))
     throw std::invalid_argument("params: tolerances must be >= 0");
   if (p.mg_min_cells < 1) throw std::invalid_argument("params: mg_min_cells must be >= 1");

@@ -29,7 +29,9 @@ struct EngineParams {
   double cheb_range = 15.0;        // :491
   int cheb_eig_its = 10;           // :493
   int gmres_restart = 28;          // deal.II SolverGMRES default (30 temporary vectors)
-  int precision = 0;               // 0: FP64 ; 1: mixed (FP32 MG levels, cellwise-direct A^{-1})
+  // 0: FP64 with the reference's solvers; 1: mixed (FP32 MG levels, cellwise-direct FP64 A^{-1});
+  // 2: the mixed form's FP64 comparator (FP64 MG levels, the same cellwise-direct A^{-1})
+  int precision = 0;
 };
 
 // One cell of a locally refined mesh: base cell (bi, bj), quadtree level l,

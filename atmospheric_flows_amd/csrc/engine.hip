@@ -182,7 +182,7 @@ class EngineK : public Engine {
     pres_fix.zero(s_);
     for (auto* b : {&cg_g, &cg_d, &cg_h, &s_t1, &s_t2, &in_g, &in_d, &in_h}) b->alloc(2 * Ns);
     s_t3.alloc(Ns);
-    if (P.precision == 1) ainv.alloc((size_t)fine().ncell * SymBlock<K>::SIZE);
+    if (direct_ainv()) ainv.alloc((size_t)fine().ncell * SymBlock<K>::SIZE);
     gm_V.alloc((size_t)(P.gmres_restart + 2) * gm_ld());
     gm_p.alloc(Ns);
     gm_coef.alloc(MAX_GMRES + 2);
@@ -251,7 +251,12 @@ class EngineK : public Engine {
     oc_ = consts();
     if (rebuild) gm_V.alloc((size_t)(P.gmres_restart + 2) * gm_ld());
     if (prec_change) {
-      if (P.precision == 1 && !ainv.p) ainv.alloc((size_t)fine().ncell * SymBlock<K>::SIZE);
+      if (direct_ainv() && !ainv.p) ainv.alloc((size_t)fine().ncell * SymBlock<K>::SIZE);
+      // the FP32 copies of the geometry-only diagonals exist only once a form with FP32 levels ran
+      for (auto& L : lv) {
+        level_diag(*L, BLK_RHO);
+        level_diag(*L, BLK_D);
+      }
       refresh_levels_rho();
     }
   }
@@ -378,7 +383,7 @@ class EngineK : public Engine {
           break;
         }
         case 8:  // mixed Schur: B and the cellwise A^{-1} in one pass
-          if (!ainv.p) throw std::invalid_argument("time_kernel 8 needs the mixed form (precision 1)");
+          if (!ainv.p) throw std::invalid_argument("time_kernel 8 needs a cellwise-direct A^{-1} (precision 1 or 2)");
           launch_b_ainv<K>(m, oc_, bc_coeff(), p_old.p, ainv.p, in_h.p, Ns, s_);
           // p, y (2 components), and the packed inverse block per cell
           fields = 3 + (double)SymBlock<K>::SIZE / ((K + 1) * (K + 1));
@@ -535,6 +540,9 @@ class EngineK : public Engine {
 
   Level& fine() { return *lv.back(); }
   const Level& fine() const { return *lv.back(); }
+  // precision 1 (mixed) and 2 (its FP64 comparator) invert A cell by cell; only 1 runs the multigrid in FP32
+  bool direct_ainv() const { return P.precision == 1 || P.precision == 2; }
+  bool mg32() const { return P.precision == 1; }
   static int block_of(int ns) { return (ns == 1 || ns == 4) ? BLK_RHO : ((ns == 3 || ns == 5) ? BLK_A : BLK_D); }
   static int ncomp(int blk) { return blk == BLK_A ? 2 : 1; }
   double coeff(int blk) const { return blk == BLK_D ? 1.0 / (P.gamma_gas - 1.0) : 1.0; }
@@ -670,7 +678,7 @@ class EngineK : public Engine {
   void level_diag(Level& L, int blk) {
     const bool w = blk == BLK_A;
     launch_inv_diag<K, double>(L.m, ncomp(blk), w, L.rho64.p, L.inv64[blk].p, coeff(blk), L.Ns, s_);
-    if (P.precision == 1) vec_cast<float, double>((long)ncomp(blk) * L.Ns, L.inv64[blk].p, L.inv32[blk].p, s_);
+    if (mg32()) vec_cast<float, double>((long)ncomp(blk) * L.Ns, L.inv64[blk].p, L.inv32[blk].p, s_);
   }
 
   void refresh_levels_rho() {
@@ -678,11 +686,11 @@ class EngineK : public Engine {
     vec_copy(F.Ns, rho_fix.p, F.rho64.p, s_);
     for (int l = (int)lv.size() - 1; l > 0; --l) launch_project_down<K>(lv[l]->m, lv[l - 1]->m, lv[l]->rho64.p, lv[l - 1]->rho64.p, s_);
     for (auto& L : lv) {
-      if (P.precision == 1) vec_cast<float, double>(L->Ns, L->rho64.p, L->rho32.p, s_);
+      if (mg32()) vec_cast<float, double>(L->Ns, L->rho64.p, L->rho32.p, s_);
       level_diag(*L, BLK_A);
     }
     estimate_eig_levels(BLK_A);
-    if (P.precision == 1) build_block_inverse<K>(F.m, rho_fix.p, ainv.p, s_);
+    if (direct_ainv()) build_block_inverse<K>(F.m, rho_fix.p, ainv.p, s_);
   }
 
   // Chebyshev eigenvalue estimate: <= cheb_eig_its CG iterations with the
@@ -891,7 +899,7 @@ class EngineK : public Engine {
   void mg_prec(int blk, const double* src, double* dst, const double* tol, int max_its) {
     ++st_.mg_cycles;
     const int top = (int)lv.size() - 1;
-    if (P.precision == 1) {
+    if (mg32()) {
       Level& F = fine();
       const long n = (long)ncomp(blk) * F.Ns;
       vec_cast<float, double>(n, src, F.b32.p, s_);
@@ -986,7 +994,7 @@ class EngineK : public Engine {
   void schur_velocity(const double* src) {
     ++st_.schur_matvecs;
     const long Ns = fine().Ns;
-    if (P.precision == 1) {
+    if (direct_ainv()) {
       launch_b_ainv<K>(fine().m, oc_, bc_coeff(), src, ainv.p, s_t2.p, Ns, s_);
     } else {
       launch_B<K>(fine().m, oc_, bc_coeff(), src, s_t1.p, Ns, s_);
@@ -999,7 +1007,7 @@ class EngineK : public Engine {
   // form); cellwise-direct with the FP64 cell inverses (mixed form). The
   // velocity updates at the outer tolerance eps keep CG + MG in both forms.
   void solve_A_tight(double* x, const double* b) {
-    if (P.precision == 1) launch_ainv_apply<K>(fine().m, ainv.p, b, x, fine().Ns, s_);
+    if (direct_ainv()) launch_ainv_apply<K>(fine().m, ainv.p, b, x, fine().Ns, s_);
     else pcg_mg(BLK_A, x, b, P.a_solve_tol, P.max_its);
   }
 
@@ -1278,7 +1286,7 @@ size_t engine_device_bytes(int k, const MeshDesc& mesh, const EngineParams& p) {
   b += ENGINE_EIG_STREAMS * RED_MAX_BLOCKS * D;               // their reduction partials on the extra streams
   b += 2 * Ns * D;                        // rho_fix, pres_fix
   b += 8 * 2 * Ns * D + Ns * D;           // CG / Schur / inner work
-  if (p.precision == 1) b += (size_t)mesh.nx * mesh.nz * nd2 * (nd2 + 1) / 2 * D;  // packed FP64 cell-block inverses of A
+  if (p.precision >= 1) b += (size_t)mesh.nx * mesh.nz * nd2 * (nd2 + 1) / 2 * D;  // packed FP64 cell-block inverses of A
   b += (size_t)(p.gmres_restart + 2) * (Ns + 32) * D + Ns * D + (MAX_GMRES + 2) * D;  // basis: leading dim N_s + 32
   b += (size_t)MAX_GMRES * GM_REC * D + (MAX_GMRES + 2) * D;  // GMRES records and basis scales
   b += (size_t)RED_MAX_SLOTS * ((mesh.nx * mesh.nz + 64 / (2 * k + 1) - 1) / (64 / (2 * k + 1))) * D;  // Arnoldi partials

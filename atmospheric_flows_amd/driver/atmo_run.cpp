@@ -39,7 +39,7 @@ int run(const atmo::DataStorage& d) {
   p.dt = d.dt;
   p.eps = d.eps;
   p.max_iterations = d.max_iterations;
-  p.precision = d.precision == "mixed" ? 1 : 0;
+  p.precision = d.precision == "mixed" ? 1 : (d.precision == "fp64_direct" ? 2 : 0);
   atmo::Context ctx(mesh, p);
   std::printf("Number of cells %d, scalar DoFs per field %ld, MG levels %d, precision %s\n", ctx.n_cells(),
               ctx.scalar_dofs(), ctx.n_levels(), d.precision.c_str());

@@ -127,7 +127,7 @@ inline void DataStorage::read_data(const std::string& filename) {
       {{"Device", "hill_center"}, {dbl(hill_center, any)}},
       {{"Device", "precision"},
        {[this](const std::string& t) {
-         if (t != "fp64" && t != "mixed") return false;
+         if (t != "fp64" && t != "mixed" && t != "fp64_direct") return false;
          precision = t;
          return true;
        }}},

@@ -31,7 +31,7 @@ _ENTRIES = {
     ("Device", "hill_height"): ("hill_height", float, 400.0, lambda v: v >= 0),
     ("Device", "hill_half_width"): ("hill_half_width", float, 1000.0, lambda v: v > 0),
     ("Device", "hill_center"): ("hill_center", float, 20000.0, lambda v: True),
-    ("Device", "precision"): ("precision", str, "fp64", lambda v: v in ("fp64", "mixed")),
+    ("Device", "precision"): ("precision", str, "fp64", lambda v: v in ("fp64", "mixed", "fp64_direct")),
     ("Device", "gravity"): ("gravity", float, 9.81, lambda v: v > 0),
     ("Device", "wind"): ("wind", float, 10.0, lambda v: True),
     ("Device", "brunt_vaisala"): ("brunt_vaisala", float, 0.01, lambda v: v > 0),

@@ -1,13 +1,14 @@
 """Headline benchmark: IMEX-DG steps on the 512x256 Q3 terrain-following hill
 mesh (8,388,608 DoF, dt = 1 s), mixed form (FP32 multigrid inside FP64
-Krylov, FP64 cellwise-direct velocity-block inverse) beside the plain FP64
-GPU path with the reference's solvers, on one MI355X.
+Krylov, FP64 cellwise-direct velocity-block inverse) beside two FP64 GPU
+forms on one MI355X: its precision-only comparator (the same cellwise-direct
+inverse, FP64 multigrid) and the plain FP64 path with the reference's solvers.
 
 Prints one JSON line. `value` is the mixed-precision form's DoF-steps/s over
 exactly --steps timed steps (inputs resident in HBM, device-synchronized
-bracket); the FP64 form runs the same steps from the same state and its
-throughput, the speed-up and the relative L2 error between the two are
-reported beside it. `roofline` is the dominant kernel of the mixed form
+bracket); the FP64 forms run the same steps from the same state and their
+throughput, the speed-ups and the relative L2 errors are reported beside it,
+with `accuracy` stating whether each error meets the 1e-8 target. `roofline` is the dominant kernel of the mixed form
 (GMRES's Arnoldi step: the scaled D p - C y with the dot products of its
 result against 14 basis vectors) timed with HIP events on the engine's stream;
 `cpu_baseline` is the CPU FP64 oracle on a bounded sample (see `sample`).
@@ -46,16 +47,24 @@ def run_gpu(args):
     states = {}
     forms = [("mixed", dict(precision=1))]
     if not args.mixed_only:
+        # the precision-only comparator: the mixed form's algorithm with FP64 multigrid levels
+        forms.append(("fp64_direct", dict(precision=2)))
         forms.append(("fp64", dict(precision=0)))
-    if not args.no_tight and not args.mixed_only:
+    if args.tight and not args.mixed_only:
         # the FP64 reference form solved to convergence: the yardstick both forms are measured against
         forms.append(("tight", dict(precision=0, eps=1e-13, a_solve_tol=1e-14, schur_inner_tol=1e-14)))
         # the FP64 form at the same outer tolerances with its inner A solves (1e-12 in the reference) run to
         # 1e-14: separates the effect of the reference's inner-solve truncation from that of the mixed form
         forms.append(("fp64_inner", dict(precision=0, a_solve_tol=1e-14, schur_inner_tol=1e-14)))
+    if not args.mixed_only:
+        # the comparator again from the initial density perturbed by a few ulps (relative 1e-15): how far two
+        # FP64 runs of one algorithm drift apart at these solver tolerances when only rounding differs
+        forms.append(("fp64_direct_ulp", dict(precision=2)))
     for name, prm in forms:
         ctx = af.DeviceContext(k, nx, nz, h=h, Ma=1.0, Fr=fr, dt=W["dt"], **{"eps": 1e-10, **prm})
         rho, u, p = ctx.init_hill(seed=20240601)
+        if name == "fp64_direct_ulp":
+            rho = rho * (1.0 + 1e-15 * np.random.default_rng(7).uniform(-1.0, 1.0, rho.shape))
         ctx.upload_state(rho, u, p)
         t0 = time.perf_counter()
         ctx.imex_step(args.warmup)
@@ -82,14 +91,30 @@ def run_gpu(args):
     def rel_errs(a, b):
         return {nm: float(np.linalg.norm(x - y) / np.linalg.norm(y)) for nm, x, y in zip(("rho", "u", "p"), a, b)}
 
+    if "fp64_direct" in states:
+        # one step of the mixed form and of its comparator from the same state (the comparator's last one):
+        # the error the levels' precision adds per step
+        one = {}
+        for name, prec in (("mixed", 1), ("fp64_direct", 2)):
+            ctx = af.DeviceContext(k, nx, nz, h=h, Ma=1.0, Fr=fr, dt=W["dt"], eps=1e-10, precision=prec)
+            ctx.upload_state(*states["fp64_direct"])
+            ctx.imex_step(1)
+            one[name] = ctx.download_state()
+            del ctx
+        res["rel_err_one_step_mixed_vs_fp64_direct"] = rel_errs(one["mixed"], one["fp64_direct"])
+        res["rel_err_fp64_direct_vs_ulp_perturbed"] = rel_errs(states["fp64_direct_ulp"], states["fp64_direct"])
+
     if "fp64" in states:
         res["rel_err_mixed_vs_fp64"] = rel_errs(states["mixed"], states["fp64"])
+    if "fp64_direct" in states:
+        res["rel_err_mixed_vs_fp64_direct"] = rel_errs(states["mixed"], states["fp64_direct"])
+        res["rel_err_fp64_direct_vs_fp64"] = rel_errs(states["fp64_direct"], states["fp64"])
     if "tight" in states:
-        res["rel_err_vs_converged_fp64"] = {"mixed": rel_errs(states["mixed"], states["tight"]),
-                                            "fp64": rel_errs(states["fp64"], states["tight"])}
+        res["rel_err_vs_converged_fp64"] = {f: rel_errs(states[f], states["tight"])
+                                            for f in ("mixed", "fp64_direct", "fp64")}
     if "fp64_inner" in states:
-        res["rel_err_vs_fp64_exact_inner"] = {"mixed": rel_errs(states["mixed"], states["fp64_inner"]),
-                                              "fp64": rel_errs(states["fp64"], states["fp64_inner"])}
+        res["rel_err_vs_fp64_exact_inner"] = {f: rel_errs(states[f], states["fp64_inner"])
+                                              for f in ("mixed", "fp64_direct", "fp64")}
     res["dofs"] = dof_count(nx, nz, k)
     res["nx"], res["nz"] = nx, nz
     return res
@@ -222,8 +247,8 @@ def run_amr(args):
     out = dict(workload="hill_amr_40x20_3levels_Q2_dt0.5", cells=int(len(leaves)),
                cells_per_level=np.bincount(leaves[:, 2]).tolist(), steps=args.amr_steps)
     states = {}
-    forms = [("mixed", dict(precision=1)), ("fp64", dict(precision=0))]
-    if not args.no_tight:
+    forms = [("mixed", dict(precision=1)), ("fp64_direct", dict(precision=2)), ("fp64", dict(precision=0))]
+    if args.tight:
         forms.append(("tight", dict(precision=0, eps=1e-13, a_solve_tol=1e-14, schur_inner_tol=1e-14)))
         forms.append(("fp64_inner", dict(precision=0, a_solve_tol=1e-14, schur_inner_tol=1e-14)))
     for name, prm in forms:
@@ -248,11 +273,12 @@ def run_amr(args):
         return {nm: float(np.linalg.norm(a - b) / np.linalg.norm(b)) for nm, a, b in zip(("rho", "u", "p"), x, y)}
 
     out["rel_err_mixed_vs_fp64"] = rel_errs(states["mixed"], states["fp64"])
+    out["rel_err_mixed_vs_fp64_direct"] = rel_errs(states["mixed"], states["fp64_direct"])
     if "tight" in states:
-        out["rel_err_vs_converged_fp64"] = {"mixed": rel_errs(states["mixed"], states["tight"]),
-                                            "fp64": rel_errs(states["fp64"], states["tight"])}
-        out["rel_err_vs_fp64_exact_inner"] = {"mixed": rel_errs(states["mixed"], states["fp64_inner"]),
-                                              "fp64": rel_errs(states["fp64"], states["fp64_inner"])}
+        out["rel_err_vs_converged_fp64"] = {f: rel_errs(states[f], states["tight"])
+                                            for f in ("mixed", "fp64_direct", "fp64")}
+        out["rel_err_vs_fp64_exact_inner"] = {f: rel_errs(states[f], states["fp64_inner"])
+                                              for f in ("mixed", "fp64_direct", "fp64")}
         del out["tight"], out["fp64_inner"]
     return out
 
@@ -287,28 +313,62 @@ def run_serve(args, precision):
     print(f"[bench] serve ({'mixed' if precision else 'fp64'}): {len(arrivals)} requests", file=sys.stderr, flush=True)
     wall = ens.run()
     res = [ens.result(i) for i in range(len(arrivals))]
-    out = dict(requests=len(arrivals), rate_per_s=args.serve_rate, seconds=args.serve_seconds, wall_s=wall,
+    out = dict(requests=len(arrivals), rate_per_s=args.serve_rate, seconds=args.serve_seconds,
+               scaled_from="Poisson 20 req/s for 300 s (about 6,000 requests: hours of work for one GPU), "
+                           f"run at {args.serve_rate:g} req/s for {args.serve_seconds:g} s with "
+                           f"{args.serve_min_steps}-{args.serve_max_steps} steps per request", wall_s=wall,
                dof_steps_per_s=dofs / wall, failed=sum(r["status"] != 0 for r in res))
     out.update(af.latency_summary(res))
     return out
 
 
-def cpu_baseline(threads):
-    """Oracle (C++ FP64, OpenMP) on a bounded sample of the same workload:
-    one IMEX step on 128x64 Q3 at dt = 4 s, the same acoustic CFL (~13) and
-    degree as 512x256 Q3 at dt = 1 s, i.e. the same Krylov/MG work per DoF on
-    1/16 of the cells."""
+def host_cpu():
+    """Logical CPUs the host shows and its CPU model."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return os.cpu_count() or 1, model
+
+
+def cpu_run(threads, nx, nz, k, dt, steps, warmup=0):
+    """The oracle (C++ FP64, OpenMP over cells) on the hill flow: `warmup`
+    untimed steps, then `steps` timed ones; returns DoF-steps/s and seconds."""
     from oracle.pyoracle import Oracle, lib, witch_of_agnesi
     lib().orc_set_threads(threads)
-    nx, nz, k, dt = 128, 64, 3, 4.0
     o = Oracle(k, nx, nz, hx=witch_of_agnesi(nx))
-    o.set_params(dt=dt, eps=1e-10)
+    o.set_params(Ma=1.0, Fr=1.0 / np.sqrt(9.81), dt=dt, eps=1e-10)
     rho, u, p = o.init_hill(seed=20240601)
+    if warmup:
+        rho, u, p, _ = o.run(rho, u, p, warmup)
     t0 = time.perf_counter()
-    o.run(rho, u, p, 1)
+    o.run(rho, u, p, steps)
     el = time.perf_counter() - t0
-    return dict(value=dof_count(nx, nz, k) / el, unit="DoF-steps/s", cores=lib().orc_get_threads(), kind="port",
-                sample=f"1 IMEX step of 128x64 Q3 at dt=4 s (same CFL as 512x256 dt=1 s), {el:.1f} s wall")
+    return dof_count(nx, nz, k) * steps / el, el, lib().orc_get_threads()
+
+
+def cpu_baseline(threads):
+    """Reported CPU baseline: the oracle on the whole 40x20 Q2 hill case
+    (dt = 5 s, all 100 IMEX steps, GMRES 1e-10), the reference's
+    CPU-runnable configuration, on `threads` host threads."""
+    value, el, used = cpu_run(threads, 40, 20, 2, 5.0, 100)
+    nproc, model = host_cpu()
+    return dict(value=value, unit="DoF-steps/s", cores=used, kind="port", host_nproc=nproc, cpu_model=model,
+                sample=f"all 100 IMEX steps of the 40x20 Q2 hill case at dt=5 s (28,800 DoF), {el:.1f} s wall")
+
+
+def cpu_headline(threads, steps):
+    """The oracle on the headline workload itself (512x256 Q3, dt = 1 s):
+    `steps` timed steps after one warm-up step (several minutes of CPU)."""
+    value, el, used = cpu_run(threads, 512, 256, 3, 1.0, steps, warmup=1)
+    nproc, model = host_cpu()
+    return dict(value=value, unit="DoF-steps/s", cores=used, kind="port", host_nproc=nproc, cpu_model=model,
+                sample=f"{steps} IMEX steps of 512x256 Q3 at dt=1 s after 1 warm-up step, {el:.1f} s wall")
 
 
 def main():
@@ -324,20 +384,25 @@ def main():
     ap.add_argument("--only", default="")
     ap.add_argument("--no-traffic", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
-    ap.add_argument("--no-tight", action="store_true")
+    ap.add_argument("--tight", action="store_true",
+                    help="also run the FP64 form solved to convergence and with exact inner solves (yardsticks)")
     ap.add_argument("--no-amr", action="store_true")
     ap.add_argument("--mixed-only", action="store_true",
                     help="profiling runs: the mixed form's time loop only (no FP64 form, no kernel table, no roofline)")
-    ap.add_argument("--amr-steps", type=int, default=50)
-    ap.add_argument("--serve", action="store_true", help="also run the Poisson-arrival serving stream")
+    ap.add_argument("--amr-steps", type=int, default=200)
+    ap.add_argument("--no-serve", action="store_true", help="skip the Poisson-arrival serving stream")
     ap.add_argument("--serve-rate", type=float, default=2.0)
     ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--serve-min-steps", type=int, default=10)
     ap.add_argument("--serve-max-steps", type=int, default=40)
     ap.add_argument("--ens-members", type=int, default=64)
-    ap.add_argument("--ens-steps", type=int, default=5)
+    ap.add_argument("--ens-steps", type=int, default=100)
     ap.add_argument("--ens-slots", type=int, default=8)
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    # one GPU's share of the host is 16 threads (OMP_NUM_THREADS on the GPU hosts)
+    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", 0)) or
+                    min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-headline-steps", type=int, default=0,
+                    help="only the CPU oracle on the headline workload for this many steps (no GPU)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
@@ -346,9 +411,12 @@ def main():
     if args.kernels_only:
         kernels_only(args)
         return
+    if args.cpu_headline_steps:
+        print(json.dumps({"cpu_headline": cpu_headline(args.cpu_threads, args.cpu_headline_steps)}))
+        return
     traffic = None
     if args.mixed_only:
-        args.no_traffic = args.no_ensemble = args.no_amr = args.no_cpu = True
+        args.no_traffic = args.no_ensemble = args.no_amr = args.no_cpu = args.no_serve = True
     if not args.no_traffic:
         # counter passes run in child processes before this process touches the GPU
         nx, nz = (args.nx or WORKLOAD["nx"]), (args.nz or WORKLOAD["nz"])
@@ -358,6 +426,24 @@ def main():
     dofs = r["dofs"]
     mixed = dofs * args.steps / r["mixed"]["seconds"]
     fp64 = dofs * args.steps / r["fp64"]["seconds"] if "fp64" in r else None
+    direct = dofs * args.steps / r["fp64_direct"]["seconds"] if "fp64_direct" in r else None
+    acc = None
+    if "rel_err_mixed_vs_fp64_direct" in r:
+        e_dir = max(r["rel_err_mixed_vs_fp64_direct"].values())
+        e_ref = max(r["rel_err_mixed_vs_fp64"].values())
+        e_one = max(r["rel_err_one_step_mixed_vs_fp64_direct"].values())
+        e_ulp = max(r["rel_err_fp64_direct_vs_ulp_perturbed"].values())
+        acc = dict(target_rel_l2=1e-8, steps_compared=args.warmup + args.steps,
+                   mixed_vs_fp64_direct_max=e_dir, met_vs_fp64_direct=e_dir <= 1e-8,
+                   mixed_vs_fp64_max=e_ref, met_vs_fp64=e_ref <= 1e-8,
+                   one_step_mixed_vs_fp64_direct_max=e_one, met_one_step=e_one <= 1e-8,
+                   fp64_direct_vs_ulp_perturbed_max=e_ulp,
+                   note="fp64_direct differs from the mixed form only in the multigrid levels' precision; over "
+                        "many steps both drift from each other as far as the FP64 comparator drifts from itself "
+                        "when its initial density is perturbed by 1e-15 (fp64_direct_vs_ulp_perturbed): at "
+                        "eps = 1e-10 the Krylov and fixed-point stopping decisions fork on rounding-level "
+                        "differences, so the many-step figure does not measure precision; the one-step figure "
+                        "does")
     roof = None
     if "kernels" in r:
         dom = r["kernels"][DOMINANT[1]]
@@ -366,7 +452,7 @@ def main():
                     ms_per_launch=dom["ms"], traffic_detail=traffic)
     ensemble = None if args.no_ensemble else run_ensemble(args)
     amr = None if args.no_amr else run_amr(args)
-    serve = {"mixed": run_serve(args, 1), "fp64": run_serve(args, 0)} if args.serve else None
+    serve = None if args.no_serve else {"mixed": run_serve(args, 1), "fp64": run_serve(args, 0)}
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_threads)
     out = {
         "metric": "DoF-steps/s mixed vs FP64 GPU at <=1e-8 rel err; p50/p99 ensemble step latency",
@@ -377,10 +463,16 @@ def main():
                    "nx": r["nx"], "nz": r["nz"], "degree": 3, "dt": 1.0, "dofs": dofs, "gmres_rel_tol": 1e-10,
                    "parallelism": "single_gpu"},
         "fp64_value": fp64, "speedup_mixed_vs_fp64": mixed / fp64 if fp64 else None,
+        "fp64_direct_value": direct, "speedup_mixed_vs_fp64_direct": mixed / direct if direct else None,
+        "accuracy": acc,
         "rel_err_mixed_vs_fp64": r.get("rel_err_mixed_vs_fp64"),
+        "rel_err_mixed_vs_fp64_direct": r.get("rel_err_mixed_vs_fp64_direct"),
+        "rel_err_fp64_direct_vs_fp64": r.get("rel_err_fp64_direct_vs_fp64"),
+        "rel_err_one_step_mixed_vs_fp64_direct": r.get("rel_err_one_step_mixed_vs_fp64_direct"),
+        "rel_err_fp64_direct_vs_ulp_perturbed": r.get("rel_err_fp64_direct_vs_ulp_perturbed"),
         "rel_err_vs_converged_fp64": r.get("rel_err_vs_converged_fp64"),
         "rel_err_vs_fp64_exact_inner": r.get("rel_err_vs_fp64_exact_inner"),
-        "solver_stats": {f: r[f]["stats"] for f in ("mixed", "fp64") if f in r},
+        "solver_stats": {f: r[f]["stats"] for f in ("mixed", "fp64_direct", "fp64") if f in r},
         "converged_fp64_seconds": r["tight"]["seconds"] if "tight" in r else None,
         "kernels": r.get("kernels"), "roofline": roof, "cpu_baseline": cpu, "ensemble": ensemble,
         "serve": serve, "amr": amr,

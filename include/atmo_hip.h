@@ -77,7 +77,9 @@ typedef struct {
   int cheb_eig_its;            /* Atmospheric_Flow.cc:493 (10) */
   int gmres_restart;           /* deal.II SolverGMRES default (28) */
   int precision;               /* 0: FP64, the reference's solvers; 1: mixed form: FP32 multigrid inside
-                                  FP64 Krylov, FP64 cellwise-direct A^{-1} (inner Schur and A solves) */
+                                  FP64 Krylov, FP64 cellwise-direct A^{-1} (inner Schur and A solves);
+                                  2: the mixed form's FP64 comparator: the same cellwise-direct A^{-1}
+                                  with FP64 multigrid (differs from 1 only in the levels' precision) */
 } atmo_params;
 
 typedef struct {
