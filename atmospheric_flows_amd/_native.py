@@ -84,6 +84,10 @@ SIGNATURES = [
     ("atmo_vec_upload", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, _dp, ctypes.c_size_t]),
     ("atmo_vec_download", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, _dp, ctypes.c_size_t]),
     ("atmo_vec_device_ptr", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
+    ("atmo_vec_axpby", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_double,
+                                      ctypes.c_void_p]),
+    ("atmo_vec_dot", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, _dp]),
+    ("atmo_vec_linf", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, _dp]),
     ("atmo_set_stages", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     ("atmo_set_rho_for_fixed", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ("atmo_set_pres_fixed", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),

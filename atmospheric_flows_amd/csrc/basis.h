@@ -194,49 +194,10 @@ constexpr Transfer<K> make_transfer() {
   return tr;
 }
 
-// Lagrange values of FE_DGQ(k) at the points of QGauss<1>(k+1), the rule of
-// the reference's per-step diagnostics (quadrature_{density,velocity,
-// temperature}(degree+1), Atmospheric_Flow.cc:230-232).
-template <int K>
-struct Eval {
-  static constexpr int ND = K + 1;
-  double x[ND];
-  double S[ND][ND];
-};
-
-template <int K>
-constexpr Eval<K> make_eval() {
-  constexpr int ND = K + 1;
-  constexpr Tables<K> t = make_tables<K>();
-  Eval<K> e{};
-  for (int i = 0; i < ND; ++i) {
-    double z = ccos(kPi * (i + 0.75) / (ND + 0.5));
-    for (int it = 0; it < 100; ++it) {
-      double p = 0, pm1 = 0;
-      legendre(ND, z, p, pm1);
-      const double dp = ND * (z * p - pm1) / (z * z - 1.0);
-      const double dz = p / dp;
-      z -= dz;
-      if (cabs(dz) < 1e-17) break;
-    }
-    e.x[i] = (1.0 - z) / 2.0;
-  }
-  for (int q = 0; q < ND; ++q)
-    for (int i = 0; i < ND; ++i) {
-      double v = 1.0;
-      for (int m = 0; m < ND; ++m)
-        if (m != i) v *= (e.x[q] - t.gll[m]) / (t.gll[i] - t.gll[m]);
-      e.S[q][i] = v;
-    }
-  return e;
-}
-
 template <int K>
 inline constexpr Tables<K> kT = make_tables<K>();
 template <int K>
 inline constexpr Transfer<K> kTr = make_transfer<K>();
-template <int K>
-inline constexpr Eval<K> kE = make_eval<K>();
 
 }  // namespace basis
 }  // namespace atmo

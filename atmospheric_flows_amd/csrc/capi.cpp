@@ -3,6 +3,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <new>
 #include <string>
@@ -67,6 +68,9 @@ static atmo_status guarded(atmo_ctx* c, F f) {
     return ATMO_DEVICE;
   }
 }
+
+// the same for context-free entry points defined in other files (errors read with atmo_last_error(NULL))
+atmo_status atmo_guarded_noctx(const std::function<atmo_status()>& f) { return guarded(nullptr, f); }
 
 static EngineParams to_engine(const atmo_params& p) {
   EngineParams e;
@@ -159,6 +163,10 @@ static MeshDesc to_mesh(const atmo_mesh_desc* mesh) {
   if (!(mesh->Lx > 0) || !(mesh->H > 0)) throw std::invalid_argument("mesh: Lx, H must be > 0");
   for (int i = 0; i <= mesh->nx; ++i)
     if (!std::isfinite(mesh->h[i]) || mesh->h[i] >= mesh->H) throw std::invalid_argument("mesh: bad terrain height");
+  // x-periodic: the seam's two sides are one face, so its end columns must coincide (deal.II's
+  // collect_periodic_faces rejects unmatched faces); otherwise the seam flux is not conservative
+  if (mesh->periodic && std::fabs(mesh->h[0] - mesh->h[mesh->nx]) > 1e-12 * mesh->H)
+    throw std::invalid_argument("mesh: periodic mesh needs h[0] == h[nx]");
   MeshDesc m;
   m.nx = mesh->nx;
   m.nz = mesh->nz;
@@ -323,6 +331,32 @@ atmo_status atmo_vec_device_ptr(atmo_vec* v, double** dev) {
   return ATMO_OK;
 }
 
+atmo_status atmo_vec_axpby(atmo_ctx* ctx, double a, const atmo_vec* x, double b, atmo_vec* y) {
+  return guarded(ctx, [&]() {
+    if (!ctx || !x || !y) throw std::invalid_argument("null argument");
+    if (x->n != y->n) throw std::invalid_argument("axpby: vectors of different blocks");
+    ctx->eng->axpby((long)y->n, a, x->p, b, y->p);
+    return ATMO_OK;
+  });
+}
+
+atmo_status atmo_vec_dot(atmo_ctx* ctx, const atmo_vec* x, const atmo_vec* y, double* result) {
+  return guarded(ctx, [&]() {
+    if (!ctx || !x || !y || !result) throw std::invalid_argument("null argument");
+    if (x->n != y->n) throw std::invalid_argument("dot: vectors of different blocks");
+    *result = ctx->eng->dot((long)x->n, x->p, y->p);
+    return ATMO_OK;
+  });
+}
+
+atmo_status atmo_vec_linf(atmo_ctx* ctx, const atmo_vec* x, double* result) {
+  return guarded(ctx, [&]() {
+    if (!ctx || !x || !result) throw std::invalid_argument("null argument");
+    *result = ctx->eng->linf((long)x->n, x->p);
+    return ATMO_OK;
+  });
+}
+
 atmo_status atmo_set_stages(atmo_ctx* ctx, int hyperbolic_stage, int ns_stage) {
   return guarded(ctx, [&]() {
     if (!ctx) throw std::invalid_argument("null context");
@@ -484,8 +518,9 @@ atmo_status atmo_state_upload(atmo_ctx* ctx, const double* rho, const double* u,
     if (!ctx || !rho || !u || !p) throw std::invalid_argument("null argument");
     const long Ns = ctx->eng->ns();
     for (long i = 0; i < Ns; ++i)
-      if (!std::isfinite(rho[i]) || !std::isfinite(p[i]) || !(rho[i] > 0))
-        throw std::runtime_error("state_upload: non-finite or non-positive density/pressure");
+      if (!std::isfinite(rho[i]) || !std::isfinite(p[i]) || !(rho[i] > 0) || !std::isfinite(u[i]) ||
+          !std::isfinite(u[Ns + i]))
+        throw std::runtime_error("state_upload: non-finite state or non-positive density");
     hipStream_t s = ctx->eng->stream();
     ATMO_HIP_CHECK(hipMemcpyAsync(ctx->eng->state_ptr(0), rho, Ns * sizeof(double), hipMemcpyHostToDevice, s));
     ATMO_HIP_CHECK(hipMemcpyAsync(ctx->eng->state_ptr(1), u, 2 * Ns * sizeof(double), hipMemcpyHostToDevice, s));
@@ -550,16 +585,8 @@ atmo_status atmo_init_hill_state(const atmo_mesh_desc* mesh, double theta0, doub
                                  double* u, double* p) {
   return guarded(nullptr, [&]() {
     if (!mesh || !mesh->h || !rho || !u || !p) throw std::invalid_argument("null argument");
-    if (mesh->degree < 1 || mesh->degree > 3 || mesh->nx < 1 || mesh->nz < 1)
-      throw std::invalid_argument("bad mesh description");
+    const MeshDesc m = to_mesh(mesh);
     if (!(N > 0) || !(theta0 > 0) || !(p0 > 0) || !(g > 0)) throw std::invalid_argument("bad background parameters");
-    MeshDesc m;
-    m.nx = mesh->nx;
-    m.nz = mesh->nz;
-    m.periodic = mesh->periodic;
-    m.Lx = mesh->Lx;
-    m.H = mesh->H;
-    m.h.assign(mesh->h, mesh->h + mesh->nx + 1);
     hill_initial_state(mesh->degree, m, theta0, p0, N, U, g, R, gamma_gas, perturbation, seed, rho, u, p);
     return ATMO_OK;
   });

@@ -126,7 +126,7 @@ struct CFSmem {
   // neighbour traces then come from LDS)
   static constexpr int HALO = !AMR ? 1 : 0;
   // the staged inputs are dead once the column phase has read them; the partials written at its end reuse
-  // their space (a one-wave block: its LDS reads are issued before those writes)
+  // their space (after a barrier that follows the last read)
   union {
     double in[Ph::NF][(CPW + 2 * HALO) * CS];  // staged inputs, block cell w at (w + HALO)*CS
     double T[NPLANE][CPW][ND][NQ];             // column partials of the value/eta and xi test functions
@@ -182,6 +182,11 @@ __device__ inline void cellface_columns(const MeshDev& m, const Ph& ph, const Fi
   pre();
   __syncthreads();
   const CellVerts v = cell_verts_form(m, vcell, vr);
+  double Tv[Ph::NO][ND], Td[Ph::NO][ND];
+#pragma unroll
+  for (int o = 0; o < NO; ++o)
+#pragma unroll
+    for (int j = 0; j < ND; ++j) Tv[o][j] = Td[o][j] = 0;
   if (L.active) {
     const ColGeo<double> g = col_geo<K, double>(v, L.c);
     const int off = (L.cw + HALO) * CS;
@@ -195,11 +200,6 @@ __device__ inline void cellface_columns(const MeshDev& m, const Ph& ph, const Fi
         for (int i = 0; i < ND; ++i) s += Sc[i] * sh.in[f][off + j * ND + i];
         tv[f][j] = s;
       }
-    double Tv[NO][ND], Td[NO][ND];
-#pragma unroll
-    for (int o = 0; o < NO; ++o)
-#pragma unroll
-      for (int j = 0; j < ND; ++j) Tv[o][j] = Td[o][j] = 0;
     const double wdx = g.wc * g.dx, wze = g.wc * g.ze, jc = wdx * g.ze;
     // Rolled point loop, table rows read from constant memory (scalar loads
     // per row): unrolled, every FP64 table entry becomes an SGPR-pair
@@ -424,6 +424,11 @@ __device__ inline void cellface_columns(const MeshDev& m, const Ph& ph, const Fi
         if (L.c == 0) sh.ft[L.cw][f] = type == FACE_FINER ? 2 : 1;
       }
     }
+  }
+  // the column partials reuse the staged inputs' LDS: every lane's last read of the inputs (own and
+  // neighbour face traces) is ordered before the first write of a partial
+  __syncthreads();
+  if (L.active) {
 #pragma unroll
     for (int o = 0; o < NO; ++o)
 #pragma unroll

@@ -2,7 +2,8 @@
 // density bounds printed after each density update and the celerity and
 // velocity maxima behind the acoustic and advective Courant numbers
 // (Atmospheric_Flow.cc:991-1087, 1121-1122, 1164-1165, 1205-1206, 1225-1235),
-// evaluated at the QGauss<dim>(k+1) points of the reference's FEValues.
+// evaluated at the QGaussLobatto<dim>(k+1) points of the reference's FEValues
+// (Atmospheric_Flow.cc:86-88, 233-235): FE_DGQ's support points, i.e. the nodal values.
 #pragma once
 #include "common.h"
 

@@ -1,10 +1,12 @@
 // Per-step diagnostics (Atmospheric_Flow.cc:991-1087): one thread per
-// (cell, QGauss(k+1) point); the (k+1)^2 points of a cell also index its
-// (k+1)^2 dofs, so the nodal L-infinity norm comes from the same pass.
-// Maxima and minima skip non-finite values and count them instead, so a
-// NaN in the state shows up as a nonzero count rather than vanishing in
-// fmax/fmin. Block partials are combined by one finalize block; the result
-// stays on the device until the caller reads the record.
+// (cell, dof). The reference evaluates the fields at the QGaussLobatto(k+1)
+// points (Atmospheric_Flow.cc:86-88, 233-235), which are FE_DGQ's support
+// points, so each point value is the nodal value of the same index and the
+// nodal L-infinity norm comes from the same pass. Maxima and minima skip
+// non-finite values and count them instead, so a NaN in the state shows up
+// as a nonzero count rather than vanishing in fmax/fmin. Block partials are
+// combined by one finalize block; the result stays on the device until the
+// caller reads the record.
 #include <cmath>
 #include <limits>
 
@@ -49,36 +51,20 @@ __device__ inline void block_max2_sum(double& a, double& b, double& c) {
   }
 }
 
-// value of one cell's field at the point (qx, qz) of QGauss(k+1)
-template <int K>
-__device__ inline double point_value(const double* __restrict__ f, long cell, int qx, int qz) {
-  constexpr int ND = K + 1;
-  const double* c = f + cell * ND * ND;
-  double v = 0.0;
-#pragma unroll
-  for (int j = 0; j < ND; ++j) {
-    double r = 0.0;
-#pragma unroll
-    for (int i = 0; i < ND; ++i) r += basis::kE<K>.S[qx][i] * c[j * ND + i];
-    v += basis::kE<K>.S[qz][j] * r;
-  }
-  return v;
-}
-
 // partials: [0,nb) = max of -rho at the points (the min, negated),
 // [nb,2nb) = max |rho| at the dofs, [2nb,3nb) = non-finite count
-template <int K>
 __global__ void __launch_bounds__(DT) k_diag_density(const double* __restrict__ rho, long items,
                                                      double* __restrict__ part) {
-  constexpr int ND = K + 1, ND2 = ND * ND;
   double nmin = -std::numeric_limits<double>::max(), amax = 0.0, bad = 0.0;
   for (long t = (long)blockIdx.x * DT + threadIdx.x; t < items; t += (long)gridDim.x * DT) {
-    const long cell = t / ND2;
-    const int pt = (int)(t - cell * ND2);
-    const double v = point_value<K>(rho, cell, pt % ND, pt / ND);
     const double d = rho[t];
-    if (isfinite(v)) nmin = fmax(nmin, -v); else bad += 1.0;
-    if (isfinite(d)) amax = fmax(amax, fabs(d)); else bad += 1.0;
+    // the point value (min) and the dof value (linfty_norm) are the same number: counted once per use
+    if (isfinite(d)) {
+      nmin = fmax(nmin, -d);
+      amax = fmax(amax, fabs(d));
+    } else {
+      bad += 2.0;
+    }
   }
   block_max2_sum(nmin, amax, bad);
   if (threadIdx.x == 0) {
@@ -89,17 +75,12 @@ __global__ void __launch_bounds__(DT) k_diag_density(const double* __restrict__ 
 }
 
 // partials: [0,nb) = max celerity, [nb,2nb) = max |u|, [2nb,3nb) = non-finite count
-template <int K>
 __global__ void __launch_bounds__(DT) k_diag_flow(const double* __restrict__ rho, const double* __restrict__ u,
                                                   const double* __restrict__ p, double gamma_gas, long Ns,
                                                   double* __restrict__ part) {
-  constexpr int ND = K + 1, ND2 = ND * ND;
   double cmax = 0.0, vmax = 0.0, bad = 0.0;
   for (long t = (long)blockIdx.x * DT + threadIdx.x; t < Ns; t += (long)gridDim.x * DT) {
-    const long cell = t / ND2;
-    const int pt = (int)(t - cell * ND2), qx = pt % ND, qz = pt / ND;
-    const double r = point_value<K>(rho, cell, qx, qz), pp = point_value<K>(p, cell, qx, qz);
-    const double u0 = point_value<K>(u, cell, qx, qz), u1 = point_value<K>(u + Ns, cell, qx, qz);
+    const double r = rho[t], pp = p[t], u0 = u[t], u1 = u[Ns + t];
     const double c = sqrt(gamma_gas * pp / r), w = sqrt(u0 * u0 + u1 * u1);
     if (isfinite(c)) cmax = fmax(cmax, c); else bad += 1.0;
     if (isfinite(w)) vmax = fmax(vmax, w); else bad += 1.0;
@@ -142,7 +123,7 @@ void launch_diag_density(const MeshDev& m, const double* rho, int stage, double*
                          hipStream_t s) {
   const long items = (long)m.ncell * (K + 1) * (K + 1);
   const int nb = diag_blocks(items);
-  hipLaunchKernelGGL(k_diag_density<K>, dim3(nb), dim3(DT), 0, s, rho, items, partials);
+  hipLaunchKernelGGL(k_diag_density, dim3(nb), dim3(DT), 0, s, rho, items, partials);
   hipLaunchKernelGGL(k_diag_fin, dim3(1), dim3(DT), 0, s, partials, nb, -1.0, rec, DG_MIN_RHO + stage,
                      DG_MAX_RHO + stage, DG_NONFINITE + stage);
 }
@@ -151,7 +132,7 @@ template <int K>
 void launch_diag_flow(const MeshDev& m, const double* rho, const double* u, const double* p, double gamma_gas,
                       long Ns, double* partials, double* rec, hipStream_t s) {
   const int nb = diag_blocks(Ns);
-  hipLaunchKernelGGL(k_diag_flow<K>, dim3(nb), dim3(DT), 0, s, rho, u, p, gamma_gas, Ns, partials);
+  hipLaunchKernelGGL(k_diag_flow, dim3(nb), dim3(DT), 0, s, rho, u, p, gamma_gas, Ns, partials);
   hipLaunchKernelGGL(k_diag_fin, dim3(1), dim3(DT), 0, s, partials, nb, 1.0, rec, DG_MAX_CEL, DG_MAX_VEL,
                      DG_NONFINITE + 3);
 }

@@ -90,6 +90,11 @@ class Engine {
   virtual int solve_mass_mg(int ns, double* x, const double* b, double tol_factor) = 0;
   virtual int solve_schur(double* x, const double* b, double tol_factor) = 0;
 
+  // ---- vector operations on n doubles of device memory, on the engine's stream
+  virtual void axpby(long n, double a, const double* x, double b, double* y) = 0;  // y = a x + b y
+  virtual double dot(long n, const double* x, const double* y) = 0;               // synchronizes
+  virtual double linf(long n, const double* x) = 0;                                // max |x_i|; synchronizes
+
   // ---- IMEX driver (state is the engine's own)
   virtual void load_state(const double* rho, const double* u, const double* p) = 0;  // device pointers
   virtual void store_state(double* rho, double* u, double* p) = 0;

@@ -314,6 +314,21 @@ class EngineK : public Engine {
 
   int solve_schur(double* x, const double* b, double tol_factor) override { return gmres(x, b, tol_factor); }
 
+  // ----------------------------------------------------------- vector operations
+  void axpby(long n, double a, const double* x, double b, double* y) override { vec_axpby(n, a, x, b, y, s_); }
+  double dot(long n, const double* x, const double* y) override {
+    vec_dot(n, x, y, scal.p + 240, false, rb, s_);
+    ATMO_HIP_CHECK(hipMemcpyAsync(hscal + 240, scal.p + 240, sizeof(double), hipMemcpyDeviceToHost, s_));
+    sync();
+    return hscal[240];
+  }
+  double linf(long n, const double* x) override {
+    vec_linf(n, x, nullptr, scal.p + 241, rb, s_);
+    ATMO_HIP_CHECK(hipMemcpyAsync(hscal + 241, scal.p + 241, sizeof(double), hipMemcpyDeviceToHost, s_));
+    sync();
+    return hscal[241];
+  }
+
   // ----------------------------------------------------------- IMEX driver
   void load_state(const double* rho, const double* u, const double* p) override {
     const long Ns = fine().Ns;

@@ -58,6 +58,31 @@ class DeviceVector:
         N.check(N.load().atmo_vec_download(self.ctx.handle, self.handle, _ptr(out), out.size), self.ctx.handle)
         return out
 
+    # deal.II LinearAlgebra::distributed::Vector operations, on the device
+    def sadd(self, s, a, x):
+        """self = s * self + a * x"""
+        N.check(N.load().atmo_vec_axpby(self.ctx.handle, float(a), x.handle, float(s), self.handle), self.ctx.handle)
+        return self
+
+    def equ(self, a, x):
+        """self = a * x"""
+        return self.sadd(0.0, a, x)
+
+    def add(self, a, x):
+        """self += a * x"""
+        return self.sadd(1.0, a, x)
+
+    def __mul__(self, other):
+        """scalar product self . other"""
+        r = ctypes.c_double()
+        N.check(N.load().atmo_vec_dot(self.ctx.handle, self.handle, other.handle, ctypes.byref(r)), self.ctx.handle)
+        return r.value
+
+    def linfty_norm(self):
+        r = ctypes.c_double()
+        N.check(N.load().atmo_vec_linf(self.ctx.handle, self.handle, ctypes.byref(r)), self.ctx.handle)
+        return r.value
+
     def __del__(self):
         if getattr(self, "handle", None) and getattr(self.ctx, "handle", None):
             N.load().atmo_vec_free(self.ctx.handle, self.handle)

@@ -128,6 +128,21 @@ atmo_status atmo_vec_upload(atmo_ctx* ctx, atmo_vec* v, const double* host, size
 atmo_status atmo_vec_download(atmo_ctx* ctx, const atmo_vec* v, double* host, size_t n);
 /* Device address of the vector's storage (for zero-copy callers on ctx's stream). */
 atmo_status atmo_vec_device_ptr(atmo_vec* v, double** dev);
+/* y = a*x + b*y on the device (b == 0 overwrites y): LinearAlgebra::distributed::
+ * Vector::equ / add / sadd, e.g. pres_tmp.equ(1, pres_fixed); pres_tmp.add(-1,
+ * pres_fixed_old) of the fixed-point test (Atmospheric_Flow.cc:1144-1145).
+ * x and y belong to the same block. */
+atmo_status atmo_vec_axpby(atmo_ctx* ctx, double a, const atmo_vec* x, double b, atmo_vec* y);
+/* Scalar product x.y (Vector::operator*), summed in a fixed order (bitwise
+ * reproducible); synchronizes ctx's stream. */
+atmo_status atmo_vec_dot(atmo_ctx* ctx, const atmo_vec* x, const atmo_vec* y, double* result);
+/* L-infinity norm max_i |x_i|: the fixed-point test's integrate_difference(...,
+ * ZeroFunction, quadrature_temperature, Linfty_norm) + compute_global_error
+ * (Atmospheric_Flow.cc:1140-1148), whose QGaussLobatto(k+1) points are the
+ * FE_DGQ support points, so the point values are the vector's entries; for
+ * the velocity block the maximum also runs over both components, as deal.II's
+ * Linfty_norm does. Synchronizes ctx's stream. */
+atmo_status atmo_vec_linf(atmo_ctx* ctx, const atmo_vec* x, double* result);
 
 /* EULEROperator::set_HYPERBOLIC_stage / set_NS_stage (euler_operator.h:283-303). */
 atmo_status atmo_set_stages(atmo_ctx* ctx, int hyperbolic_stage, int ns_stage);
@@ -170,7 +185,8 @@ atmo_status atmo_imex_step(atmo_ctx* ctx, int nsteps);
  * every step without host synchronization: get_min_density / get_max_density
  * after each density update (Atmospheric_Flow.cc:1121-1122, 1164-1165,
  * 1205-1206; :1020-1061), compute_max_celerity and get_max_velocity of the new
- * state (:1066-1087, :992-1015) at the QGauss(k+1) points, and the Courant
+ * state (:1066-1087, :992-1015) at the QGaussLobatto(k+1) points (the FE_DGQ
+ * support points, :86-88, 233-235: the nodal values), and the Courant
  * numbers built from them with GridTools::minimal_cell_diameter (:1225-1235). */
 typedef struct {
   double min_density[3];  /* min rho at the points after the density update of stage 1..3 */

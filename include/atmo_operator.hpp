@@ -71,6 +71,20 @@ class Vector {
     return p;
   }
   atmo_vec* handle() const { return v_; }
+  // LinearAlgebra::distributed::Vector operations (Atmospheric_Flow.cc:1133-1152), on the device
+  void sadd(double s, double a, const Vector& x) { check(atmo_vec_axpby(ctx_, a, x.v_, s, v_), ctx_); }
+  void equ(double a, const Vector& x) { sadd(0.0, a, x); }
+  void add(double a, const Vector& x) { sadd(1.0, a, x); }
+  double operator*(const Vector& o) const {
+    double r = 0.0;
+    check(atmo_vec_dot(ctx_, v_, o.v_, &r), ctx_);
+    return r;
+  }
+  double linfty_norm() const {
+    double r = 0.0;
+    check(atmo_vec_linf(ctx_, v_, &r), ctx_);
+    return r;
+  }
 
  private:
   atmo_ctx* ctx_;

@@ -2190,27 +2190,33 @@ int orc_run(void* hp, double* rho, double* u, double* p, int nsteps, long* stats
 }
 
 // Per-step diagnostics of EulerSolver::run on one state, evaluated at the
-// QGauss<dim>(k+1) points the reference's FEValues use
-// (Atmospheric_Flow.cc:230-232): out[0] = min rho at the points
-// (get_min_density, :1020-1049), out[1] = max |rho| over the dofs
-// (get_max_density = linfty_norm, :1052-1061), out[2] = max sqrt(gamma p / rho)
-// (compute_max_celerity, :1066-1087), out[3] = max |u| (get_max_velocity,
-// :992-1015), out[4] = number of non-finite values met.
+// QGaussLobatto<dim>(k+1) points the reference's FEValues use
+// (quadrature_density/velocity/temperature, Atmospheric_Flow.cc:86-88,
+// 233-235): out[0] = min rho at the points (get_min_density, :1020-1049),
+// out[1] = max |rho| over the dofs (get_max_density = linfty_norm,
+// :1052-1061), out[2] = max sqrt(gamma p / rho) (compute_max_celerity,
+// :1066-1087), out[3] = max |u| (get_max_velocity, :992-1015), out[4] =
+// number of non-finite values met (a guard the reference does not have).
+// The points are FE_DGQ's support points, so the shape values there are
+// 0 and 1 (the Lagrange products hold an exact zero factor or only factors
+// equal to 1); zero terms are skipped, so a non-finite dof is counted at its
+// own point only.
 void orc_diagnostics(void* hp, const double* rho, const double* u, const double* p, double gamma_gas, double* out) {
   Solver* s = ((OrcHandle*)hp)->s;
   const Tables& T = s->fine->T;
   const int nd = T.nd, Ns = s->fine->Ns, nc = s->fine->ncell;
-  Vec xq, wq;
-  gauss_legendre(nd, xq, wq);
   std::vector<double> L(nd * nd);
   for (int q = 0; q < nd; ++q)
-    for (int i = 0; i < nd; ++i) L[q * nd + i] = lagrange(T.gll, i, xq[q]);
+    for (int i = 0; i < nd; ++i) L[q * nd + i] = lagrange(T.gll, i, T.gll[q]);
   double mn = std::numeric_limits<double>::max(), mx = 0.0, cel = 0.0, vel = 0.0;
   long nonfinite = 0;
   auto at = [&](const double* f, int c, int qx, int qz) {
     double v = 0.0;
     for (int j = 0; j < nd; ++j)
-      for (int i = 0; i < nd; ++i) v += L[qz * nd + j] * L[qx * nd + i] * f[c * nd * nd + j * nd + i];
+      for (int i = 0; i < nd; ++i) {
+        const double w = L[qz * nd + j] * L[qx * nd + i];
+        if (w != 0.0) v += w * f[c * nd * nd + j * nd + i];
+      }
     return v;
   };
   for (int c = 0; c < nc; ++c) {

@@ -219,7 +219,7 @@ class Oracle:
         return rho, u, p, dict(zip(keys, list(st)))
 
     def diagnostics(self, rho, u, p, gamma=1.4):
-        """(min rho at QGauss(k+1) points, max |rho| over dofs, max celerity,
+        """(min rho at QGaussLobatto(k+1) points, max |rho| over dofs, max celerity,
         max |u|, number of non-finite values) of one state."""
         out = np.zeros(5)
         a = [np.ascontiguousarray(x, dtype=np.float64) for x in (rho, u, p)]

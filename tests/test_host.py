@@ -185,3 +185,24 @@ def test_cpp_operator_header_compiles(tmp_path):
     r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-I" + inc, str(src)],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
+
+
+def test_periodic_mesh_needs_matching_end_columns():
+    """x-periodic: the seam's two sides are one face, so h[0] must equal h[nx]
+    (deal.II's periodic-face matching rejects other meshes); a wall mesh may
+    have any end heights."""
+    k, nx, nz = 2, 4, 2
+    ns = nx * nz * (k + 1) ** 2
+    out = [np.zeros(ns), np.zeros(2 * ns), np.zeros(ns)]
+    dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))  # noqa: E731
+    h = np.array([0.0, 10.0, 20.0, 10.0, 5.0])
+    for periodic, want in ((1, N.ATMO_INVALID_ARG), (0, N.ATMO_OK)):
+        mesh = N.MeshDesc(k, nx, nz, periodic, 40000.0, 20000.0, dp(h))
+        st = N.load().atmo_init_hill_state(ctypes.byref(mesh), 300.0, 1e5, 0.01, 10.0, 9.81, 287.0, 1.4, 1e-4, 1,
+                                           *[dp(a) for a in out])
+        assert st == want, periodic
+    assert b"periodic" in N.load().atmo_last_error(None)
+    h[-1] = 0.0
+    mesh = N.MeshDesc(k, nx, nz, 1, 40000.0, 20000.0, dp(h))
+    assert N.load().atmo_init_hill_state(ctypes.byref(mesh), 300.0, 1e5, 0.01, 10.0, 9.81, 287.0, 1.4, 1e-4, 1,
+                                         *[dp(a) for a in out]) == N.ATMO_OK

@@ -1,7 +1,13 @@
-"""Pins the oracle's per-step diagnostics (Atmospheric_Flow.cc:991-1087) with
-closed forms: fields that are polynomials of degree <= k are represented
-exactly, so their extrema over the QGauss(k+1) points follow from the Gauss
-nodes alone (numpy leggauss), independently of the oracle's interpolation."""
+"""Pins the oracle's per-step diagnostics (Atmospheric_Flow.cc:991-1087).
+
+The reference evaluates the fields at the QGaussLobatto(k+1) points
+(quadrature_density/velocity/temperature, Atmospheric_Flow.cc:86-88,
+233-235). Fields that are polynomials of degree <= k per direction are
+represented exactly, so their extrema over those points follow from the
+Gauss-Lobatto nodes alone, computed here from the roots of P_k' (numpy),
+independently of the oracle's tables and interpolation. The fields are
+chosen so that the Gauss points of the same count give other extrema, so a
+diagnostic evaluated at the wrong point set fails."""
 import numpy as np
 import pytest
 
@@ -10,9 +16,21 @@ from oracle.pyoracle import Oracle, tables, witch_of_agnesi
 LX, H = 40000.0, 20000.0
 
 
+def gll_unit(k):
+    """Gauss-Lobatto nodes of k+1 points on [0, 1], ascending: the end points
+    and the roots of the derivative of the Legendre polynomial P_k."""
+    inner = np.polynomial.legendre.Legendre.basis(k).deriv().roots() if k > 1 else np.array([])
+    x = np.concatenate([[-1.0], np.sort(inner.real), [1.0]])
+    return 0.5 * (x + 1.0)
+
+
+def gauss_unit(k):
+    return np.sort(0.5 * (np.polynomial.legendre.leggauss(k + 1)[0] + 1.0))
+
+
 def nodal(o, k, nx, nz, f):
     """Nodal values of f(x, z) on a flat mesh, [cell][dof] layout."""
-    gll = tables(k)[0]
+    gll = gll_unit(k)
     dx, dz = LX / nx, H / nz
     out = np.zeros(o.Ns)
     nd = k + 1
@@ -25,26 +43,46 @@ def nodal(o, k, nx, nz, f):
     return out
 
 
+def extremum(k, nx, nz, f, pts, op):
+    """op (min or max) of f over the points pts (on [0,1]) of every cell of a flat mesh."""
+    dx, dz = LX / nx, H / nz
+    xs = np.concatenate([(i + pts) * dx for i in range(nx)])
+    zs = np.concatenate([(j + pts) * dz for j in range(nz)])
+    X, Z = np.meshgrid(xs, zs)
+    return op(f(X, Z))
+
+
+@pytest.mark.parametrize("k", [1, 2, 3])
+def test_gll_table_is_gauss_lobatto(k):
+    np.testing.assert_allclose(tables(k)[0], gll_unit(k), rtol=0, atol=1e-15)
+
+
 @pytest.mark.parametrize("k", [2, 3])
-def test_extrema_of_polynomial_fields(k):
+def test_extrema_at_gauss_lobatto_points(k):
     nx, nz = 4, 2
     o = Oracle(k, nx, nz)
-    a, b = 2e-6, 3e-6
-    rho = nodal(o, k, nx, nz, lambda x, z: 1.0 + a * x + b * z)
-    c = 500.0
-    p = nodal(o, k, nx, nz, lambda x, z: c * (1.0 + a * x + b * z) ** 2)
-    u = np.concatenate([nodal(o, k, nx, nz, lambda x, z: 10.0 + 1e-4 * z), nodal(o, k, nx, nz, lambda x, z: -2.0)])
+    # interior extrema: a parabola in x with its vertex inside a cell (degree 2 <= k)
+    x0, z0 = 0.2 * LX / nx, 0.15 * H / nz
+    fr = lambda x, z: 1.0 + 1e-9 * (x - x0) ** 2 + 2e-9 * (z - z0) ** 2
+    fp = lambda x, z: 1e5 * (1.0 - 1e-10 * (x - x0) ** 2)
+    fu = lambda x, z: 10.0 - 1e-8 * (x - x0) ** 2
+    fw = lambda x, z: 1e-4 * (z - z0)
+    rho, p = nodal(o, k, nx, nz, fr), nodal(o, k, nx, nz, fp)
+    u = np.concatenate([nodal(o, k, nx, nz, fu), nodal(o, k, nx, nz, fw)])
     g = 1.4
     mn, mx, cel, vel, bad = o.diagnostics(rho, u, p, g)
-    xg = 0.5 * (1.0 - np.polynomial.legendre.leggauss(k + 1)[0][::-1])  # ascending on [0,1]
-    dx, dz = LX / nx, H / nz
-    x_lo, z_lo = xg[0] * dx, xg[0] * dz
-    x_hi, z_hi = LX - dx + xg[-1] * dx, H - dz + xg[-1] * dz
-    assert mn == pytest.approx(1.0 + a * x_lo + b * z_lo, rel=1e-14)
-    assert mx == pytest.approx(1.0 + a * LX + b * H, rel=1e-14)  # GLL nodes include the far corner
-    assert cel == pytest.approx(np.sqrt(g * c * (1.0 + a * x_hi + b * z_hi)), rel=1e-13)
-    assert vel == pytest.approx(np.hypot(10.0 + 1e-4 * z_hi, 2.0), rel=1e-14)
+    gl, ga = gll_unit(k), gauss_unit(k)
+    want_min = extremum(k, nx, nz, fr, gl, np.min)
+    want_cel = extremum(k, nx, nz, lambda x, z: np.sqrt(g * fp(x, z) / fr(x, z)), gl, np.max)
+    want_vel = extremum(k, nx, nz, lambda x, z: np.hypot(fu(x, z), fw(x, z)), gl, np.max)
+    assert mn == pytest.approx(want_min, rel=1e-14)
+    assert mx == pytest.approx(np.max(np.abs(rho)), rel=0)  # linfty_norm of the dofs
+    assert cel == pytest.approx(want_cel, rel=1e-13)
+    assert vel == pytest.approx(want_vel, rel=1e-14)
     assert bad == 0
+    # the Gauss points of the same count give other extrema: the test sees the point set
+    assert abs(extremum(k, nx, nz, fr, ga, np.min) - want_min) > 1e-12 * want_min
+    assert abs(extremum(k, nx, nz, lambda x, z: np.hypot(fu(x, z), fw(x, z)), ga, np.max) - want_vel) > 1e-10
 
 
 def test_nonfinite_values_are_counted_not_hidden():
@@ -55,8 +93,8 @@ def test_nonfinite_values_are_counted_not_hidden():
     u = np.zeros(2 * o.Ns)
     rho[5] = np.nan
     mn, mx, cel, vel, bad = o.diagnostics(rho, u, p)
-    # the NaN dof enters all 16 points of its cell (density and celerity) and itself once
-    assert bad == 16 + 16 + 1
+    # the NaN dof is its own point's value: counted by the density minimum, the celerity and linfty_norm
+    assert bad == 3
     assert mn == pytest.approx(1.0, rel=1e-14) and mx == 1.0
 
 
