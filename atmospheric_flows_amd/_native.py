@@ -88,6 +88,8 @@ SIGNATURES = [
                                       ctypes.c_void_p]),
     ("atmo_vec_dot", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, _dp]),
     ("atmo_vec_linf", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, _dp]),
+    ("atmo_dot_noalloc", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p,
+                                        ctypes.c_void_p]),
     ("atmo_set_stages", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     ("atmo_set_rho_for_fixed", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ("atmo_set_pres_fixed", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),

@@ -144,6 +144,21 @@ atmo_status atmo_vec_dot(atmo_ctx* ctx, const atmo_vec* x, const atmo_vec* y, do
  * Linfty_norm does. Synchronizes ctx's stream. */
 atmo_status atmo_vec_linf(atmo_ctx* ctx, const atmo_vec* x, double* result);
 
+/* No counterpart in the reference: a variant of atmo_vec_dot for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). The same fixed-order scalar product
+ * (bitwise equal to atmo_vec_dot on the same data) on caller-owned device
+ * memory, with no context; it allocates nothing: its block partials live in a
+ * static device array of the library (1024 doubles, 8 KiB per GPU, held until
+ * the process ends). x, y: n doubles on the current HIP device; result: one
+ * double of device memory, written by the call's last kernel; stream: a
+ * hipStream_t (NULL = the null stream). Returns without synchronizing. Every
+ * call on one GPU shares that GPU's array: calls that overlap in time on one
+ * GPU race (serialize them, e.g. on one stream). n <= ATMO_DOT_NOALLOC_MAX_N,
+ * else ATMO_INVALID_ARG. */
+#define ATMO_DOT_NOALLOC_MAX_N (((size_t)1) << 36)
+atmo_status atmo_dot_noalloc(const double* x, const double* y, size_t n, void* stream, double* result);
+
 /* EULEROperator::set_HYPERBOLIC_stage / set_NS_stage (euler_operator.h:283-303). */
 atmo_status atmo_set_stages(atmo_ctx* ctx, int hyperbolic_stage, int ns_stage);
 /* set_rho_for_fixed (:312-315): also refreshes the level densities
