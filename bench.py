@@ -202,28 +202,39 @@ def measure_traffic(dom_kernel_prefix, copy_bytes, reps=3, timeout=240):
                 fetch_scale=res["FETCH_SIZE"][1] / 1024.0, write_scale=res["WRITE_SIZE"][1] / 1024.0)
 
 
+def member_dt(nx, k, U):
+    """Time step of an ensemble or serving member: the headline's acoustic
+    Courant number scaled to the mesh (512 s / nx), capped so that the
+    advective Courant number dt |u|_max k sqrt(2) / dx stays <= 0.35 with
+    |u|_max = 3 U: flow over a steep, coarsely resolved hill reaches about
+    2.5 U (a 64x32 Q3 member with h0 = 586 m at 1 U went unstable after 40
+    steps)."""
+    dx = 40000.0 / nx
+    return min(512.0 / nx, 0.35 * dx / (3.0 * U * k * np.sqrt(2.0)))
+
+
 def run_ensemble(args):
     """Ensemble of independent members on one GPU: 64 seed-drawn members,
     mesh in {64x32, 128x64, 256x128}, degree in {2, 3}, hill height
     U[100, 600] m, wind U[5, 15] m/s, N U[0.008, 0.012] 1/s, mixed
-    precision, dt = the headline's acoustic Courant number scaled to the
-    mesh, capped so the advective Courant number dt*U*k*sqrt(2)/dx stays
-    <= 0.35; --ens-steps steps each. Reports per member-step and
+    precision, dt from member_dt; --ens-steps steps each. Reports per member-step and
     per-request latency percentiles and the aggregate throughput."""
     import atmospheric_flows_amd as af
     rng = np.random.default_rng(20240601)
     ens = af.Ensemble(slots=args.ens_slots)
     meshes = ((64, 32), (128, 64), (256, 128))
     dofs = 0
+    desc = []
     for i in range(args.ens_members):
         nx, nz = meshes[rng.integers(3)]
         k = int(rng.integers(2, 4))
         h0, U, Nbv = float(rng.uniform(100, 600)), float(rng.uniform(5, 15)), float(rng.uniform(0.008, 0.012))
         dx = 40000.0 / nx
-        dt = min(512.0 / nx, 0.35 * dx / (U * k * np.sqrt(2.0)))
+        dt = member_dt(nx, k, U)
         m = af.Member(k, nx, nz, nsteps=args.ens_steps, h0=h0, U=U, N_bv=Nbv, seed=1000 + i,
                       params=dict(dt=dt, eps=1e-10, precision=1))
         ens.submit(m)
+        desc.append(dict(id=i, k=k, nx=nx, nz=nz, h0=h0, U=U, N=Nbv, dt=dt))
         dofs += 4 * nx * nz * (k + 1) ** 2 * args.ens_steps
     print(f"[bench] ensemble: {args.ens_members} members x {args.ens_steps} steps, {args.ens_slots} slots",
           file=sys.stderr, flush=True)
@@ -231,7 +242,9 @@ def run_ensemble(args):
     res = [ens.result(i) for i in range(args.ens_members)]
     failed = [r["error"] for r in res if r["status"] != 0]
     out = dict(members=args.ens_members, steps_per_member=args.ens_steps, slots=args.ens_slots, wall_s=wall,
-               dof_steps_per_s=dofs / wall, failed=len(failed), errors=sorted(set(failed))[:3])
+               dof_steps_per_s=dofs / wall, failed=len(failed), errors=sorted(set(failed))[:3],
+               failed_members=[dict(desc[i], steps_done=r["steps_done"], error=r["error"])
+                               for i, r in enumerate(res) if r["status"] != 0][:8])
     out.update(af.latency_summary(res))
     return out
 
@@ -304,7 +317,7 @@ def run_serve(args, precision):
     for i, ta in enumerate(arrivals):
         nx, nz, k = meshes[rng.integers(4)]
         U = float(rng.uniform(5, 15))
-        dt = min(512.0 / nx, 0.35 * (40000.0 / nx) / (U * k * np.sqrt(2.0)))
+        dt = member_dt(nx, k, U)
         n = int(rng.integers(args.serve_min_steps, args.serve_max_steps + 1))
         ens.submit(af.Member(k, nx, nz, nsteps=n, h0=float(rng.uniform(100, 600)), U=U,
                              N_bv=float(rng.uniform(0.008, 0.012)), seed=5000 + i, arrival_s=ta,
