@@ -376,12 +376,42 @@ def cpu_baseline(threads):
 
 
 def cpu_headline(threads, steps):
-    """The oracle on the headline workload itself (512x256 Q3, dt = 1 s):
-    `steps` timed steps after one warm-up step (several minutes of CPU)."""
-    value, el, used = cpu_run(threads, 512, 256, 3, 1.0, steps, warmup=1)
+    """The oracle on the headline workload itself (512x256 Q3, dt = 1 s,
+    reference tolerances): `steps` (>= 3) timed steps from the seeded state,
+    several minutes of CPU. The state after the first two is compared with
+    the GPU's plain FP64 form and the mixed form after the same two steps."""
+    from oracle.pyoracle import Oracle, lib, witch_of_agnesi
+    import atmospheric_flows_amd as af
+    steps = max(3, steps)
+    lib().orc_set_threads(threads)
+    nx, nz, k, dt = 512, 256, 3, 1.0
+    h = witch_of_agnesi(nx)
+    o = Oracle(k, nx, nz, hx=h)
+    o.set_params(Ma=1.0, Fr=1.0 / np.sqrt(9.81), dt=dt, eps=1e-10)
+    s0 = o.init_hill(seed=20240601)
+    per_step = []
+    state, s2 = s0, None
+    for i in range(steps):
+        t0 = time.perf_counter()
+        state = o.run(*state, 1)[:3]
+        per_step.append(time.perf_counter() - t0)
+        print(f"[bench] oracle 512x256 Q3 step {i + 1}: {per_step[-1]:.1f} s", file=sys.stderr, flush=True)
+        if i == 1:
+            s2 = state
+    med = float(np.median(per_step))
     nproc, model = host_cpu()
-    return dict(value=value, unit="DoF-steps/s", cores=used, kind="port", host_nproc=nproc, cpu_model=model,
-                sample=f"{steps} IMEX steps of 512x256 Q3 at dt=1 s after 1 warm-up step, {el:.1f} s wall")
+    cpu = dict(value=dof_count(nx, nz, k) / med, unit="DoF-steps/s", cores=lib().orc_get_threads(), kind="port",
+               host_nproc=nproc, cpu_model=model, steps=steps, seconds_per_step=per_step,
+               sample=f"{steps} IMEX steps of 512x256 Q3 at dt=1 s from the seeded state; median step {med:.1f} s")
+    gpu = {}
+    for name, prec in (("fp64", 0), ("mixed", 1)):
+        ctx = af.DeviceContext(k, nx, nz, h=h, Ma=1.0, Fr=1.0 / np.sqrt(9.81), dt=dt, eps=1e-10, precision=prec)
+        ctx.upload_state(*s0)
+        ctx.imex_step(2)
+        d = ctx.download_state()
+        gpu[name] = {nm: float(np.linalg.norm(a - b) / np.linalg.norm(b)) for nm, a, b in zip(("rho", "u", "p"), d, s2)}
+        del ctx
+    return dict(cpu=cpu, rel_err_vs_oracle_after_2_steps=gpu)
 
 
 def main():

@@ -155,3 +155,76 @@ def test_amr_imex_steps_parity_and_mixed_form():
         assert diag["nonfinite"] == 0
     for a, b in zip(res[1], res[0]):
         assert rel(a, b) < 1e-8
+
+
+def test_c3_mesh_against_oracle():
+    """The AMR workload's own mesh (40x20 base, 3 levels in |x - 20 km| < 3 km,
+    z < 6 km plus one level where the seeded density gradient exceeds its 90th
+    percentile; 3386 cells, Q2, dt = 0.5 s): every explicit residual and the
+    mass-type and Schur products against the oracle, two IMEX steps in parity
+    mode (tight tolerances, fixed-point count locked), and the mixed form
+    against its FP64 comparator (same algorithm, FP64 multigrid) over two steps
+    at the reference tolerances."""
+    k, nx, nz, dt = 2, 40, 20, 0.5
+    leaves, h = hill_amr_leaves(nx=nx, nz=nz, levels=3)
+    tight = dict(eps=1e-13, schur_inner_tol=1e-14, a_solve_tol=1e-14, fp_lock=3)
+    dev = af.DeviceContext(k, nx, nz, h=h, leaves=leaves, Ma=1.0, Fr=FR, dt=dt, **tight)
+    orc = Oracle(k, nx, nz, hx=h, leaves=leaves)
+    orc.set_params(Ma=1.0, Fr=FR, dt=dt, **tight)
+    assert dev.ns == orc.Ns == 3386 * 9
+    op = af.EULEROperator(dev)
+    st = [random_state(dev, 500 + i) for i in range(4)]
+    R, B, P = af.BLOCK_DENSITY, af.BLOCK_VELOCITY, af.BLOCK_PRESSURE
+    dv = {}
+    for s, (rho, u, p) in enumerate(st):
+        dv[f"r{s}"], dv[f"u{s}"], dv[f"p{s}"] = dev.vector(R, rho), dev.vector(B, u), dev.vector(P, p)
+    get = {"r": 0, "u": 1, "p": 2}
+    host = lambda names: [st[int(n[1])][get[n[0]]] for n in names]  # noqa: E731
+    worst = 0.0
+    for hs in (1, 2, 3):
+        op.set_HYPERBOLIC_stage(hs)
+        orc.set_stage(hs, 1)
+        for fn, ofn, srcs, blk in (
+                (op.vmult_rhs_rho_update, orc.rhs_rho, sum(([f"r{s}", f"u{s}"] for s in range(hs)), []), R),
+                (op.vmult_rhs_pressure, orc.rhs_pressure,
+                 sum(([f"r{s}", f"u{s}", f"p{s}"] for s in range(hs)), []) + ["r3", "u3"], P),
+                (op.vmult_rhs_velocity_fixed, orc.rhs_velocity,
+                 sum(([f"r{s}", f"u{s}", f"p{s}"] for s in range(hs)), []) + (["r3"] if hs < 3 else []), B)):
+            out = dev.vector(blk)
+            fn(out, [dv[n] for n in srcs])
+            e = rel(out.download(), ofn(host(srcs)))
+            worst = max(worst, e)
+            assert e < 1e-12, (fn.__name__, hs)
+    op.set_HYPERBOLIC_stage(2)
+    orc.set_stage(2, 2)
+    op.set_rho_for_fixed(dv["r2"])
+    orc.set_rho_for_fixed(st[2][0])
+    op.set_pres_fixed(dv["p2"])
+    orc.set_pres_fixed(st[2][2])
+    for ns, block, src, tol in ((1, R, "r1", 1e-12), (3, B, "u1", 1e-12), (6, P, "p1", 1e-12), (2, P, "p1", 1e-11)):
+        op.set_NS_stage(ns)
+        out = dev.vector(block)
+        op.vmult(out, dv[src])
+        e = rel(out.download(), orc.vmult(st[1][get[src[0]]], ns))
+        print(f"C3 vmult NS {ns}: {e:.2e}", flush=True)
+        assert e < tol, ns
+    print(f"C3 residuals: worst rel L2 {worst:.2e}", flush=True)
+    rho, u, p = dev.init_hill(seed=20240603)
+    dev.upload_state(rho, u, p)
+    dev.imex_step(2)
+    out = dev.download_state()
+    ref = orc.run(rho, u, p, 2)
+    for name, a, b in zip(("rho", "u", "p"), out, ref[:3]):
+        e = rel(a, b)
+        print(f"C3 two steps {name}: rel L2 {e:.3e}", flush=True)
+        assert e < 1e-11, name
+    res = {}
+    for prec in (1, 2):
+        d = af.DeviceContext(k, nx, nz, h=h, leaves=leaves, Ma=1.0, Fr=FR, dt=dt, eps=1e-10, precision=prec)
+        d.upload_state(rho, u, p)
+        d.imex_step(2)
+        res[prec] = d.download_state()
+    for name, a, b in zip(("rho", "u", "p"), res[1], res[2]):
+        e = rel(a, b)
+        print(f"C3 mixed vs FP64 comparator {name}: {e:.3e}", flush=True)
+        assert e < 1e-8, name

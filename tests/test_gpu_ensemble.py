@@ -68,3 +68,29 @@ def test_admission_respects_the_hbm_budget_and_arrivals():
     rl = ens.result(late)
     assert rl["status"] == 0 and rl["admit_s"] >= 0.4 - 1e-6
     assert ens.result(too_big)["status"] == af._native.ATMO_OOM
+
+
+def test_ensemble_mesh_set_alone_and_in_the_mix():
+    """The ensemble workload's mesh set (64x32, 128x64, 256x128 at Q2 and Q3,
+    twice each with other drawn parameters: 12 members, mixed precision): each
+    member's state after its steps is bit for bit the same run in a mix of all
+    twelve on 8 slots as run alone in its own context."""
+    rng = np.random.default_rng(77)
+    members = []
+    for rep in range(2):
+        for nx, nz in ((64, 32), (128, 64), (256, 128)):
+            for k in (2, 3):
+                U = float(rng.uniform(5, 15))
+                dt = min(512.0 / nx, 0.35 * (40000.0 / nx) / (3.0 * U * k * np.sqrt(2.0)))
+                members.append(af.Member(k, nx, nz, nsteps=2 + rep, h0=float(rng.uniform(100, 600)), U=U,
+                                         N_bv=float(rng.uniform(0.008, 0.012)), seed=700 + len(members),
+                                         params=dict(dt=dt, eps=1e-10, precision=1)))
+    ens = af.Ensemble(slots=8, keep_states=True)
+    ids = [ens.submit(m) for m in members]
+    ens.run()
+    for i, m in zip(ids, members):
+        r = ens.result(i)
+        assert r["status"] == 0, r["error"]
+        alone = run_alone(m)
+        for a, b in zip(ens.state(i), alone):
+            assert np.array_equal(a, b), (m.nx, m.degree)
