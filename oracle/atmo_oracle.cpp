@@ -31,6 +31,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <limits>
@@ -1842,6 +1843,12 @@ struct Solver {
     }
   }
 
+  // progress on stderr after each stage when ORC_PROGRESS is set (long runs on large meshes)
+  void progress(const char* what) const {
+    static const bool on = std::getenv("ORC_PROGRESS") != nullptr;
+    if (on) std::fprintf(stderr, "[oracle] %s: %ld GMRES its, %ld CG its so far\n", what, stats.gmres_its, stats.cg_its);
+  }
+
   // one IMEX step (Atmospheric_Flow.cc:1115-1222)
   void step() {
     // stage 1
@@ -1853,6 +1860,7 @@ struct Solver {
     fixed_point_loop(1);
     p2 = p_fixed;
     u2 = u_fixed;
+    progress("stage 1");
     // stage 2
     update_density(2);
     op->rho_for_fixed = rho3;
@@ -1862,6 +1870,7 @@ struct Solver {
     fixed_point_loop(2);
     p3 = p_fixed;
     u3 = u_fixed;
+    progress("stage 2");
     // stage 3
     update_density(3);
     op->rho_for_fixed = rho_curr;
@@ -1870,6 +1879,7 @@ struct Solver {
     update_pressure();
     rho_old = rho_curr;
     u_old = u_curr;
+    progress("stage 3");
   }
 };
 
