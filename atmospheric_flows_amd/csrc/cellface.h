@@ -161,12 +161,17 @@ __device__ inline double face_trace(const double* __restrict__ u, int f, const d
 struct NoPrefetch {
   __device__ void operator()() const {}
 };
+// post(): runs once the staged inputs are visible to the whole block, before the column phase (a caller
+// that needs staged values after the column partials have taken over their LDS copies them out)
+struct NoPost {
+  __device__ void operator()() const {}
+};
 
 // pre(): issued after the staging loads and before the first barrier (a
 // caller's own global loads, in flight during the column phase).
-template <int K, class Ph, bool AMR, class Pre = NoPrefetch>
+template <int K, class Ph, bool AMR, class Pre = NoPrefetch, class Post = NoPost>
 __device__ inline void cellface_columns(const MeshDev& m, const Ph& ph, const FieldPtrs<Ph::NF>& in,
-                                        CFSmem<K, Ph, AMR>& sh, const Pre& pre = Pre()) {
+                                        CFSmem<K, Ph, AMR>& sh, const Pre& pre = Pre(), const Post& post = Post()) {
   using C = Cfg<K>;
   constexpr int ND = C::ND, NQ = C::NQ, ND2 = C::ND2;
   constexpr int NF = Ph::NF, NO = Ph::NO;
@@ -181,6 +186,7 @@ __device__ inline void cellface_columns(const MeshDev& m, const Ph& ph, const Fi
   stage_fields<K, NF, double, double, CS, HALO>(&sh.in[0][0], in.p, m.ncell);
   pre();
   __syncthreads();
+  post();
   const CellVerts v = cell_verts_form(m, vcell, vr);
   double Tv[Ph::NO][ND], Td[Ph::NO][ND];
 #pragma unroll

@@ -424,11 +424,14 @@ class EngineK : public Engine {
         case 13:  // the same kernel with no basis vector (D - C and w.w only)
         {
           if (P.gmres_restart < 13) throw std::invalid_argument("time_kernel 12/13 need gmres_restart >= 13");
+          // as GMRES runs it: the operator applied to the newest basis vector (V_13), whose staged values also
+          // serve its own dot product (p, y, p_fix, the Jacobi scale, 13 other basis vectors in; w out)
           const int nv = which == 12 ? 14 : 0;
           const int G = cell_blocks<K>(fine().ncell);
-          launch_d_minus_c_dots<K>(m, oc_, bc_coeff(), pres_fix.p, u_old.p, p_old.p, fine().inv64[BLK_D].p,
+          const double* p = which == 12 ? gm_V.p + 13 * gm_ld() : p_old.p;
+          launch_d_minus_c_dots<K>(m, oc_, bc_coeff(), pres_fix.p, u_old.p, p, fine().inv64[BLK_D].p,
                                    gm_V.p + 14 * gm_ld(), Ns, gm_V.p, gm_ld(), nv, gm_part.p, G, s_);
-          fields = 6 + nv;
+          fields = nv ? 6 + nv - 1 : 6;
           break;
         }
         default:

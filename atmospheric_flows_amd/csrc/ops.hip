@@ -439,26 +439,41 @@ __device__ inline double wave_sum8(const double (&v)[8]) {
 // and w.w: slot k of block b at part[k*pstride + b], slot nv is w.w. The
 // Arnoldi step of GMRES: its Schur product and the classical Gram-Schmidt dot
 // products without a second pass over w. Slots go in chunks of 8 whose loads
-// are all in flight at once (slots past nv load nothing).
+// are all in flight at once (slots past nv load nothing). With p_last, p is
+// column nv-1 of V (GMRES applies the operator to its newest basis vector):
+// that slot's values come from the staged copy of p instead of a second read.
 template <int K, bool AMR>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(AMR ? 1 : 4))) void k_dmc_dots(MeshDev m, PhysDminusC ph, FieldPtrs<4> in, double* __restrict__ dst,
                                                  const double* __restrict__ scale, long Ns,
-                                                 const double* __restrict__ V, long ldv, int nv,
+                                                 const double* __restrict__ V, long ldv, int nv, int p_last,
                                                  double* __restrict__ part, long pstride) {
   constexpr int ND2 = Cfg<K>::ND2, CPW = Cfg<K>::CPW, NPASS = (CPW * ND2 + 63) / 64;
-  __shared__ CFSmem<K, PhysDminusC, AMR> sh;
+  using SM = CFSmem<K, PhysDminusC, AMR>;
+  __shared__ SM sh;
+  // the block's own p values (field 3 of the staged inputs), kept past the column phase
+  __shared__ double pk[CPW * ND2];
   const int blk = cell_block();
   const long base = (long)blk * CPW * ND2;
   const long lim = (long)m.ncell * ND2;
   // the Jacobi scale of the block's dofs is loaded with the staged fields (in flight during the column phase)
   double sc[NPASS];
-  cellface_columns<K, PhysDminusC, AMR>(m, ph, in, sh, [&]() {
+  cellface_columns<K, PhysDminusC, AMR>(
+      m, ph, in, sh,
+      [&]() {
 #pragma unroll
-    for (int ps = 0; ps < NPASS; ++ps) {
-      const int t = threadIdx.x + 64 * ps;
-      sc[ps] = (t < CPW * ND2 && base + t < lim) ? scale[base + t] : 0.0;
-    }
-  });
+        for (int ps = 0; ps < NPASS; ++ps) {
+          const int t = threadIdx.x + 64 * ps;
+          sc[ps] = (t < CPW * ND2 && base + t < lim) ? scale[base + t] : 0.0;
+        }
+      },
+      [&]() {
+        if (!p_last) return;
+#pragma unroll
+        for (int ps = 0; ps < NPASS; ++ps) {
+          const int t = threadIdx.x + 64 * ps;
+          if (t < CPW * ND2) pk[t] = sh.in[3][(t / ND2 + SM::HALO) * SM::CS + t % ND2];
+        }
+      });
   OutRows<K> R = out_rows<K>(threadIdx.x);
   // the block's dofs: full passes of 64 lanes, then a tail of TL < 64 dofs
   constexpr int NFULL = (CPW * ND2) / 64, TL = CPW * ND2 - 64 * NFULL;
@@ -499,10 +514,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(AMR ? 1 : 4)
     double y[8][NL];
     double2 t;
   };
+  // the last slot that is read from memory
+  const int nload = p_last ? nv - 1 : nv;
   auto load = [&](int k0, Chunk& c) {
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
-      if (k0 + s < nv) {  // uniform: slots past the basis load nothing
+      if (k0 + s < nload) {  // uniform: slots past the basis (and the staged p) load nothing
         const long col = (long)(k0 + s) * ldv;
 #pragma unroll
         for (int ps = 0; ps < NL; ++ps) c.y[s][ps] = __builtin_nontemporal_load(V + col + gi[ps]);
@@ -514,12 +531,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(AMR ? 1 : 4)
     if constexpr (PAIRS) {
       const int s = l >> 3;
       c.t = make_double2(0.0, 0.0);
-      if (k0 + s < nv) {
+      if (k0 + s < nload) {
         typedef double nt_d2 __attribute__((ext_vector_type(2)));
         const nt_d2 tv = __builtin_nontemporal_load(reinterpret_cast<const nt_d2*>(V + (long)(k0 + s) * ldv + gt));
         c.t = make_double2(tv.x, tv.y);
       }
     }
+  };
+  // slot k's value at full pass ps: the loaded column, the staged p, w itself, or nothing
+  auto yv = [&](int k, const Chunk& c, int s, int ps) {
+    return k < nload ? c.y[s][ps] : (k < nv ? pk[threadIdx.x + 64 * ps] : (k == nv ? wv[ps] : 0.0));
   };
   auto reduce = [&](int k0, const Chunk& c) {
     double v[8];
@@ -528,13 +549,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(AMR ? 1 : 4)
       const int k = k0 + s;
       double acc = 0.0;
 #pragma unroll
-      for (int ps = 0; ps < NL; ++ps) acc += wv[ps] * (k < nv ? c.y[s][ps] : (k == nv ? wv[ps] : 0.0));
+      for (int ps = 0; ps < NL; ++ps) acc += wv[ps] * yv(k, c, s, ps);
       v[s] = acc;
     }
     if constexpr (PAIRS) {
       const int k = k0 + (l >> 3);
-      const double t0 = k < nv ? c.t.x : (k == nv ? wt0 : 0.0);
-      const double t1 = k < nv ? c.t.y : (k == nv ? wt1 : 0.0);
+      const int tl = 64 * NFULL + 2 * (l & 7);  // the block-local dofs of this lane's tail pair
+      const double t0 = k < nload ? c.t.x : (k < nv ? pk[tl] : (k == nv ? wt0 : 0.0));
+      const double t1 = k < nload ? c.t.y : (k < nv ? pk[tl + 1] : (k == nv ? wt1 : 0.0));
       const double tail = wt0 * t0 + wt1 * t1;
 #pragma unroll
       for (int s = 0; s < 8; ++s)
@@ -555,6 +577,8 @@ template <int K>
 void launch_d_minus_c_dots(const MeshDev& m, const OpConsts& oc, double coeff, const double* pfix, const double* u,
                            const double* p, const double* scale, double* dst, long Ns, const double* V, long ldv,
                            int nv, double* part, long pstride, hipStream_t s) {
+  // GMRES applies the operator to its newest basis vector: then slot nv-1 is p itself
+  const int p_last = nv > 0 && p == V + (long)(nv - 1) * ldv;
   const int grid = ceil_div(m.ncell, Cfg<K>::CPW);
   if (grid == 0) return;
   if (grid > pstride) throw std::invalid_argument("d_minus_c_dots: partial buffer too small");
@@ -563,11 +587,11 @@ void launch_d_minus_c_dots(const MeshDev& m, const OpConsts& oc, double coeff, c
   ph.cD = 1.0 / (oc.gamma_gas - 1.0);
   FieldPtrs<4> in{{pfix, u, u + Ns, p}};
   if (m.cface)
-    hipLaunchKernelGGL((k_dmc_dots<K, true>), dim3(grid), dim3(64), 0, s, m, ph, in, dst, scale, Ns, V, ldv, nv, part,
-                       pstride);
+    hipLaunchKernelGGL((k_dmc_dots<K, true>), dim3(grid), dim3(64), 0, s, m, ph, in, dst, scale, Ns, V, ldv, nv,
+                       p_last, part, pstride);
   else
-    hipLaunchKernelGGL((k_dmc_dots<K, false>), dim3(grid), dim3(64), 0, s, m, ph, in, dst, scale, Ns, V, ldv, nv, part,
-                       pstride);
+    hipLaunchKernelGGL((k_dmc_dots<K, false>), dim3(grid), dim3(64), 0, s, m, ph, in, dst, scale, Ns, V, ldv, nv,
+                       p_last, part, pstride);
   ATMO_HIP_CHECK(hipGetLastError());
 }
 

@@ -251,7 +251,8 @@ atmo_status atmo_write_vtu_host(const atmo_mesh_desc* mesh, const double* rho, c
  * vectors with the norm of the result (k_maxpy_norm + finalize; needs
  * gmres_restart >= 13 for both), 12 the GMRES Arnoldi step's scaled D - C
  * with the dot products of its result against 14 basis vectors
- * (k_dmc_dots, partial sums only), 13 the same kernel with no basis vector.
+ * (k_dmc_dots, partial sums only; applied to the newest basis vector, whose
+ * staged values serve its own dot product), 13 the same kernel with no basis vector.
  * Also reports the kernel's algorithmic HBM bytes per launch. */
 atmo_status atmo_time_kernel(atmo_ctx* ctx, int which, int nrep, double* ms_per_launch, double* bytes_per_launch);
 
