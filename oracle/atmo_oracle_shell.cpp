@@ -1,0 +1,1464 @@
+// CPU FP64 restatement of the reference's own 3D case of Atmospheric_Flows:
+// the IMEX-DG operator and time step in dim = 3 at FE_DGQ(1) on a
+// spherical shell with a degree-2 mapping and radial gravity
+// (Atmospheric_Flow.cc:279-289, 1277; euler_operator.h with dim = 3;
+// equation_data.h:47-176 for the solid-body-rotation state).
+//
+// TEST INFRASTRUCTURE ONLY. This file is the parity oracle of the shell
+// path: tests/ and bench.py's cpu_baseline leg load it as the checker (and
+// as the timed CPU path). Nothing in atmospheric_flows_amd/ links, loads or
+// calls it.
+//
+// Independent of the device path by construction:
+//   - geometry: a generic Q2 hexahedron map from the 27 support points of a
+//     cell (the device path uses the shell's factorized form r(zeta) n(xi, eta));
+//   - neighbours: found by matching face centres geometrically, and the
+//     neighbour's trace is evaluated at the neighbour's reference
+//     coordinates of each face quadrature point, found by Newton iteration
+//     on the neighbour's map (the device path uses orientation codes);
+//   - mass-type blocks (M_rho, A, D, all block diagonal in DG) solved by
+//     dense Cholesky per cell, the exact limit of the reference's CG + MG
+//     solves (Atmospheric_Flow.cc:452-536, 701-870) and of the Schur
+//     complement's inner CG (euler_operator.h:2383-2400).
+//
+// Cell-centric evaluation: every cell integrates the fluxes of its six
+// faces against its own test functions, with its own outward normal; the
+// fluxes are antisymmetric, so this equals deal.II's face loop with an
+// interior and an exterior side (euler_operator.h:497-530).
+//
+// Gravity: the reference takes e_r and a^2/r^2 of every SIMD lane of a cell
+// batch from the batch's last lane (euler_operator.h:791-811 and the same
+// pattern in the other cell terms), which makes its result depend on the
+// vector width deal.II was compiled for; this restatement evaluates them at
+// each quadrature point (the intended physics).
+//
+// A second mesh kind, an axis-aligned box with walls on all sides and
+// constant gravity along z, exists only to pin the 3D forms against the 2D
+// oracle: on states that do not depend on y and have no y velocity, every
+// 3D residual equals the 2D one of the x-z slice.
+//
+// Parity status: geometry (volume, area), B*1 = 0, C = -kappa B^T, mass
+// telescoping, the reduction to the 2D oracle on the box, and consistency
+// with the analytic solid-body-rotation steady state under refinement are
+// pinned in tests/test_oracle_shell.py. The vertex positions of deal.II's
+// SphericalManifold refinement and MappingQ(2) support points are not
+// reproduced (build-defined panel map; deal.II is not importable here).
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include <omp.h>
+
+namespace shell_orc {
+
+typedef std::vector<double> Vec;
+
+// ---------------------------------------------------------------- tables
+// QGauss<1>(3) on [0,1] (Atmospheric_Flow.cc:345, n = 2k+1 with k = 1)
+struct Gauss3 {
+  double x[3], w[3];
+  Gauss3() {
+    const double s = std::sqrt(0.6);
+    x[0] = 0.5 - 0.5 * s;
+    x[1] = 0.5;
+    x[2] = 0.5 + 0.5 * s;
+    w[0] = 5.0 / 18.0;
+    w[1] = 8.0 / 18.0;
+    w[2] = 5.0 / 18.0;
+  }
+};
+static const Gauss3 G3;
+
+// QGauss<1>(2) on [0,1] (integrate_difference, Atmospheric_Flow.cc:934-966)
+static const double G2X[2] = {0.5 - 0.5 / 1.7320508075688772935, 0.5 + 0.5 / 1.7320508075688772935};
+
+// FE_DGQ(1): Lagrange on the Gauss-Lobatto nodes {0, 1}
+static inline double q1(int i, double x) { return i ? x : 1.0 - x; }
+static inline double dq1(int i) { return i ? 1.0 : -1.0; }
+// MappingQ(2): Lagrange on {0, 1/2, 1}
+static inline double q2(int a, double x) {
+  if (a == 0) return 2.0 * (x - 0.5) * (x - 1.0);
+  if (a == 1) return -4.0 * x * (x - 1.0);
+  return 2.0 * x * (x - 0.5);
+}
+static inline double dq2(int a, double x) {
+  if (a == 0) return 4.0 * x - 3.0;
+  if (a == 1) return 4.0 - 8.0 * x;
+  return 4.0 * x - 1.0;
+}
+
+// ---------------------------------------------------------------- mesh
+// ncell hexahedra, each given by the 27 support points of its Q2 map
+// (node (a, b, c) at a + 3b + 9c, axes xi, eta, zeta); faces
+// 0/1: xi = 0/1, 2/3: eta = 0/1, 4/5: zeta = 0/1.
+struct Mesh {
+  int ncell = 0;
+  Vec X;                  // [cell][27][3]
+  std::vector<int> nb;    // [cell][6] neighbour cell or -1 (wall)
+  std::vector<int> nbf;   // [cell][6] neighbour's face
+  bool radial = true;     // radial gravity (shell) or constant e_z (box)
+  double planet = 1.0;    // a of the a^2/r^2 factor
+};
+
+static void normalize3(double v[3]) {
+  double n = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  v[0] /= n;
+  v[1] /= n;
+  v[2] /= n;
+}
+
+// Corner directions (d00, d10, d01, d11) of the coarse panels.
+// 6: the faces of the cube (GridGenerator::hyper_shell's default coarse
+// mesh in 3D); 12: the rhombi of the rhombic dodecahedron (its n_cells = 12).
+static std::vector<std::array<double, 12>> panels(int np) {
+  std::vector<std::array<double, 12>> P;
+  if (np == 6) {
+    for (int a = 0; a < 3; ++a)
+      for (int sg = -1; sg <= 1; sg += 2) {
+        double u[3] = {0, 0, 0}, v[3] = {0, 0, 0};
+        int ua = (a + 1) % 3, va = (a + 2) % 3;
+        if (sg < 0) std::swap(ua, va);
+        u[ua] = 1;
+        v[va] = 1;
+        std::array<double, 12> c{};
+        for (int t = 0; t < 2; ++t)
+          for (int s = 0; s < 2; ++s) {
+            double p[3];
+            for (int d = 0; d < 3; ++d) p[d] = (d == a ? sg : 0.0) + (2 * s - 1) * u[d] + (2 * t - 1) * v[d];
+            normalize3(p);
+            for (int d = 0; d < 3; ++d) c[(s + 2 * t) * 3 + d] = p[d];
+          }
+        P.push_back(c);
+      }
+  } else if (np == 12) {
+    for (int a = 0; a < 3; ++a)
+      for (int b = a + 1; b < 3; ++b)
+        for (int sa = -1; sa <= 1; sa += 2)
+          for (int sb = -1; sb <= 1; sb += 2) {
+            int cax = 3 - a - b;
+            double A[3] = {0, 0, 0}, B[3] = {0, 0, 0}, C1[3], C2[3];
+            A[a] = sa;
+            B[b] = sb;
+            for (int d = 0; d < 3; ++d) {
+              C1[d] = A[d] + B[d] + (d == cax ? 1.0 : 0.0);
+              C2[d] = A[d] + B[d] - (d == cax ? 1.0 : 0.0);
+            }
+            normalize3(C1);
+            normalize3(C2);
+            // cycle A, C1, B, C2 -> bilinear corners d00 = A, d10 = C1, d11 = B, d01 = C2,
+            // oriented so that d/ds x d/dt points outwards
+            double e1[3], e2[3], cr[3], m[3];
+            for (int d = 0; d < 3; ++d) {
+              e1[d] = C1[d] - A[d];
+              e2[d] = C2[d] - A[d];
+              m[d] = A[d] + B[d];
+            }
+            cr[0] = e1[1] * e2[2] - e1[2] * e2[1];
+            cr[1] = e1[2] * e2[0] - e1[0] * e2[2];
+            cr[2] = e1[0] * e2[1] - e1[1] * e2[0];
+            if (cr[0] * m[0] + cr[1] * m[1] + cr[2] * m[2] < 0) std::swap(C1, C2);
+            std::array<double, 12> c{};
+            for (int d = 0; d < 3; ++d) {
+              c[0 + d] = A[d];
+              c[3 + d] = C1[d];
+              c[6 + d] = C2[d];
+              c[9 + d] = B[d];
+            }
+            P.push_back(c);
+          }
+  } else {
+    throw std::runtime_error("shell: 6 or 12 coarse panels");
+  }
+  return P;
+}
+
+// direction of panel point (s, t): the normalized bilinear blend of the corners
+static void panel_dir(const std::array<double, 12>& c, double s, double t, double n[3]) {
+  for (int d = 0; d < 3; ++d)
+    n[d] = (1 - s) * (1 - t) * c[d] + s * (1 - t) * c[3 + d] + (1 - s) * t * c[6 + d] + s * t * c[9 + d];
+  normalize3(n);
+}
+
+static void match_faces(Mesh& m, double tol);
+
+// Shell between radii r_in and r_out, np coarse panels, each refined
+// n_ref times (2^n_ref cells per panel edge and across the shell).
+// Cell ((p*n + k)*n + j)*n + i: panel p, layer k (outwards), row j, column i.
+static Mesh shell_mesh(int n_ref, int np, double r_in, double r_out, double planet) {
+  if (n_ref < 0 || n_ref > 7) throw std::runtime_error("shell: refinements 0..7");
+  if (!(r_in > 0 && r_out > r_in)) throw std::runtime_error("shell: 0 < r_in < r_out");
+  auto P = panels(np);
+  const int n = 1 << n_ref;
+  Mesh m;
+  m.ncell = np * n * n * n;
+  m.X.resize((size_t)m.ncell * 81);
+  m.radial = true;
+  m.planet = planet;
+  for (int p = 0; p < np; ++p)
+    for (int k = 0; k < n; ++k)
+      for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+          int c = ((p * n + k) * n + j) * n + i;
+          double* X = &m.X[(size_t)c * 81];
+          for (int cc = 0; cc < 3; ++cc) {
+            double r = r_in + (r_out - r_in) * ((k + 0.5 * cc) / n);
+            for (int b = 0; b < 3; ++b)
+              for (int a = 0; a < 3; ++a) {
+                double dir[3];
+                panel_dir(P[p], (i + 0.5 * a) / n, (j + 0.5 * b) / n, dir);
+                for (int d = 0; d < 3; ++d) X[(a + 3 * b + 9 * cc) * 3 + d] = r * dir[d];
+              }
+          }
+        }
+  match_faces(m, 1e-9 * r_out);
+  return m;
+}
+
+// Axis-aligned box [0,Lx] x [0,Ly] x [0,H], walls everywhere, constant
+// gravity along z; cell (j*ny + jy)*nx + i.
+static Mesh box_mesh(int nx, int ny, int nz, double Lx, double Ly, double H) {
+  Mesh m;
+  m.ncell = nx * ny * nz;
+  m.X.resize((size_t)m.ncell * 81);
+  m.radial = false;
+  for (int j = 0; j < nz; ++j)
+    for (int jy = 0; jy < ny; ++jy)
+      for (int i = 0; i < nx; ++i) {
+        int c = (j * ny + jy) * nx + i;
+        double* X = &m.X[(size_t)c * 81];
+        for (int cc = 0; cc < 3; ++cc)
+          for (int b = 0; b < 3; ++b)
+            for (int a = 0; a < 3; ++a) {
+              double* x = &X[(a + 3 * b + 9 * cc) * 3];
+              x[0] = (i + 0.5 * a) * Lx / nx;
+              x[1] = (jy + 0.5 * b) * Ly / ny;
+              x[2] = (j + 0.5 * cc) * H / nz;
+            }
+      }
+  match_faces(m, 1e-9 * std::max(Lx, std::max(Ly, H)));
+  return m;
+}
+
+static const int FACE_CENTRE[6] = {12, 14, 10, 16, 4, 22};
+
+// neighbours: faces whose centre support points coincide (within tol)
+static void match_faces(Mesh& m, double tol) {
+  const int nf = m.ncell * 6;
+  m.nb.assign(nf, -1);
+  m.nbf.assign(nf, -1);
+  std::vector<int> order(nf);
+  for (int i = 0; i < nf; ++i) order[i] = i;
+  auto cx = [&](int id) { return m.X[(size_t)(id / 6) * 81 + FACE_CENTRE[id % 6] * 3]; };
+  std::sort(order.begin(), order.end(), [&](int a, int b) { return cx(a) < cx(b); });
+  for (int u = 0; u < nf; ++u) {
+    const int a = order[u];
+    const double* pa = &m.X[(size_t)(a / 6) * 81 + FACE_CENTRE[a % 6] * 3];
+    for (int v = u + 1; v < nf; ++v) {
+      const int b = order[v];
+      const double* pb = &m.X[(size_t)(b / 6) * 81 + FACE_CENTRE[b % 6] * 3];
+      if (pb[0] - pa[0] > tol) break;
+      double d = std::abs(pa[0] - pb[0]) + std::abs(pa[1] - pb[1]) + std::abs(pa[2] - pb[2]);
+      if (d <= tol && a / 6 != b / 6) {
+        if (m.nb[a] >= 0 || m.nb[b] >= 0) throw std::runtime_error("shell: a face matched twice");
+        m.nb[a] = b / 6;
+        m.nbf[a] = b % 6;
+        m.nb[b] = a / 6;
+        m.nbf[b] = a % 6;
+      }
+    }
+  }
+}
+
+// x(xi) and J[d][e] = d x_d / d xi_e of the Q2 map of a cell
+static void map_point(const double* X, const double xi[3], double x[3], double J[3][3]) {
+  double L[3][3], dL[3][3];
+  for (int e = 0; e < 3; ++e)
+    for (int a = 0; a < 3; ++a) {
+      L[e][a] = q2(a, xi[e]);
+      dL[e][a] = dq2(a, xi[e]);
+    }
+  for (int d = 0; d < 3; ++d) {
+    x[d] = 0;
+    for (int e = 0; e < 3; ++e) J[d][e] = 0;
+  }
+  for (int c = 0; c < 3; ++c)
+    for (int b = 0; b < 3; ++b)
+      for (int a = 0; a < 3; ++a) {
+        const double* p = &X[(a + 3 * b + 9 * c) * 3];
+        const double w = L[0][a] * L[1][b] * L[2][c];
+        const double w0 = dL[0][a] * L[1][b] * L[2][c];
+        const double w1 = L[0][a] * dL[1][b] * L[2][c];
+        const double w2 = L[0][a] * L[1][b] * dL[2][c];
+        for (int d = 0; d < 3; ++d) {
+          x[d] += w * p[d];
+          J[d][0] += w0 * p[d];
+          J[d][1] += w1 * p[d];
+          J[d][2] += w2 * p[d];
+        }
+      }
+}
+
+// det and inverse Ji[e][d] = d xi_e / d x_d
+static double invert3(const double J[3][3], double Ji[3][3]) {
+  const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+  const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+  const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+  const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+  const double id = 1.0 / det;
+  Ji[0][0] = c00 * id;
+  Ji[1][0] = c01 * id;
+  Ji[2][0] = c02 * id;
+  Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+  Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+  Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+  Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+  Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+  Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+  return det;
+}
+
+// reference point of face f at face quadrature point (gs, gt); the face's
+// own coordinates are the two other axes in increasing order
+static void face_ref(int f, double s, double t, double xi[3]) {
+  const int e = f / 2;
+  const int a0 = e == 0 ? 1 : 0, a1 = e == 2 ? 1 : 2;
+  xi[e] = (f & 1) ? 1.0 : 0.0;
+  xi[a0] = s;
+  xi[a1] = t;
+}
+
+// ---------------------------------------------------------------- geometry
+struct CellGeo {
+  double JxW[27], Ji[27][3][3], x[27][3];
+};
+struct FaceGeo {
+  double JxW[9], n[9][3], x[9][3];
+  double xi[9][3];    // own reference coordinates
+  double xinb[9][3];  // neighbour's reference coordinates (inner faces)
+};
+
+struct Geo {
+  std::vector<CellGeo> cg;
+  std::vector<FaceGeo> fg;  // [cell][6]
+};
+
+// reference coordinates of the physical point x in the cell with support
+// points X, x lying on the cell's face `face`; false when Newton's method
+// does not reach round-off
+static bool newton_inverse(const double* X, const double x[3], int face, double xi[3]) {
+  // start at the centre of the face the point lies on
+  face_ref(face, 0.5, 0.5, xi);
+  int small = 0;
+  for (int it = 0; it < 60 && small < 2; ++it) {
+    double y[3], J[3][3], Ji[3][3];
+    map_point(X, xi, y, J);
+    invert3(J, Ji);
+    double dx[3] = {y[0] - x[0], y[1] - x[1], y[2] - x[2]};
+    double step = 0;
+    for (int e = 0; e < 3; ++e) {
+      double s = Ji[e][0] * dx[0] + Ji[e][1] * dx[1] + Ji[e][2] * dx[2];
+      xi[e] -= s;
+      step = std::max(step, std::abs(s));
+    }
+    if (step < 1e-13) ++small;  // two more iterations once at round-off level
+  }
+  xi[face / 2] = (face & 1) ? 1.0 : 0.0;  // exactly on the face
+  double y[3], J[3][3];
+  map_point(X, xi, y, J);
+  const double res = std::abs(y[0] - x[0]) + std::abs(y[1] - x[1]) + std::abs(y[2] - x[2]);
+  const double scale = std::abs(x[0]) + std::abs(x[1]) + std::abs(x[2]);
+  return small >= 2 && res <= 1e-12 * scale;
+}
+
+static Geo make_geo(const Mesh& m) {
+  Geo g;
+  g.cg.resize(m.ncell);
+  g.fg.resize((size_t)m.ncell * 6);
+  bool bad = false, bad_newton = false;
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < m.ncell; ++c) {
+    const double* X = &m.X[(size_t)c * 81];
+    CellGeo& cg = g.cg[c];
+    for (int q2_ = 0; q2_ < 3; ++q2_)
+      for (int q1_ = 0; q1_ < 3; ++q1_)
+        for (int q0 = 0; q0 < 3; ++q0) {
+          const int q = q0 + 3 * q1_ + 9 * q2_;
+          double xi[3] = {G3.x[q0], G3.x[q1_], G3.x[q2_]}, J[3][3];
+          map_point(X, xi, cg.x[q], J);
+          double det = invert3(J, cg.Ji[q]);
+          if (!(det > 0)) bad = true;
+          cg.JxW[q] = det * G3.w[q0] * G3.w[q1_] * G3.w[q2_];
+        }
+    for (int f = 0; f < 6; ++f) {
+      FaceGeo& fg = g.fg[(size_t)c * 6 + f];
+      const int e = f / 2;
+      const double sg = (f & 1) ? 1.0 : -1.0;
+      for (int gt = 0; gt < 3; ++gt)
+        for (int gs = 0; gs < 3; ++gs) {
+          const int s = gs + 3 * gt;
+          face_ref(f, G3.x[gs], G3.x[gt], fg.xi[s]);
+          double J[3][3], Ji[3][3];
+          map_point(X, fg.xi[s], fg.x[s], J);
+          double det = invert3(J, Ji);
+          double nv[3];
+          for (int d = 0; d < 3; ++d) nv[d] = sg * det * Ji[e][d];
+          double a = std::sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+          for (int d = 0; d < 3; ++d) fg.n[s][d] = nv[d] / a;
+          fg.JxW[s] = a * G3.w[gs] * G3.w[gt];
+          const int cn = m.nb[c * 6 + f];
+          if (cn >= 0 && !newton_inverse(&m.X[(size_t)cn * 81], fg.x[s], m.nbf[c * 6 + f], fg.xinb[s]))
+            bad_newton = true;
+        }
+    }
+  }
+  if (bad) throw std::runtime_error("shell: non-positive Jacobian");
+  if (bad_newton) throw std::runtime_error("shell: Newton inverse of a neighbour map did not converge");
+  return g;
+}
+
+// ---------------------------------------------------------------- fields
+// a field block is [comp][cell][8], dof i + 2j + 4k at the vertex (i, j, k)
+static inline double eval(const double* u, const double xi[3]) {
+  double v = 0;
+  for (int k = 0; k < 2; ++k)
+    for (int j = 0; j < 2; ++j)
+      for (int i = 0; i < 2; ++i) v += u[i + 2 * j + 4 * k] * q1(i, xi[0]) * q1(j, xi[1]) * q1(k, xi[2]);
+  return v;
+}
+static inline double phi(int dof, const double xi[3]) {
+  return q1(dof & 1, xi[0]) * q1((dof >> 1) & 1, xi[1]) * q1(dof >> 2, xi[2]);
+}
+static inline void dphi(int dof, const double xi[3], double g[3]) {
+  const int i = dof & 1, j = (dof >> 1) & 1, k = dof >> 2;
+  g[0] = dq1(i) * q1(j, xi[1]) * q1(k, xi[2]);
+  g[1] = q1(i, xi[0]) * dq1(j) * q1(k, xi[2]);
+  g[2] = q1(i, xi[0]) * q1(j, xi[1]) * dq1(k);
+}
+static inline void cell_xi(int q, double xi[3]) {
+  xi[0] = G3.x[q % 3];
+  xi[1] = G3.x[(q / 3) % 3];
+  xi[2] = G3.x[q / 9];
+}
+
+// ------------------------------------------------------------ operator
+struct Params {
+  double Ma = 1, Fr = 1, dt = 1;
+  double gamma_gas = 1.4;  // EquationData::Cp_Cv (equation_data.h:24)
+  double eps = 1e-10;      // Data solve / eps
+  int max_its = 30000;     // Data solve / max_iterations
+  int fp_max_iter = 100;   // Atmospheric_Flow.cc:1135
+  double fp_tol = 1e-10;   // Atmospheric_Flow.cc:1149
+  int fp_lock = 0;         // >0: exactly this many fixed-point iterations
+  int gmres_restart = 28;  // deal.II SolverGMRES default (30 temporary vectors)
+};
+
+struct Tableau {
+  double g, a21, a22, a31, a32, a33, a21t, a22t, a31t, a32t, a33t, b1, b2, b3;
+  Tableau() {
+    // euler_operator.h:220-224
+    g = 2.0 - std::sqrt(2.0);
+    a21 = g;
+    a22 = 0;
+    a31 = 0.5;
+    a32 = 0.5;
+    a33 = 0;
+    a21t = 0.5 * g;
+    a22t = 0.5 * g;
+    a31t = std::sqrt(2.0) / 4.0;
+    a32t = std::sqrt(2.0) / 4.0;
+    a33t = 1.0 - std::sqrt(2.0) / 2.0;
+    b1 = a31t;
+    b2 = a32t;
+    b3 = a33t;
+  }
+};
+
+// Per-point gravity data: unit direction e_r and factor a^2/r^2 (radial),
+// or e_z and 1 (box).
+static inline void gravity(const Mesh& m, const double x[3], double er[3], double& G) {
+  if (!m.radial) {
+    er[0] = 0;
+    er[1] = 0;
+    er[2] = 1;
+    G = 1;
+    return;
+  }
+  const double r = std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+  er[0] = x[0] / r;
+  er[1] = x[1] / r;
+  er[2] = x[2] / r;
+  G = m.planet * m.planet / (r * r);
+}
+
+static inline double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+// EULEROperator restatement in dim = 3 (euler_operator.h:24-211)
+struct Op3 {
+  const Mesh& m;
+  const Geo& g;
+  Params P;
+  Tableau tb;
+  int hs = 1;  // HYPERBOLIC_stage
+  int ns = 1;  // NS_stage
+  int Ns;
+  Vec rho_for_fixed, pres_fixed;
+  Op3(const Mesh& mesh, const Geo& geo) : m(mesh), g(geo), Ns(mesh.ncell * 8) {}
+
+  double bc_coeff() const { return hs == 1 ? tb.a22t : tb.a33t; }
+
+  // values of a block (ncomp components) at the cell's quadrature points
+  void cell_vals(const double* f, int ncomp, int c, double out[][27]) const {
+    for (int comp = 0; comp < ncomp; ++comp)
+      for (int q = 0; q < 27; ++q) {
+        double xi[3];
+        cell_xi(q, xi);
+        out[comp][q] = eval(f + (size_t)comp * Ns + (size_t)c * 8, xi);
+      }
+  }
+  // trace of a block on face f of cell c at its 9 points (own side) and the neighbour's
+  void face_vals(const double* f, int ncomp, int c, int face, double own[][9], double nbv[][9]) const {
+    const FaceGeo& fg = g.fg[(size_t)c * 6 + face];
+    const int cn = m.nb[c * 6 + face];
+    for (int comp = 0; comp < ncomp; ++comp)
+      for (int s = 0; s < 9; ++s) {
+        own[comp][s] = eval(f + (size_t)comp * Ns + (size_t)c * 8, fg.xi[s]);
+        if (cn >= 0) nbv[comp][s] = eval(f + (size_t)comp * Ns + (size_t)cn * 8, fg.xinb[s]);
+      }
+  }
+  // out[i] += sum_q V phi_i + F . grad phi_i (F physical, JxW included here)
+  void integrate_cell(int c, const double* V, const double (*F)[3], double* out) const {
+    const CellGeo& cg = g.cg[c];
+    for (int i = 0; i < 8; ++i) {
+      double acc = 0;
+      for (int q = 0; q < 27; ++q) {
+        double xi[3], gr[3];
+        cell_xi(q, xi);
+        double t = 0;
+        if (V) t += V[q] * phi(i, xi);
+        if (F) {
+          dphi(i, xi, gr);
+          for (int d = 0; d < 3; ++d) {
+            double gd = cg.Ji[q][0][d] * gr[0] + cg.Ji[q][1][d] * gr[1] + cg.Ji[q][2][d] * gr[2];
+            t += F[q][d] * gd;
+          }
+        }
+        acc += t * cg.JxW[q];
+      }
+      out[i] += acc;
+    }
+  }
+  // out[i] += sum_s val_s phi_i(face point s) JxW_s
+  void integrate_face(int c, int face, const double* val, double* out) const {
+    const FaceGeo& fg = g.fg[(size_t)c * 6 + face];
+    for (int i = 0; i < 8; ++i) {
+      double acc = 0;
+      for (int s = 0; s < 9; ++s) acc += val[s] * phi(i, fg.xi[s]) * fg.JxW[s];
+      out[i] += acc;
+    }
+  }
+
+  // ---------------------------------------------------- density rhs
+  // euler_operator.h:344-473 (cell), :476-692 (face); boundary empty (:111-115)
+  void rhs_rho(double* dst, const double* const* src, int nsrc) const {
+    const int nst = hs;
+    if (nsrc != 2 * nst) throw std::runtime_error("rhs_rho: wrong number of sources");
+    double w[3];
+    if (nst == 1) { w[0] = tb.a21; }
+    else if (nst == 2) { w[0] = tb.a31; w[1] = tb.a32; }
+    else { w[0] = tb.b1; w[1] = tb.b2; w[2] = tb.b3; }
+    const double dt = P.dt;
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < m.ncell; ++c) {
+      double* o = dst + (size_t)c * 8;
+      std::fill(o, o + 8, 0.0);
+      double r[3][1][27], u[3][3][27];
+      for (int s = 0; s < nst; ++s) {
+        cell_vals(src[2 * s], 1, c, r[s]);
+        cell_vals(src[2 * s + 1], 3, c, u[s]);
+      }
+      double V[27], F[27][3];
+      for (int q = 0; q < 27; ++q) {
+        V[q] = r[0][0][q];
+        for (int d = 0; d < 3; ++d) {
+          F[q][d] = 0;
+          for (int s = 0; s < nst; ++s) F[q][d] += w[s] * dt * r[s][0][q] * u[s][d][q];
+        }
+      }
+      integrate_cell(c, V, F, o);
+      for (int f = 0; f < 6; ++f) {
+        if (m.nb[c * 6 + f] < 0) continue;
+        const FaceGeo& fg = g.fg[(size_t)c * 6 + f];
+        double rp[3][1][9], rm[3][1][9], up[3][3][9], um[3][3][9];
+        for (int s = 0; s < nst; ++s) {
+          face_vals(src[2 * s], 1, c, f, rp[s], rm[s]);
+          face_vals(src[2 * s + 1], 3, c, f, up[s], um[s]);
+        }
+        double val[9];
+        for (int q = 0; q < 9; ++q) {
+          const double* n = fg.n[q];
+          double acc = 0;
+          for (int s = 0; s < nst; ++s) {
+            double fn = 0, unp = 0, unm = 0;
+            for (int d = 0; d < 3; ++d) {
+              fn += 0.5 * (rp[s][0][q] * up[s][d][q] + rm[s][0][q] * um[s][d][q]) * n[d];
+              unp += up[s][d][q] * n[d];
+              unm += um[s][d][q] * n[d];
+            }
+            double lam = std::max(std::abs(unp), std::abs(unm));
+            acc += w[s] * dt * (fn + 0.5 * lam * (rp[s][0][q] - rm[s][0][q]));
+          }
+          val[q] = -acc;
+        }
+        integrate_face(c, f, val, o);
+      }
+    }
+  }
+
+  // ---------------------------------------------------- pressure rhs
+  // euler_operator.h:745-1007 (cell), :1012-1361 (face); boundary empty
+  void rhs_pressure(double* dst, const double* const* src, int nsrc) const {
+    const int st = hs;
+    const int need = st == 1 ? 5 : (st == 2 ? 8 : 11);
+    if (nsrc != need) throw std::runtime_error("rhs_pressure: wrong number of sources");
+    const double gm = P.gamma_gas, Ma2 = P.Ma * P.Ma, dt = P.dt, iFr2 = 1.0 / (P.Fr * P.Fr);
+    const int nex = st;
+    double wa[3], wt[3];
+    if (st == 1) { wa[0] = tb.a21; wt[0] = tb.a21t; }
+    else if (st == 2) { wa[0] = tb.a31; wt[0] = tb.a31t; wa[1] = tb.a32; wt[1] = tb.a32t; }
+    else { wa[0] = tb.b1; wt[0] = tb.b1; wa[1] = tb.b2; wt[1] = tb.b2; wa[2] = tb.b3; wt[2] = tb.b3; }
+    // gravity work terms: weight, triple or kinetic slot
+    double gwt[3];
+    int gsl[3], ngr;
+    if (st == 1) { gwt[0] = tb.a21t; gsl[0] = 0; gwt[1] = tb.a22t; gsl[1] = -1; ngr = 2; }
+    else if (st == 2) { gwt[0] = tb.a31t; gsl[0] = 0; gwt[1] = tb.a32t; gsl[1] = 1; gwt[2] = tb.a33t; gsl[2] = -1; ngr = 3; }
+    else { gwt[0] = tb.b1; gsl[0] = 0; gwt[1] = tb.b2; gsl[1] = 1; gwt[2] = tb.b3; gsl[2] = 2; ngr = 3; }
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < m.ncell; ++c) {
+      double* o = dst + (size_t)c * 8;
+      std::fill(o, o + 8, 0.0);
+      const CellGeo& cg = g.cg[c];
+      double r[3][1][27], u[3][3][27], p[3][1][27], rk[1][27], uk[3][27];
+      for (int s = 0; s < nex; ++s) {
+        cell_vals(src[3 * s], 1, c, r[s]);
+        cell_vals(src[3 * s + 1], 3, c, u[s]);
+        cell_vals(src[3 * s + 2], 1, c, p[s]);
+      }
+      cell_vals(src[3 * nex], 1, c, rk);
+      cell_vals(src[3 * nex + 1], 3, c, uk);
+      double V[27], F[27][3];
+      for (int q = 0; q < 27; ++q) {
+        double er[3], G;
+        gravity(m, cg.x[q], er, G);
+        double E[3], uu[3];
+        for (int s = 0; s < nex; ++s) {
+          uu[s] = u[s][0][q] * u[s][0][q] + u[s][1][q] * u[s][1][q] + u[s][2][q] * u[s][2][q];
+          E[s] = 1.0 / (gm - 1.0) * (p[s][0][q] / r[s][0][q]) + 0.5 * Ma2 * uu[s];
+        }
+        const double ukk = uk[0][q] * uk[0][q] + uk[1][q] * uk[1][q] + uk[2][q] * uk[2][q];
+        double val = r[0][0][q] * E[0] - 0.5 * rk[0][q] * Ma2 * ukk;
+        for (int t = 0; t < ngr; ++t) {
+          const int sl = gsl[t];
+          const double rr = sl < 0 ? rk[0][q] : r[sl][0][q];
+          const double ue = sl < 0 ? uk[0][q] * er[0] + uk[1][q] * er[1] + uk[2][q] * er[2]
+                                   : u[sl][0][q] * er[0] + u[sl][1][q] * er[1] + u[sl][2][q] * er[2];
+          val -= gwt[t] * dt * Ma2 * iFr2 * rr * ue * G;
+        }
+        V[q] = val;
+        for (int d = 0; d < 3; ++d) F[q][d] = 0;
+        for (int s = 0; s < nex; ++s) {
+          const double kin = 0.5 * wa[s] * dt * Ma2 * uu[s] * r[s][0][q];
+          const double ent = wt[s] * dt * (r[s][0][q] * (E[s] - 0.5 * Ma2 * uu[s]) + p[s][0][q]);
+          for (int d = 0; d < 3; ++d) F[q][d] += (kin + ent) * u[s][d][q];
+        }
+      }
+      integrate_cell(c, V, F, o);
+      for (int f = 0; f < 6; ++f) {
+        if (m.nb[c * 6 + f] < 0) continue;
+        const FaceGeo& fg = g.fg[(size_t)c * 6 + f];
+        double rp[3][1][9], rm[3][1][9], up[3][3][9], um[3][3][9], pp[3][1][9], pm[3][1][9];
+        for (int s = 0; s < nex; ++s) {
+          face_vals(src[3 * s], 1, c, f, rp[s], rm[s]);
+          face_vals(src[3 * s + 1], 3, c, f, up[s], um[s]);
+          face_vals(src[3 * s + 2], 1, c, f, pp[s], pm[s]);
+        }
+        double val[9];
+        for (int q = 0; q < 9; ++q) {
+          const double* n = fg.n[q];
+          double acc = 0;
+          for (int s = 0; s < nex; ++s) {
+            double uP[3] = {up[s][0][q], up[s][1][q], up[s][2][q]}, uM[3] = {um[s][0][q], um[s][1][q], um[s][2][q]};
+            const double rP = rp[s][0][q], rM = rm[s][0][q], pP = pp[s][0][q], pM = pm[s][0][q];
+            const double upp = dot3(uP, uP), umm = dot3(uM, uM);
+            const double unP = dot3(uP, n), unM = dot3(uM, n);
+            const double akn = 0.5 * (0.5 * upp * rP * unP + 0.5 * umm * rM * unM);
+            const double Ep = 1.0 / (gm - 1.0) * pP / rP + 0.5 * Ma2 * upp;
+            const double Em = 1.0 / (gm - 1.0) * pM / rM + 0.5 * Ma2 * umm;
+            const double hp = (Ep - 0.5 * Ma2 * upp) * rP + pP;
+            const double hm = (Em - 0.5 * Ma2 * umm) * rM + pM;
+            const double aen = 0.5 * (hp * unP + hm * unM);
+            const double lam = std::max(std::abs(unP), std::abs(unM));
+            if (st < 3) {
+              double jkin;
+              if (st == 2 && s == 1) {
+                // euler_operator.h:1182-1183 (u_tmp_2 . u_old, reproduced as written)
+                double u0P[3] = {up[0][0][q], up[0][1][q], up[0][2][q]};
+                double u0M[3] = {um[0][0][q], um[0][1][q], um[0][2][q]};
+                jkin = rP * 0.5 * dot3(uP, u0P) - rM * 0.5 * dot3(uM, u0M);
+              } else {
+                jkin = rP * 0.5 * upp - rM * 0.5 * umm;
+              }
+              const double je = rP * (Ep - 0.5 * Ma2 * upp) - rM * (Em - 0.5 * Ma2 * umm);
+              acc += -wa[s] * dt * Ma2 * (akn + 0.5 * lam * jkin) - wt[s] * dt * (aen + 0.5 * lam * je);
+            } else {
+              const double jrE = rP * Ep - rM * Em;
+              acc += -wa[s] * dt * Ma2 * akn - wa[s] * dt * aen - wa[s] * dt * 0.5 * lam * jrE;
+            }
+          }
+          val[q] = acc;
+        }
+        integrate_face(c, f, val, o);
+      }
+    }
+  }
+
+  // ---------------------------------------------------- velocity rhs
+  // euler_operator.h:1620-1886 (cell), :1889-2184 (face), :2187-2305 (boundary)
+  void rhs_velocity(double* dst, const double* const* src, int nsrc) const {
+    const int st = hs;
+    const int need = st == 1 ? 4 : (st == 2 ? 7 : 9);
+    if (nsrc != need) throw std::runtime_error("rhs_velocity: wrong number of sources");
+    const double Ma2 = P.Ma * P.Ma, dt = P.dt, iFr2 = 1.0 / (P.Fr * P.Fr);
+    const int nex = st;
+    double wa[3], wt[3];
+    if (st == 1) { wa[0] = tb.a21; wt[0] = tb.a21t; }
+    else if (st == 2) { wa[0] = tb.a31; wt[0] = tb.a31t; wa[1] = tb.a32; wt[1] = tb.a32t; }
+    else { wa[0] = tb.b1; wt[0] = tb.b1; wa[1] = tb.b2; wt[1] = tb.b2; wa[2] = tb.b3; wt[2] = tb.b3; }
+    int ng;
+    double gw[3];
+    const double* gr[3];
+    if (st == 1) { gw[0] = tb.a21t; gr[0] = src[0]; gw[1] = tb.a22t; gr[1] = src[3]; ng = 2; }
+    else if (st == 2) { gw[0] = tb.a31t; gr[0] = src[0]; gw[1] = tb.a32t; gr[1] = src[3]; gw[2] = tb.a33t; gr[2] = src[6]; ng = 3; }
+    else { gw[0] = tb.b1; gr[0] = src[0]; gw[1] = tb.b2; gr[1] = src[3]; gw[2] = tb.b3; gr[2] = src[6]; ng = 3; }
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < m.ncell; ++c) {
+      const CellGeo& cg = g.cg[c];
+      double r[3][1][27], u[3][3][27], p[3][1][27], grv[3][1][27];
+      for (int s = 0; s < nex; ++s) {
+        cell_vals(src[3 * s], 1, c, r[s]);
+        cell_vals(src[3 * s + 1], 3, c, u[s]);
+        cell_vals(src[3 * s + 2], 1, c, p[s]);
+      }
+      for (int s = 0; s < ng; ++s) cell_vals(gr[s], 1, c, grv[s]);
+      double frp[6][3][1][9], frm[6][3][1][9], fup[6][3][3][9], fum[6][3][3][9], fpp[6][3][1][9], fpm[6][3][1][9];
+      for (int f = 0; f < 6; ++f)
+        for (int s = 0; s < nex; ++s) {
+          face_vals(src[3 * s], 1, c, f, frp[f][s], frm[f][s]);
+          face_vals(src[3 * s + 1], 3, c, f, fup[f][s], fum[f][s]);
+          face_vals(src[3 * s + 2], 1, c, f, fpp[f][s], fpm[f][s]);
+        }
+      for (int comp = 0; comp < 3; ++comp) {
+        double* o = dst + (size_t)comp * Ns + (size_t)c * 8;
+        std::fill(o, o + 8, 0.0);
+        double V[27], F[27][3];
+        for (int q = 0; q < 27; ++q) {
+          double er[3], G;
+          gravity(m, cg.x[q], er, G);
+          double val = r[0][0][q] * u[0][comp][q];
+          for (int s = 0; s < ng; ++s) val -= gw[s] * dt * iFr2 * grv[s][0][q] * er[comp] * G;
+          V[q] = val;
+          for (int d = 0; d < 3; ++d) {
+            F[q][d] = 0;
+            for (int s = 0; s < nex; ++s)
+              F[q][d] += wa[s] * dt * r[s][0][q] * u[s][comp][q] * u[s][d][q] + (d == comp ? wt[s] * dt / Ma2 * p[s][0][q] : 0.0);
+          }
+        }
+        integrate_cell(c, V, F, o);
+        for (int f = 0; f < 6; ++f) {
+          const FaceGeo& fg = g.fg[(size_t)c * 6 + f];
+          double val[9];
+          if (m.nb[c * 6 + f] < 0) {
+            for (int q = 0; q < 9; ++q) {
+              double acc = 0;
+              for (int s = 0; s < nex; ++s) {
+                const double pP = fpp[f][s][0][q], pD = pP;
+                acc += -wt[s] * dt / Ma2 * 0.5 * (pP + pD) * fg.n[q][comp];
+              }
+              val[q] = acc;
+            }
+          } else {
+            for (int q = 0; q < 9; ++q) {
+              const double* n = fg.n[q];
+              double acc = 0;
+              for (int s = 0; s < nex; ++s) {
+                double uP[3] = {fup[f][s][0][q], fup[f][s][1][q], fup[f][s][2][q]};
+                double uM[3] = {fum[f][s][0][q], fum[f][s][1][q], fum[f][s][2][q]};
+                const double rP = frp[f][s][0][q], rM = frm[f][s][0][q];
+                const double unP = dot3(uP, n), unM = dot3(uM, n);
+                const double tn = 0.5 * (rP * uP[comp] * unP + rM * uM[comp] * unM);
+                const double avgp = 0.5 * (fpp[f][s][0][q] + fpm[f][s][0][q]);
+                const double jr = rP * uP[comp] - rM * uM[comp];
+                const double lam = std::max(std::abs(unP), std::abs(unM));
+                acc += -wa[s] * dt * tn - wt[s] * dt / Ma2 * avgp * n[comp] - wa[s] * dt * 0.5 * lam * jr;
+              }
+              val[q] = acc;
+            }
+          }
+          integrate_face(c, f, val, o);
+        }
+      }
+    }
+  }
+
+  // B: euler_operator.h:1510-1617, vmult_pressure :2420-2435
+  void vmult_pressure(double* dst, const double* src) const {
+    const double cf = bc_coeff() * P.dt / (P.Ma * P.Ma);
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < m.ncell; ++c) {
+      double pv[1][27];
+      cell_vals(src, 1, c, pv);
+      double fpo[6][1][9], fpn[6][1][9];
+      for (int f = 0; f < 6; ++f) face_vals(src, 1, c, f, fpo[f], fpn[f]);
+      for (int comp = 0; comp < 3; ++comp) {
+        double* o = dst + (size_t)comp * Ns + (size_t)c * 8;
+        std::fill(o, o + 8, 0.0);
+        double F[27][3];
+        for (int q = 0; q < 27; ++q)
+          for (int d = 0; d < 3; ++d) F[q][d] = d == comp ? -cf * pv[0][q] : 0.0;
+        integrate_cell(c, nullptr, F, o);
+        for (int f = 0; f < 6; ++f) {
+          const FaceGeo& fg = g.fg[(size_t)c * 6 + f];
+          const bool wall = m.nb[c * 6 + f] < 0;
+          double val[9];
+          for (int q = 0; q < 9; ++q) {
+            const double pP = fpo[f][0][q], pM = wall ? pP : fpn[f][0][q];
+            val[q] = cf * 0.5 * (pP + pM) * fg.n[q][comp];
+          }
+          integrate_face(c, f, val, o);
+        }
+      }
+    }
+  }
+
+  // C: euler_operator.h:1411-1507, vmult_enthalpy :2438-2453; boundary empty
+  void vmult_enthalpy(double* dst, const double* src) const {
+    const double gm = P.gamma_gas, coeff = bc_coeff();
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < m.ncell; ++c) {
+      double* o = dst + (size_t)c * 8;
+      std::fill(o, o + 8, 0.0);
+      double pf[1][27], u[3][27];
+      cell_vals(pres_fixed.data(), 1, c, pf);
+      cell_vals(src, 3, c, u);
+      double F[27][3];
+      for (int q = 0; q < 27; ++q) {
+        const double s = -coeff * P.dt * gm / (gm - 1.0) * pf[0][q];
+        for (int d = 0; d < 3; ++d) F[q][d] = s * u[d][q];
+      }
+      integrate_cell(c, nullptr, F, o);
+      for (int f = 0; f < 6; ++f) {
+        if (m.nb[c * 6 + f] < 0) continue;
+        const FaceGeo& fg = g.fg[(size_t)c * 6 + f];
+        double pp[1][9], pm[1][9], up[3][9], um[3][9];
+        face_vals(pres_fixed.data(), 1, c, f, pp, pm);
+        face_vals(src, 3, c, f, up, um);
+        double val[9];
+        for (int q = 0; q < 9; ++q) {
+          const double* n = fg.n[q];
+          double uP[3] = {up[0][q], up[1][q], up[2][q]}, uM[3] = {um[0][q], um[1][q], um[2][q]};
+          const double unP = dot3(uP, n), unM = dot3(uM, n);
+          const double an = 0.5 * gm / (gm - 1.0) * (pp[0][q] * unP + pm[0][q] * unM);
+          const double lam = std::max(std::abs(unP), std::abs(unM));
+          const double jump = 1.0 / (gm - 1.0) * (pp[0][q] - pm[0][q]);
+          val[q] = coeff * P.dt * (an + 0.5 * lam * jump);
+        }
+        integrate_face(c, f, val, o);
+      }
+    }
+  }
+
+  // dense cell block of the mass-type matrix: weight(q) phi_i phi_j JxW
+  void cell_block(int c, const double* w27, double B[8][8]) const {
+    const CellGeo& cg = g.cg[c];
+    for (int i = 0; i < 8; ++i)
+      for (int j = 0; j < 8; ++j) B[i][j] = 0;
+    for (int q = 0; q < 27; ++q) {
+      double xi[3], ph[8];
+      cell_xi(q, xi);
+      for (int i = 0; i < 8; ++i) ph[i] = phi(i, xi);
+      const double wq = (w27 ? w27[q] : 1.0) * cg.JxW[q];
+      for (int i = 0; i < 8; ++i)
+        for (int j = 0; j < 8; ++j) B[i][j] += wq * ph[i] * ph[j];
+    }
+  }
+  // block of the mass-type operator of NS_stage ns (1/4 M_rho, 3/5 A, 2/6 D)
+  void block(int ns_, int c, double B[8][8]) const {
+    if (ns_ == 3 || ns_ == 5) {
+      double r[1][27];
+      cell_vals(rho_for_fixed.data(), 1, c, r);
+      cell_block(c, r[0], B);
+    } else {
+      cell_block(c, nullptr, B);
+      if (ns_ == 2 || ns_ == 6)
+        for (int i = 0; i < 8; ++i)
+          for (int j = 0; j < 8; ++j) B[i][j] /= (P.gamma_gas - 1.0);
+    }
+  }
+  int ncomp(int ns_) const { return (ns_ == 3 || ns_ == 5) ? 3 : 1; }
+
+  // dst = block(ns) src (mass-type blocks, euler_operator.h:717-740, 2328-2357)
+  void apply_block(int ns_, double* dst, const double* src) const {
+    const int nc = ncomp(ns_);
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < m.ncell; ++c) {
+      double B[8][8];
+      block(ns_, c, B);
+      for (int comp = 0; comp < nc; ++comp) {
+        const double* s = src + (size_t)comp * Ns + (size_t)c * 8;
+        double* o = dst + (size_t)comp * Ns + (size_t)c * 8;
+        for (int i = 0; i < 8; ++i) {
+          double acc = 0;
+          for (int j = 0; j < 8; ++j) acc += B[i][j] * s[j];
+          o[i] = acc;
+        }
+      }
+    }
+  }
+  // x = block(ns)^{-1} b by dense Cholesky per cell
+  void solve_block(int ns_, double* x, const double* b) const {
+    const int nc = ncomp(ns_);
+    bool bad = false;
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < m.ncell; ++c) {
+      double B[8][8], Lc[8][8] = {{0}};
+      block(ns_, c, B);
+      for (int j = 0; j < 8; ++j) {
+        double s = B[j][j];
+        for (int k = 0; k < j; ++k) s -= Lc[j][k] * Lc[j][k];
+        if (!(s > 0)) {
+          bad = true;
+          s = 1;
+        }
+        Lc[j][j] = std::sqrt(s);
+        for (int i = j + 1; i < 8; ++i) {
+          double t = B[i][j];
+          for (int k = 0; k < j; ++k) t -= Lc[i][k] * Lc[j][k];
+          Lc[i][j] = t / Lc[j][j];
+        }
+      }
+      for (int comp = 0; comp < nc; ++comp) {
+        const double* bb = b + (size_t)comp * Ns + (size_t)c * 8;
+        double* xx = x + (size_t)comp * Ns + (size_t)c * 8;
+        double y[8];
+        for (int i = 0; i < 8; ++i) {
+          double t = bb[i];
+          for (int k = 0; k < i; ++k) t -= Lc[i][k] * y[k];
+          y[i] = t / Lc[i][i];
+        }
+        for (int i = 7; i >= 0; --i) {
+          double t = y[i];
+          for (int k = i + 1; k < 8; ++k) t -= Lc[k][i] * xx[k];
+          xx[i] = t / Lc[i][i];
+        }
+      }
+    }
+    if (bad) throw std::runtime_error("shell: mass-type block not positive definite");
+  }
+  // inverse diagonal of the block of NS_stage ns (compute_diagonal, :2589-2626; D for ns 2)
+  void inverse_diagonal(int ns_, double* out) const {
+    const int nc = ncomp(ns_);
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < m.ncell; ++c) {
+      double B[8][8];
+      block(ns_, c, B);
+      for (int comp = 0; comp < nc; ++comp)
+        for (int i = 0; i < 8; ++i) out[(size_t)comp * Ns + (size_t)c * 8 + i] = 1.0 / B[i][i];
+    }
+  }
+  // Schur complement S p = D p - C A^{-1} B p (apply_add NS 2, euler_operator.h:2374-2408)
+  void schur(double* dst, const double* src) const {
+    Vec t1((size_t)3 * Ns), t2((size_t)3 * Ns), t3(Ns);
+    vmult_pressure(t1.data(), src);
+    solve_block(3, t2.data(), t1.data());
+    vmult_enthalpy(t3.data(), t2.data());
+    apply_block(6, dst, src);
+    for (int i = 0; i < Ns; ++i) dst[i] -= t3[i];
+  }
+};
+
+// ---------------------------------------------------------------- solvers
+static double dotv(const Vec& a, const Vec& b) {
+  double s = 0;
+#pragma omp parallel for reduction(+ : s) schedule(static)
+  for (long i = 0; i < (long)a.size(); ++i) s += a[i] * b[i];
+  return s;
+}
+static double norm2(const Vec& a) { return std::sqrt(dotv(a, a)); }
+
+struct NoConvergence : std::runtime_error {
+  NoConvergence(const std::string& s) : std::runtime_error(s) {}
+};
+
+// deal.II SolverGMRES: left preconditioning, convergence of the
+// preconditioned residual against tol, restart after `restart` iterations,
+// modified Gram-Schmidt with one more pass when the norm drops below
+// 10 sqrt(eps) of its start.
+typedef std::function<void(Vec&, const Vec&)> OpF;
+static int gmres(const OpF& A, const Vec& Pd, Vec& x, const Vec& b, double tol, int max_its, int restart) {
+  const size_t n = b.size();
+  std::vector<Vec> V(restart + 1, Vec(n));
+  std::vector<double> H((size_t)(restart + 1) * restart), gam(restart + 1), cs(restart), sn(restart);
+  Vec p(n);
+  int acc = 0, st = 0;
+  auto check = [&](int step, double res) {
+    if (res <= tol) return 1;
+    if (step >= max_its || std::isnan(res)) return -1;
+    return 0;
+  };
+  while (true) {
+    A(p, x);
+    for (size_t i = 0; i < n; ++i) V[0][i] = Pd[i] * (b[i] - p[i]);
+    double rho = norm2(V[0]);
+    st = check(acc, rho);
+    if (st != 0) break;
+    std::fill(gam.begin(), gam.end(), 0.0);
+    gam[0] = rho;
+    for (size_t i = 0; i < n; ++i) V[0][i] /= rho;
+    int dim = 0;
+    for (int j = 0; j < restart && st == 0; ++j) {
+      ++acc;
+      dim = j + 1;
+      A(p, V[j]);
+      Vec& w = V[j + 1];
+      for (size_t i = 0; i < n; ++i) w[i] = Pd[i] * p[i];
+      double* h = &H[(size_t)j * (restart + 1)];
+      const double n0 = norm2(w);
+      for (int i = 0; i <= j; ++i) {
+        h[i] = dotv(w, V[i]);
+        for (size_t l = 0; l < n; ++l) w[l] -= h[i] * V[i][l];
+      }
+      double nw = norm2(w);
+      if (nw < 10.0 * std::sqrt(2.220446049250313e-16) * n0) {
+        for (int i = 0; i <= j; ++i) {
+          const double c2 = dotv(w, V[i]);
+          h[i] += c2;
+          for (size_t l = 0; l < n; ++l) w[l] -= c2 * V[i][l];
+        }
+        nw = norm2(w);
+      }
+      h[j + 1] = nw;
+      if (nw != 0.0)
+        for (size_t l = 0; l < n; ++l) w[l] /= nw;
+      for (int i = 0; i < j; ++i) {
+        const double t = h[i];
+        h[i] = cs[i] * t + sn[i] * h[i + 1];
+        h[i + 1] = -sn[i] * t + cs[i] * h[i + 1];
+      }
+      const double r = 1.0 / std::sqrt(h[j] * h[j] + h[j + 1] * h[j + 1]);
+      sn[j] = h[j + 1] * r;
+      cs[j] = h[j] * r;
+      h[j] = cs[j] * h[j] + sn[j] * h[j + 1];
+      gam[j + 1] = -sn[j] * gam[j];
+      gam[j] *= cs[j];
+      rho = std::abs(gam[j + 1]);
+      st = check(acc, rho);
+    }
+    std::vector<double> y(dim);
+    for (int i = dim - 1; i >= 0; --i) {
+      double s = gam[i];
+      for (int l = i + 1; l < dim; ++l) s -= H[(size_t)l * (restart + 1) + i] * y[l];
+      y[i] = s / H[(size_t)i * (restart + 1) + i];
+    }
+    for (int i = 0; i < dim; ++i)
+      for (size_t l = 0; l < n; ++l) x[l] += y[i] * V[i][l];
+    if (st != 0) break;
+  }
+  if (st == -1) throw NoConvergence("gmres: no convergence");
+  return acc;
+}
+
+// ---------------------------------------------------------------- solver
+struct Solver {
+  Mesh m;
+  Geo g;
+  Op3 op;
+  long gmres_its = 0, fp_its = 0;
+  Vec rho_old, rho2, rho3, rho_curr, u_old, u2, u3, u_curr, u_fixed, p_old, p2, p3, p_fixed, p_fixed_old, tmp1;
+  Vec rhs_u, rhs_p;
+
+  explicit Solver(Mesh mesh) : m(std::move(mesh)), g(make_geo(m)), op(m, g) {}
+  int Ns() const { return op.Ns; }
+
+  void init_state(const double* rho, const double* u, const double* p) {
+    const int N = Ns();
+    rho_old.assign(rho, rho + N);
+    u_old.assign(u, u + 3 * N);
+    p_old.assign(p, p + N);
+    for (Vec* v : {&rho2, &rho3, &rho_curr, &p2, &p3, &p_fixed, &p_fixed_old}) v->assign(N, 0);
+    for (Vec* v : {&u2, &u3, &u_curr, &u_fixed, &tmp1}) v->assign(3 * N, 0);
+  }
+
+  int solve_schur(Vec& x, const Vec& b, double tol) {
+    op.ns = 2;
+    Vec Pd(Ns());
+    op.inverse_diagonal(2, Pd.data());
+    OpF S = [&](Vec& d, const Vec& s) { op.schur(d.data(), s.data()); };
+    int its = gmres(S, Pd, x, b, tol, op.P.max_its, op.P.gmres_restart);
+    gmres_its += its;
+    return its;
+  }
+
+  // update_density (Atmospheric_Flow.cc:430-537)
+  void update_density(int hs) {
+    op.hs = hs;
+    Vec rhs(Ns());
+    if (hs == 1) {
+      const double* s[2] = {rho_old.data(), u_old.data()};
+      op.rhs_rho(rhs.data(), s, 2);
+      op.solve_block(1, rho2.data(), rhs.data());
+    } else if (hs == 2) {
+      const double* s[4] = {rho_old.data(), u_old.data(), rho2.data(), u2.data()};
+      op.rhs_rho(rhs.data(), s, 4);
+      op.solve_block(1, rho3.data(), rhs.data());
+    } else {
+      const double* s[6] = {rho_old.data(), u_old.data(), rho2.data(), u2.data(), rho3.data(), u3.data()};
+      op.rhs_rho(rhs.data(), s, 6);
+      op.solve_block(4, rho_curr.data(), rhs.data());
+    }
+  }
+
+  // pressure_fixed_point (Atmospheric_Flow.cc:544-673)
+  void pressure_fixed_point(int hs) {
+    op.hs = hs;
+    op.pres_fixed = p_fixed_old;
+    const int N = Ns();
+    rhs_p.assign(N, 0);
+    rhs_u.assign(3 * N, 0);
+    if (hs == 1) {
+      const double* sp[5] = {rho_old.data(), u_old.data(), p_old.data(), rho2.data(), u_fixed.data()};
+      op.rhs_pressure(rhs_p.data(), sp, 5);
+      const double* su[4] = {rho_old.data(), u_old.data(), p_old.data(), rho2.data()};
+      op.rhs_velocity(rhs_u.data(), su, 4);
+    } else {
+      const double* sp[8] = {rho_old.data(), u_old.data(), p_old.data(), rho2.data(), u2.data(), p2.data(),
+                             rho3.data(), u_fixed.data()};
+      op.rhs_pressure(rhs_p.data(), sp, 8);
+      const double* su[7] = {rho_old.data(), u_old.data(), p_old.data(), rho2.data(), u2.data(), p2.data(),
+                             rho3.data()};
+      op.rhs_velocity(rhs_u.data(), su, 7);
+    }
+    op.solve_block(3, tmp1.data(), rhs_u.data());
+    Vec t(N);
+    op.vmult_enthalpy(t.data(), tmp1.data());
+    for (int i = 0; i < N; ++i) rhs_p[i] -= t[i];
+    p_fixed = p_fixed_old;
+    solve_schur(p_fixed, rhs_p, op.P.eps * norm2(rhs_p));
+  }
+
+  // update_velocity (Atmospheric_Flow.cc:680-781)
+  void update_velocity(int hs) {
+    op.hs = hs;
+    const int N = Ns();
+    if (hs == 1 || hs == 2) {
+      Vec t(3 * (size_t)N);
+      op.vmult_pressure(t.data(), p_fixed.data());
+      for (int i = 0; i < 3 * N; ++i) rhs_u[i] -= t[i];
+      op.solve_block(3, u_fixed.data(), rhs_u.data());
+    } else {
+      rhs_u.assign(3 * (size_t)N, 0);
+      const double* su[9] = {rho_old.data(), u_old.data(), p_old.data(), rho2.data(), u2.data(), p2.data(),
+                             rho3.data(), u3.data(), p3.data()};
+      op.rhs_velocity(rhs_u.data(), su, 9);
+      op.solve_block(5, u_curr.data(), rhs_u.data());
+    }
+  }
+
+  // update_pressure (Atmospheric_Flow.cc:788-871)
+  void update_pressure() {
+    op.hs = 3;
+    rhs_p.assign(Ns(), 0);
+    const double* sp[11] = {rho_old.data(), u_old.data(), p_old.data(), rho2.data(), u2.data(), p2.data(),
+                            rho3.data(), u3.data(), p3.data(), rho_curr.data(), u_curr.data()};
+    op.rhs_pressure(rhs_p.data(), sp, 11);
+    op.solve_block(6, p_old.data(), rhs_p.data());
+  }
+
+  void fixed_point_loop(int hs) {
+    for (int iter = 0; iter < op.P.fp_max_iter; ++iter) {
+      pressure_fixed_point(hs);
+      update_velocity(hs);
+      ++fp_its;
+      double den = 0, num = 0;
+      for (size_t i = 0; i < p_fixed.size(); ++i) {
+        den = std::max(den, std::abs(p_fixed[i]));
+        num = std::max(num, std::abs(p_fixed[i] - p_fixed_old[i]));
+      }
+      const double err = num / (den + 1e-10);
+      if (op.P.fp_lock > 0) {
+        if (iter + 1 >= op.P.fp_lock) break;
+      } else if (err < op.P.fp_tol) {
+        break;
+      }
+      p_fixed_old = p_fixed;
+    }
+  }
+
+  // one IMEX step (Atmospheric_Flow.cc:1115-1222)
+  void step() {
+    update_density(1);
+    op.rho_for_fixed = rho2;
+    p_fixed_old = p_old;
+    u_fixed = u_old;
+    fixed_point_loop(1);
+    p2 = p_fixed;
+    u2 = u_fixed;
+    update_density(2);
+    op.rho_for_fixed = rho3;
+    p_fixed_old = p2;
+    u_fixed = u2;
+    fixed_point_loop(2);
+    p3 = p_fixed;
+    u3 = u_fixed;
+    update_density(3);
+    op.rho_for_fixed = rho_curr;
+    update_velocity(3);
+    update_pressure();
+    rho_old = rho_curr;
+    u_old = u_curr;
+  }
+};
+
+// ------------------------------------------------ analytic sphere state
+// equation_data.h:47-176 in the reference's non-dimensional variables
+// (rho / rho_ref, p / p_ref, u / u_ref with u_ref = L_ref = 1; rho_ref =
+// p0 / (R T_bar) and p_ref = p0, so both profiles are exp(q)); RT = R T_bar.
+struct SbrState {
+  double Omega, a, g, RT;
+  void at(const double x[3], double& rho, double u[3], double& p) const {
+    const double r = std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+    const double s2 = x[0] * x[0] + x[1] * x[1];
+    const double q = (0.5 * Omega * Omega * s2 - a * g * (1.0 - a / r)) / RT;
+    rho = std::exp(q);
+    p = std::exp(q);
+    u[0] = -Omega * x[1];
+    u[1] = Omega * x[0];
+    u[2] = 0.0;
+  }
+};
+
+}  // namespace shell_orc
+
+// ================================================================ C API
+using namespace shell_orc;
+
+static thread_local std::string g_err;
+
+extern "C" {
+
+const char* orc3_last_error() { return g_err.c_str(); }
+
+// kind 0: shell (n_ref refinements of np panels between r_in and r_out,
+// gravity factor a^2/r^2 with a = planet); kind 1: box nx x ny x nz over
+// Lx x Ly x H (pinning only). Returns NULL on error (orc3_last_error).
+void* orc3_create_shell(int n_ref, int np, double r_in, double r_out, double planet) {
+  try {
+    return new Solver(shell_mesh(n_ref, np, r_in, r_out, planet));
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return nullptr;
+  }
+}
+void* orc3_create_box(int nx, int ny, int nz, double Lx, double Ly, double H) {
+  try {
+    return new Solver(box_mesh(nx, ny, nz, Lx, Ly, H));
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return nullptr;
+  }
+}
+void orc3_destroy(void* h) { delete (Solver*)h; }
+int orc3_ncells(void* h) { return ((Solver*)h)->m.ncell; }
+void orc3_set_threads(int n) { omp_set_num_threads(n); }
+
+void orc3_set_params(void* h, double Ma, double Fr, double dt, double gamma_gas, double eps, int max_its,
+                     double fp_tol, int fp_max_iter, int fp_lock, int gmres_restart) {
+  Params& P = ((Solver*)h)->op.P;
+  P.Ma = Ma;
+  P.Fr = Fr;
+  P.dt = dt;
+  P.gamma_gas = gamma_gas;
+  P.eps = eps;
+  P.max_its = max_its;
+  P.fp_tol = fp_tol;
+  P.fp_max_iter = fp_max_iter;
+  P.fp_lock = fp_lock;
+  P.gmres_restart = gmres_restart;
+}
+void orc3_set_stage(void* h, int hs, int ns) {
+  ((Solver*)h)->op.hs = hs;
+  ((Solver*)h)->op.ns = ns;
+}
+void orc3_set_rho_for_fixed(void* h, const double* rho) {
+  Solver* s = (Solver*)h;
+  s->op.rho_for_fixed.assign(rho, rho + s->Ns());
+}
+void orc3_set_pres_fixed(void* h, const double* p) {
+  Solver* s = (Solver*)h;
+  s->op.pres_fixed.assign(p, p + s->Ns());
+}
+
+// 0 ok, 1 error (orc3_last_error)
+#define ORC3_GUARD(...)               \
+  try {                               \
+    __VA_ARGS__;                      \
+    return 0;                         \
+  } catch (const std::exception& e) { \
+    g_err = e.what();                 \
+    return 1;                         \
+  }
+
+int orc3_rhs_rho(void* h, double* dst, const double* const* src, int nsrc) {
+  ORC3_GUARD(((Solver*)h)->op.rhs_rho(dst, src, nsrc))
+}
+int orc3_rhs_pressure(void* h, double* dst, const double* const* src, int nsrc) {
+  ORC3_GUARD(((Solver*)h)->op.rhs_pressure(dst, src, nsrc))
+}
+int orc3_rhs_velocity(void* h, double* dst, const double* const* src, int nsrc) {
+  ORC3_GUARD(((Solver*)h)->op.rhs_velocity(dst, src, nsrc))
+}
+int orc3_vmult_pressure(void* h, double* dst_u, const double* p) {
+  ORC3_GUARD(((Solver*)h)->op.vmult_pressure(dst_u, p))
+}
+int orc3_vmult_enthalpy(void* h, double* dst_p, const double* u) {
+  ORC3_GUARD(((Solver*)h)->op.vmult_enthalpy(dst_p, u))
+}
+// vmult per NS_stage: 1/4 M_rho, 3/5 A, 6 D, 2 the Schur complement
+int orc3_vmult(void* h, int ns, double* dst, const double* src) {
+  Solver* s = (Solver*)h;
+  ORC3_GUARD(if (ns == 2) s->op.schur(dst, src); else s->op.apply_block(ns, dst, src))
+}
+int orc3_inverse_diagonal(void* h, int ns, double* dst) { ORC3_GUARD(((Solver*)h)->op.inverse_diagonal(ns, dst)) }
+int orc3_solve_block(void* h, int ns, double* x, const double* b) {
+  ORC3_GUARD(((Solver*)h)->op.solve_block(ns, x, b))
+}
+// GMRES on the Schur complement to tol_factor*|b|; iterations in *its
+int orc3_solve_schur(void* h, double* x, const double* b, double tol_factor, int* its) {
+  Solver* s = (Solver*)h;
+  ORC3_GUARD({
+    Vec xv(x, x + s->Ns()), bv(b, b + s->Ns());
+    *its = s->solve_schur(xv, bv, tol_factor * norm2(bv));
+    std::copy(xv.begin(), xv.end(), x);
+  })
+}
+void orc3_set_state(void* h, const double* rho, const double* u, const double* p) {
+  ((Solver*)h)->init_state(rho, u, p);
+}
+void orc3_get_state(void* h, double* rho, double* u, double* p) {
+  Solver* s = (Solver*)h;
+  std::copy(s->rho_old.begin(), s->rho_old.end(), rho);
+  std::copy(s->u_old.begin(), s->u_old.end(), u);
+  std::copy(s->p_old.begin(), s->p_old.end(), p);
+}
+int orc3_step(void* h, int nsteps) {
+  Solver* s = (Solver*)h;
+  ORC3_GUARD(for (int i = 0; i < nsteps; ++i) s->step())
+}
+void orc3_stats(void* h, long* out) {
+  out[0] = ((Solver*)h)->gmres_its;
+  out[1] = ((Solver*)h)->fp_its;
+}
+
+// positions of the 8 support points (dofs) of every cell, [cell][8][3]
+void orc3_dof_points(void* h, double* out) {
+  Solver* s = (Solver*)h;
+  for (int c = 0; c < s->m.ncell; ++c)
+    for (int d = 0; d < 8; ++d) {
+      const int a = 2 * (d & 1), b = 2 * ((d >> 1) & 1), cc = 2 * (d >> 2);
+      for (int e = 0; e < 3; ++e) out[((size_t)c * 8 + d) * 3 + e] = s->m.X[(size_t)c * 81 + (a + 3 * b + 9 * cc) * 3 + e];
+    }
+}
+// neighbour table [cell][6] (-1: wall)
+void orc3_neighbours(void* h, int* nb, int* nbf) {
+  Solver* s = (Solver*)h;
+  std::copy(s->m.nb.begin(), s->m.nb.end(), nb);
+  std::copy(s->m.nbf.begin(), s->m.nbf.end(), nbf);
+}
+// total volume (sum of cell JxW) and area of the walls with zeta = 0 / 1
+void orc3_measures(void* h, double* out) {
+  Solver* s = (Solver*)h;
+  double vol = 0, a_in = 0, a_out = 0, a_all = 0;
+  for (int c = 0; c < s->m.ncell; ++c) {
+    for (int q = 0; q < 27; ++q) vol += s->g.cg[c].JxW[q];
+    for (int f = 0; f < 6; ++f) {
+      if (s->m.nb[c * 6 + f] >= 0) continue;
+      double a = 0;
+      for (int q = 0; q < 9; ++q) a += s->g.fg[(size_t)c * 6 + f].JxW[q];
+      a_all += a;
+      if (f == 4) a_in += a;
+      if (f == 5) a_out += a;
+    }
+  }
+  out[0] = vol;
+  out[1] = a_in;
+  out[2] = a_out;
+  out[3] = a_all;
+}
+// the analytic solid-body-rotation state at the dofs (VectorTools::interpolate
+// at the mapped support points, Atmospheric_Flow.cc:420-422)
+void orc3_sbr_state(void* h, double Omega, double a, double g, double RT, double* rho, double* u, double* p) {
+  Solver* s = (Solver*)h;
+  SbrState st{Omega, a, g, RT};
+  const int N = s->Ns();
+  for (int c = 0; c < s->m.ncell; ++c)
+    for (int d = 0; d < 8; ++d) {
+      const int aa = 2 * (d & 1), b = 2 * ((d >> 1) & 1), cc = 2 * (d >> 2);
+      const double* x = &s->m.X[(size_t)c * 81 + (aa + 3 * b + 9 * cc) * 3];
+      double uu[3];
+      st.at(x, rho[c * 8 + d], uu, p[c * 8 + d]);
+      for (int e = 0; e < 3; ++e) u[(size_t)e * N + c * 8 + d] = uu[e];
+    }
+}
+// analyze_results (Atmospheric_Flow.cc:926-985): L2 errors of the given
+// state against the analytic one with QGauss(k+1) = QGauss(2) and the cell
+// map; out = {err_u, rel_u, err_rho, rel_rho, err_p, rel_p}
+void orc3_l2_errors(void* h, double Omega, double a, double g, double RT, const double* rho, const double* u,
+                    const double* p, double* out) {
+  Solver* s = (Solver*)h;
+  SbrState st{Omega, a, g, RT};
+  const int N = s->Ns();
+  double e[3] = {0, 0, 0}, nrm[3] = {0, 0, 0};
+  for (int c = 0; c < s->m.ncell; ++c)
+    for (int q = 0; q < 8; ++q) {
+      double xi[3] = {G2X[q & 1], G2X[(q >> 1) & 1], G2X[q >> 2]}, x[3], J[3][3], Ji[3][3];
+      map_point(&s->m.X[(size_t)c * 81], xi, x, J);
+      const double JxW = invert3(J, Ji) * 0.125;
+      double re, ue[3], pe;
+      st.at(x, re, ue, pe);
+      double du = 0, nu = 0;
+      for (int d = 0; d < 3; ++d) {
+        const double v = eval(u + (size_t)d * N + (size_t)c * 8, xi);
+        du += (v - ue[d]) * (v - ue[d]);
+        nu += ue[d] * ue[d];
+      }
+      const double rv = eval(rho + (size_t)c * 8, xi), pv = eval(p + (size_t)c * 8, xi);
+      e[0] += du * JxW;
+      nrm[0] += nu * JxW;
+      e[1] += (rv - re) * (rv - re) * JxW;
+      nrm[1] += re * re * JxW;
+      e[2] += (pv - pe) * (pv - pe) * JxW;
+      nrm[2] += pe * pe * JxW;
+    }
+  for (int i = 0; i < 3; ++i) {
+    out[2 * i] = std::sqrt(e[i]);
+    out[2 * i + 1] = std::sqrt(e[i]) / std::sqrt(nrm[i]);
+  }
+}
+
+}  // extern "C"
