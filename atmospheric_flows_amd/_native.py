@@ -35,6 +35,11 @@ class Params(ctypes.Structure):
                 ("precision", ctypes.c_int)]
 
 
+class ShellDesc(ctypes.Structure):
+    _fields_ = [("refinements", ctypes.c_int), ("panels", ctypes.c_int), ("r_inner", ctypes.c_double),
+                ("r_outer", ctypes.c_double), ("planet", ctypes.c_double)]
+
+
 class Stats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_long) for n in ("cg_its", "gmres_its", "inner_cg_its", "fp_its", "mg_cycles",
                                                "coarse_its", "schur_matvecs", "host_syncs")]
@@ -118,6 +123,14 @@ SIGNATURES = [
     ("atmo_device_bytes", ctypes.c_int, [ctypes.POINTER(MeshDesc), ctypes.POINTER(Params),
                                          ctypes.POINTER(ctypes.c_size_t)]),
     ("atmo_ctx_device_bytes", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)]),
+    ("atmo_ctx_create_shell", ctypes.c_int, [ctypes.POINTER(ShellDesc), ctypes.POINTER(Params),
+                                             ctypes.POINTER(ctypes.c_void_p)]),
+    ("atmo_ctx_dim", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
+    ("atmo_shell_dof_points", ctypes.c_int, [ctypes.c_void_p, _dp]),
+    ("atmo_shell_sbr_state", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_double] * 4 + [_dp, _dp, _dp]),
+    ("atmo_shell_l2_errors", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_double] * 4 + [_dp]),
+    ("atmo_shell_device_bytes", ctypes.c_int, [ctypes.POINTER(ShellDesc), ctypes.POINTER(Params),
+                                               ctypes.POINTER(ctypes.c_size_t)]),
     ("atmo_ensemble_create", ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                             ctypes.POINTER(ctypes.c_void_p)]),
     ("atmo_ensemble_destroy", ctypes.c_int, [ctypes.c_void_p]),

@@ -14,7 +14,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libatmo_hip.so")
 DRIVER = os.path.join(HERE, "atmo_run")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
-SOURCES = ["ops.hip", "krylov.hip", "mg.hip", "diag.hip", "schur.hip", "amr.cpp", "engine.hip", "ensemble.cpp", "vtu.cpp", "capi.cpp", "noalloc.hip"]
+SOURCES = ["ops.hip", "krylov.hip", "mg.hip", "diag.hip", "schur.hip", "amr.cpp", "engine.hip", "shell.cpp", "shell_ops.hip", "shell_engine.hip", "ensemble.cpp", "vtu.cpp", "capi.cpp", "noalloc.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "-std=c++17", "-O3", "-fPIC", "--offload-arch=gfx950",

@@ -113,9 +113,14 @@ static void check_params(const atmo_params& p) {
   if (p.mg_min_cells < 1) throw std::invalid_argument("params: mg_min_cells must be >= 1");
 }
 
+// components of a block's field: the velocity has dim of them
+static size_t block_size(const atmo_ctx* c, int block) {
+  return (size_t)(block == ATMO_BLOCK_VELOCITY ? c->eng->dim() : 1) * (size_t)c->eng->ns();
+}
+
 static void check_vec(const atmo_ctx* c, const atmo_vec* v, int block) {
   if (!v) throw std::invalid_argument("null vector");
-  const size_t want = (size_t)(block == ATMO_BLOCK_VELOCITY ? 2 : 1) * (size_t)c->eng->ns();
+  const size_t want = block_size(c, block);
   if (v->n != want) throw std::invalid_argument("vector does not belong to the expected block");
 }
 
@@ -217,6 +222,7 @@ atmo_status atmo_ctx_init_hill_state(atmo_ctx* ctx, double theta0, double p0, do
                                      double* p) {
   return guarded(ctx, [&]() {
     if (!ctx || !rho || !u || !p) throw std::invalid_argument("null argument");
+    if (ctx->eng->shell()) throw std::invalid_argument("hill state: not on a shell context");
     const std::vector<double>& geo = ctx->eng->cell_geometry();
     if (geo.empty())
       hill_initial_state(ctx->degree, ctx->mesh, theta0, p0, N, U, g, R, gamma_gas, perturbation, seed, rho, u, p);
@@ -283,7 +289,7 @@ atmo_status atmo_vec_alloc(atmo_ctx* ctx, int block, atmo_vec** out) {
     if (block < 0 || block > 2) throw std::invalid_argument("block must be 0 (velocity), 1 (pressure), 2 (density)");
     auto v = std::make_unique<atmo_vec>();
     v->block = block;
-    v->n = (size_t)(block == ATMO_BLOCK_VELOCITY ? 2 : 1) * (size_t)ctx->eng->ns();
+    v->n = block_size(ctx, block);
     ATMO_HIP_CHECK(hipMalloc(&v->p, v->n * sizeof(double)));
     ATMO_HIP_CHECK(hipMemsetAsync(v->p, 0, v->n * sizeof(double), ctx->eng->stream()));
     *out = v.release();
@@ -389,7 +395,7 @@ static std::vector<const double*> gather(const atmo_ctx* c, const atmo_vec* cons
   std::vector<const double*> out(nsrc);
   for (int i = 0; i < nsrc; ++i) {
     if (!src[i]) throw std::invalid_argument("null source vector");
-    if (src[i]->n != (size_t)(src[i]->block == ATMO_BLOCK_VELOCITY ? 2 : 1) * (size_t)c->eng->ns())
+    if (src[i]->n != block_size(c, src[i]->block))
       throw std::invalid_argument("source vector of another context");
     out[i] = src[i]->p;
   }
@@ -517,13 +523,15 @@ atmo_status atmo_state_upload(atmo_ctx* ctx, const double* rho, const double* u,
   return guarded(ctx, [&]() {
     if (!ctx || !rho || !u || !p) throw std::invalid_argument("null argument");
     const long Ns = ctx->eng->ns();
-    for (long i = 0; i < Ns; ++i)
-      if (!std::isfinite(rho[i]) || !std::isfinite(p[i]) || !(rho[i] > 0) || !std::isfinite(u[i]) ||
-          !std::isfinite(u[Ns + i]))
-        throw std::runtime_error("state_upload: non-finite state or non-positive density");
+    const int nd = ctx->eng->dim();
+    for (long i = 0; i < Ns; ++i) {
+      bool ok = std::isfinite(rho[i]) && std::isfinite(p[i]) && rho[i] > 0;
+      for (int c = 0; c < nd; ++c) ok = ok && std::isfinite(u[c * Ns + i]);
+      if (!ok) throw std::runtime_error("state_upload: non-finite state or non-positive density");
+    }
     hipStream_t s = ctx->eng->stream();
     ATMO_HIP_CHECK(hipMemcpyAsync(ctx->eng->state_ptr(0), rho, Ns * sizeof(double), hipMemcpyHostToDevice, s));
-    ATMO_HIP_CHECK(hipMemcpyAsync(ctx->eng->state_ptr(1), u, 2 * Ns * sizeof(double), hipMemcpyHostToDevice, s));
+    ATMO_HIP_CHECK(hipMemcpyAsync(ctx->eng->state_ptr(1), u, nd * Ns * sizeof(double), hipMemcpyHostToDevice, s));
     ATMO_HIP_CHECK(hipMemcpyAsync(ctx->eng->state_ptr(2), p, Ns * sizeof(double), hipMemcpyHostToDevice, s));
     ATMO_HIP_CHECK(hipStreamSynchronize(s));
     ctx->eng->load_state(ctx->eng->state_ptr(0), ctx->eng->state_ptr(1), ctx->eng->state_ptr(2));
@@ -537,7 +545,8 @@ atmo_status atmo_state_download(atmo_ctx* ctx, double* rho, double* u, double* p
     const long Ns = ctx->eng->ns();
     hipStream_t s = ctx->eng->stream();
     ATMO_HIP_CHECK(hipMemcpyAsync(rho, ctx->eng->state_ptr(0), Ns * sizeof(double), hipMemcpyDeviceToHost, s));
-    ATMO_HIP_CHECK(hipMemcpyAsync(u, ctx->eng->state_ptr(1), 2 * Ns * sizeof(double), hipMemcpyDeviceToHost, s));
+    ATMO_HIP_CHECK(
+        hipMemcpyAsync(u, ctx->eng->state_ptr(1), ctx->eng->dim() * Ns * sizeof(double), hipMemcpyDeviceToHost, s));
     ATMO_HIP_CHECK(hipMemcpyAsync(p, ctx->eng->state_ptr(2), Ns * sizeof(double), hipMemcpyDeviceToHost, s));
     ATMO_HIP_CHECK(hipStreamSynchronize(s));
     return ATMO_OK;
@@ -571,9 +580,9 @@ atmo_status atmo_step_diagnostics(atmo_ctx* ctx, int of_state, atmo_step_diag* o
     const MeshDesc& m = ctx->mesh;
     out->min_cell_diameter = ctx->eng->min_cell_diameter() > 0 ? ctx->eng->min_cell_diameter()
                                                                  : min_cell_diameter(m.nx, m.nz, m.Lx, m.H, m.h.data());
-    // Courant numbers as printed by EulerSolver::run (Atmospheric_Flow.cc:1225-1235), dim = 2
+    // Courant numbers as printed by EulerSolver::run (Atmospheric_Flow.cc:1225-1235)
     const EngineParams& p = ctx->eng->params();
-    const double f = p.dt * ctx->degree * std::sqrt(2.0) / out->min_cell_diameter;
+    const double f = p.dt * ctx->degree * std::sqrt((double)ctx->eng->dim()) / out->min_cell_diameter;
     out->cfl_c = f * out->max_celerity / p.Ma;
     out->cfl_u = f * out->max_velocity;
     return ATMO_OK;
@@ -595,6 +604,7 @@ atmo_status atmo_init_hill_state(const atmo_mesh_desc* mesh, double theta0, doub
 atmo_status atmo_write_vtu(atmo_ctx* ctx, const char* path, int high_order) {
   return guarded(ctx, [&]() {
     if (!ctx || !path) throw std::invalid_argument("null argument");
+    if (ctx->eng->shell()) throw std::invalid_argument("write_vtu: not available on a shell context");
     const long Ns = ctx->eng->ns();
     std::vector<double> rho(Ns), u(2 * Ns), p(Ns);
     hipStream_t s = ctx->eng->stream();
@@ -626,6 +636,91 @@ atmo_status atmo_time_kernel(atmo_ctx* ctx, int which, int nrep, double* ms_per_
   });
 }
 
+
+// ------------------------------------------------------------ spherical shell
+static ShellDesc to_shell(const atmo_shell_desc* d) {
+  ShellDesc s;
+  s.refinements = d->refinements;
+  s.panels = d->panels;
+  s.r_inner = d->r_inner;
+  s.r_outer = d->r_outer;
+  s.planet = d->planet;
+  if (s.panels != 6 && s.panels != 12) throw std::invalid_argument("shell: panels must be 6 or 12");
+  if (s.refinements < 0 || s.refinements > 7) throw std::invalid_argument("shell: refinements must be in [0, 7]");
+  if (!(s.r_inner > 0) || !(s.r_outer > s.r_inner) || !std::isfinite(s.r_outer) || !(s.planet > 0) ||
+      !std::isfinite(s.planet))
+    throw std::invalid_argument("shell: need 0 < r_inner < r_outer and planet > 0");
+  return s;
+}
+
+static const ShellMesh& shell_of(atmo_ctx* ctx) {
+  if (!ctx->eng->shell()) throw std::invalid_argument("not a shell context");
+  return *ctx->eng->shell();
+}
+
+atmo_status atmo_ctx_create_shell(const atmo_shell_desc* shell, const atmo_params* params, atmo_ctx** out) {
+  return guarded(nullptr, [&]() {
+    if (!shell || !params || !out) throw std::invalid_argument("null argument");
+    *out = nullptr;
+    const ShellDesc d = to_shell(shell);
+    check_params(*params);
+    auto c = std::make_unique<atmo_ctx>();
+    c->degree = 1;
+    c->eng = Engine::create_shell(d, to_engine(*params));
+    *out = c.release();
+    return ATMO_OK;
+  });
+}
+
+atmo_status atmo_ctx_dim(atmo_ctx* ctx, int* dim) {
+  if (!ctx || !dim) return ATMO_INVALID_ARG;
+  *dim = ctx->eng->dim();
+  return ATMO_OK;
+}
+
+atmo_status atmo_shell_dof_points(atmo_ctx* ctx, double* xyz) {
+  return guarded(ctx, [&]() {
+    if (!ctx || !xyz) throw std::invalid_argument("null argument");
+    shell_dof_points(shell_of(ctx), xyz);
+    return ATMO_OK;
+  });
+}
+
+atmo_status atmo_shell_sbr_state(atmo_ctx* ctx, double Omega, double a, double g, double RT, double* rho, double* u,
+                                 double* p) {
+  return guarded(ctx, [&]() {
+    if (!ctx || !rho || !u || !p) throw std::invalid_argument("null argument");
+    if (!(RT > 0) || !(a > 0) || !std::isfinite(Omega) || !std::isfinite(g))
+      throw std::invalid_argument("sbr state: need RT > 0, a > 0");
+    shell_sbr_state(shell_of(ctx), SbrParams{Omega, a, g, RT}, rho, u, p);
+    return ATMO_OK;
+  });
+}
+
+atmo_status atmo_shell_l2_errors(atmo_ctx* ctx, double Omega, double a, double g, double RT, double* out) {
+  return guarded(ctx, [&]() {
+    if (!ctx || !out) throw std::invalid_argument("null argument");
+    const ShellMesh& m = shell_of(ctx);
+    const long Ns = ctx->eng->ns();
+    std::vector<double> rho(Ns), u(3 * Ns), p(Ns);
+    hipStream_t s = ctx->eng->stream();
+    ATMO_HIP_CHECK(hipMemcpyAsync(rho.data(), ctx->eng->state_ptr(0), Ns * sizeof(double), hipMemcpyDeviceToHost, s));
+    ATMO_HIP_CHECK(hipMemcpyAsync(u.data(), ctx->eng->state_ptr(1), 3 * Ns * sizeof(double), hipMemcpyDeviceToHost, s));
+    ATMO_HIP_CHECK(hipMemcpyAsync(p.data(), ctx->eng->state_ptr(2), Ns * sizeof(double), hipMemcpyDeviceToHost, s));
+    ATMO_HIP_CHECK(hipStreamSynchronize(s));
+    shell_l2_errors(m, SbrParams{Omega, a, g, RT}, rho.data(), u.data(), p.data(), out);
+    return ATMO_OK;
+  });
+}
+
+atmo_status atmo_shell_device_bytes(const atmo_shell_desc* shell, const atmo_params* params, size_t* bytes) {
+  return guarded(nullptr, [&]() {
+    if (!shell || !params || !bytes) throw std::invalid_argument("null argument");
+    check_params(*params);
+    *bytes = shell_device_bytes(to_shell(shell), to_engine(*params));
+    return ATMO_OK;
+  });
+}
 
 // ------------------------------------------------------------ footprint and ensemble
 atmo_status atmo_device_bytes(const atmo_mesh_desc* mesh, const atmo_params* params, size_t* bytes) {

@@ -27,6 +27,8 @@ constexpr int DIAG_PARTIALS = 3 * DIAG_MAX_BLOCKS;
 template <int K>
 void launch_diag_density(const MeshDev& m, const double* rho, int stage, double* partials, double* rec,
                          hipStream_t s);
+// The same for a density field of `items` dofs whose nodal values are the point values (any mesh).
+void launch_diag_density_n(long items, const double* rho, int stage, double* partials, double* rec, hipStream_t s);
 // Flow record of the state (rho, u, p): rec[DG_MAX_CEL], rec[DG_MAX_VEL], rec[DG_NONFINITE+3].
 template <int K>
 void launch_diag_flow(const MeshDev& m, const double* rho, const double* u, const double* p, double gamma_gas,

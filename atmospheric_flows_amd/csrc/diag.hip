@@ -118,6 +118,13 @@ int diag_blocks(long items) {
 
 }  // namespace
 
+void launch_diag_density_n(long items, const double* rho, int stage, double* partials, double* rec, hipStream_t s) {
+  const int nb = diag_blocks(items);
+  hipLaunchKernelGGL(k_diag_density, dim3(nb), dim3(DT), 0, s, rho, items, partials);
+  hipLaunchKernelGGL(k_diag_fin, dim3(1), dim3(DT), 0, s, partials, nb, -1.0, rec, DG_MIN_RHO + stage,
+                     DG_MAX_RHO + stage, DG_NONFINITE + stage);
+}
+
 template <int K>
 void launch_diag_density(const MeshDev& m, const double* rho, int stage, double* partials, double* rec,
                          hipStream_t s) {

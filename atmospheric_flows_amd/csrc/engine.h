@@ -8,6 +8,7 @@
 
 #include "common.h"
 #include "ops.h"
+#include "shell.h"
 
 namespace atmo {
 
@@ -61,6 +62,11 @@ class Engine {
   // locally refined mesh over a base mesh; leaves as for build_amr_levels (amr.h)
   static std::unique_ptr<Engine> create_amr(int k, const MeshDesc& base, const std::vector<QCell>& leaves,
                                             const EngineParams& p);
+  // the reference's 3D case: FE_DGQ(1) on the spherical shell (shell.h)
+  static std::unique_ptr<Engine> create_shell(const ShellDesc& d, const EngineParams& p);
+
+  virtual int dim() const { return 2; }  // velocity components
+  virtual const ShellMesh* shell() const { return nullptr; }
 
   virtual int degree() const = 0;
   virtual int ncells() const = 0;
@@ -112,6 +118,7 @@ class Engine {
 // Device bytes an engine of this mesh and parameters requests (equals
 // device_bytes() of such an engine; used for admission before creating it).
 size_t engine_device_bytes(int k, const MeshDesc& mesh, const EngineParams& p);
+size_t shell_device_bytes(const ShellDesc& d, const EngineParams& p);
 
 // Host-side helpers shared by the C ABI and the driver.
 void hill_initial_state(int k, const MeshDesc& m, double theta0, double p0, double Nbv, double U, double g,

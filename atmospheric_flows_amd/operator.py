@@ -108,9 +108,14 @@ class DeviceContext:
             N.check(L.atmo_ctx_create_amr(ctypes.byref(self.mesh), lv.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                                           lv.shape[0], ctypes.byref(self.params), ctypes.byref(h_)))
         self.handle = h_
-        nc, ns, nl = ctypes.c_int(), ctypes.c_long(), ctypes.c_int()
+        self._info()
+
+    def _info(self):
+        L = N.load()
+        nc, ns, nl, d = ctypes.c_int(), ctypes.c_long(), ctypes.c_int(), ctypes.c_int()
         L.atmo_ctx_info(self.handle, ctypes.byref(nc), ctypes.byref(ns), ctypes.byref(nl))
-        self.ncells, self.ns, self.nlevels = nc.value, ns.value, nl.value
+        L.atmo_ctx_dim(self.handle, ctypes.byref(d))
+        self.ncells, self.ns, self.nlevels, self.dim = nc.value, ns.value, nl.value, d.value
 
     def close(self):
         if getattr(self, "handle", None):
@@ -147,7 +152,7 @@ class DeviceContext:
         N.check(N.load().atmo_state_upload(self.handle, _ptr(rho), _ptr(u), _ptr(p)), self.handle)
 
     def download_state(self):
-        rho, u, p = np.zeros(self.ns), np.zeros(2 * self.ns), np.zeros(self.ns)
+        rho, u, p = np.zeros(self.ns), np.zeros(self.dim * self.ns), np.zeros(self.ns)
         N.check(N.load().atmo_state_download(self.handle, _ptr(rho), _ptr(u), _ptr(p)), self.handle)
         return rho, u, p
 
@@ -186,6 +191,61 @@ class DeviceContext:
         ms, b = ctypes.c_double(), ctypes.c_double()
         N.check(N.load().atmo_time_kernel(self.handle, which, nrep, ctypes.byref(ms), ctypes.byref(b)), self.handle)
         return ms.value, b.value
+
+
+# EquationData (equation_data.h:23-33) of the reference's 3D test
+EARTH_RADIUS = 6.37122e6
+GRAVITY = 9.80616
+R_GAS = 287.0
+T_BAR = 273.0
+OMEGA_SBR = 2.0 * np.pi / 86400.0
+
+
+class ShellContext(DeviceContext):
+    """The reference's own 3D case on the GPU (atmo_ctx_create_shell):
+    FE_DGQ(1) on the shell between r_inner and r_outer (0.9 a and a,
+    Atmospheric_Flow.cc:283-284), `refinements` global refinements of
+    `panels` coarse cells, radial gravity g a^2/r^2 with a = `planet`.
+    Velocity fields have 3 components; every operator, solver and IMEX
+    method of DeviceContext applies."""
+
+    def __init__(self, refinements=3, panels=6, r_inner=0.9 * EARTH_RADIUS, r_outer=EARTH_RADIUS,
+                 planet=EARTH_RADIUS, **params):
+        L = N.load()
+        self.degree = 1
+        self.desc = N.ShellDesc(refinements, panels, r_inner, r_outer, planet)
+        self.params = default_params(**params)
+        h_ = ctypes.c_void_p()
+        N.check(L.atmo_ctx_create_shell(ctypes.byref(self.desc), ctypes.byref(self.params), ctypes.byref(h_)))
+        self.handle = h_
+        self._info()
+
+    def dof_points(self):
+        out = np.zeros(self.ns * 3)
+        N.check(N.load().atmo_shell_dof_points(self.handle, _ptr(out)), self.handle)
+        return out.reshape(self.ncells, 8, 3)
+
+    def sbr_state(self, Omega=OMEGA_SBR, a=EARTH_RADIUS, g=GRAVITY, RT=R_GAS * T_BAR):
+        """The solid-body-rotation steady state at the dofs (non-dimensional)."""
+        rho, u, p = np.zeros(self.ns), np.zeros(3 * self.ns), np.zeros(self.ns)
+        N.check(N.load().atmo_shell_sbr_state(self.handle, Omega, a, g, RT, _ptr(rho), _ptr(u), _ptr(p)),
+                self.handle)
+        return rho, u, p
+
+    def l2_errors(self, Omega=OMEGA_SBR, a=EARTH_RADIUS, g=GRAVITY, RT=R_GAS * T_BAR):
+        """analyze_results on the current state: {u, rho, p} -> (absolute, relative) L2 errors."""
+        out = np.zeros(6)
+        N.check(N.load().atmo_shell_l2_errors(self.handle, Omega, a, g, RT, _ptr(out)), self.handle)
+        return {"u": (out[0], out[1]), "rho": (out[2], out[3]), "p": (out[4], out[5])}
+
+
+def shell_device_bytes(refinements=3, panels=6, r_inner=0.9 * EARTH_RADIUS, r_outer=EARTH_RADIUS,
+                       planet=EARTH_RADIUS, **params):
+    d = N.ShellDesc(refinements, panels, r_inner, r_outer, planet)
+    p = default_params(**params)
+    b = ctypes.c_size_t()
+    N.check(N.load().atmo_shell_device_bytes(ctypes.byref(d), ctypes.byref(p), ctypes.byref(b)))
+    return b.value
 
 
 class EULEROperator:

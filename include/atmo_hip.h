@@ -256,6 +256,48 @@ atmo_status atmo_write_vtu_host(const atmo_mesh_desc* mesh, const double* rho, c
  * Also reports the kernel's algorithmic HBM bytes per launch. */
 atmo_status atmo_time_kernel(atmo_ctx* ctx, int which, int nrep, double* ms_per_launch, double* bytes_per_launch);
 
+/* ------------------------------------------------------------ spherical shell
+ * The reference's own 3D case (EulerSolver<3>, Atmospheric_Flow.cc:1277):
+ * FE_DGQ(1) for density, velocity and pressure with QGauss(3)
+ * (Atmospheric_Flow.cc:345) on a shell between two spheres
+ * (GridGenerator::concentric_hyper_shells(0.9, 1.0) scaled by a and refined
+ * globally, :279-289), MappingQ(2) on the spherical manifold (:239), radial
+ * gravity g a^2/r^2 e_r (euler_operator.h:790-811), walls at both radii.
+ * A shell context takes every operator, solver and IMEX entry point above;
+ * its velocity block has 3 components. Cells: c = ((panel*n + layer)*n +
+ * row)*n + column with n = 2^refinements; dofs: d = a + 2b + 4c at the
+ * cell vertex (xi, eta, zeta) = (a, b, c), zeta outwards. The mass-type
+ * solves (atmo_solve_cg_mg, the inner solve of the Schur complement) are
+ * cell-by-cell products with the FP64 block inverses (0 iterations): the
+ * exact solutions the reference's CG + MG approach; every precision
+ * setting runs this FP64 form. Replaces the mesh of
+ * Atmospheric_Flow.cc:279-289 and the solver set-up of :296-409 for dim = 3. */
+typedef struct {
+  int refinements; /* Space discretization / n_of_refines (parameter-file.prm:19), 0..7 */
+  int panels;      /* coarse cells per shell: 6 (cube, hyper_shell's default) or 12 (rhombic dodecahedron) */
+  double r_inner, r_outer; /* shell radii: 0.9 a and a in the reference (Atmospheric_Flow.cc:283-284) */
+  double planet;   /* a of the gravity factor a^2/r^2 (EquationData::a, equation_data.h:27) */
+} atmo_shell_desc;
+
+atmo_status atmo_ctx_create_shell(const atmo_shell_desc* shell, const atmo_params* params, atmo_ctx** out);
+/* Velocity components of the context's fields: 2 (terrain), 3 (shell). */
+atmo_status atmo_ctx_dim(atmo_ctx* ctx, int* dim);
+/* Physical positions of the dofs of a shell context, [cell][8][3]. */
+atmo_status atmo_shell_dof_points(atmo_ctx* ctx, double* xyz);
+/* The solid-body-rotation steady state (EquationData::Density/Velocity/
+ * Pressure, equation_data.h:47-176) at the dofs, in the reference's
+ * non-dimensional variables (rho/rho_ref = p/p_ref = exp(q), u = Omega
+ * (-y, x, 0)); RT = R T_bar. VectorTools::interpolate, Atmospheric_Flow.cc:420-422. */
+atmo_status atmo_shell_sbr_state(atmo_ctx* ctx, double Omega, double a, double g, double RT, double* rho, double* u,
+                                 double* p);
+/* analyze_results (Atmospheric_Flow.cc:926-985) on the context's state: L2
+ * errors against the steady state with QGauss(2) on the mapped cells;
+ * out[6] = {u, u relative, rho, rho relative, p, p relative}. Copies the
+ * state to the host. */
+atmo_status atmo_shell_l2_errors(atmo_ctx* ctx, double Omega, double a, double g, double RT, double* out);
+/* Device bytes a shell context of this description and parameters allocates. */
+atmo_status atmo_shell_device_bytes(const atmo_shell_desc* shell, const atmo_params* params, size_t* bytes);
+
 /* ------------------------------------------------------------ device footprint
  * No counterpart in the reference (it sizes nothing up front): device bytes
  * a context of this mesh and parameters allocates, for admission decisions

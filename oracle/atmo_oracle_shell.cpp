@@ -352,12 +352,14 @@ struct Geo {
 
 // reference coordinates of the physical point x in the cell with support
 // points X, x lying on the cell's face `face`; false when Newton's method
-// does not reach round-off
+// does not reach round-off. Newton converges quadratically from the face
+// centre; once a step is below 1e-9, three more reach the rounding floor
+// of the map (the sum of 27 support points of size |x|).
 static bool newton_inverse(const double* X, const double x[3], int face, double xi[3]) {
   // start at the centre of the face the point lies on
   face_ref(face, 0.5, 0.5, xi);
   int small = 0;
-  for (int it = 0; it < 60 && small < 2; ++it) {
+  for (int it = 0; it < 60 && small < 3; ++it) {
     double y[3], J[3][3], Ji[3][3];
     map_point(X, xi, y, J);
     invert3(J, Ji);
@@ -368,14 +370,14 @@ static bool newton_inverse(const double* X, const double x[3], int face, double 
       xi[e] -= s;
       step = std::max(step, std::abs(s));
     }
-    if (step < 1e-13) ++small;  // two more iterations once at round-off level
+    if (step < 1e-9) ++small;
   }
   xi[face / 2] = (face & 1) ? 1.0 : 0.0;  // exactly on the face
   double y[3], J[3][3];
   map_point(X, xi, y, J);
   const double res = std::abs(y[0] - x[0]) + std::abs(y[1] - x[1]) + std::abs(y[2] - x[2]);
   const double scale = std::abs(x[0]) + std::abs(x[1]) + std::abs(x[2]);
-  return small >= 2 && res <= 1e-12 * scale;
+  return small >= 3 && res <= 1e-12 * scale;
 }
 
 static Geo make_geo(const Mesh& m) {
