@@ -3,16 +3,14 @@
 // (Atmospheric_Flow.cc:345), on MappingQ(2)-mapped cells, radial gravity
 // g a^2/r^2 e_r evaluated at every quadrature point.
 //
-// Launch shape: one wave64 per two cells; lanes 0-31 work on the first,
-// 32-63 on the second. Lane l < 27 of a cell owns quadrature point
-// (l % 3, l / 3 % 3, l / 9) in the cell phase and face points l and l + 27
-// (face (l + 27h) / 9, point (l + 27h) % 9) in the face phase; lanes
-// l < 8 * NOUT then integrate test function l % 8 of output component l / 8
-// from the LDS records of both phases. The cell's dofs are staged through
-// LDS; a neighbour's dofs are read from global memory (L2) at the face
-// point its orientation code maps to. J is rebuilt from the cell's 9
-// lateral support directions (formed once per cell by lanes 0-8) and its
-// layer radii, so no per-point geometry is stored.
+// Launch shape: one wave64 per 7 cells, 9 lanes per cell, one per lateral
+// quadrature column. The cell map factorizes as X = r(zeta) N(xi, eta) (the
+// shell's Q2 map, shell.h), so J = [r N_xi | r N_eta | dr N]: a lane forms
+// the lateral part once (N and the cofactor rows of [N_xi | N_eta | N]) and
+// gets the three radial points of its column by scaling with r. The cell's
+// dofs are staged through LDS; a neighbour's dofs are read from global
+// memory (L2) at the face point its orientation code maps to; no per-point
+// geometry is stored.
 #include <cmath>
 
 #include "ops.h"
@@ -33,74 +31,6 @@ __device__ inline double q2v(int a, double x) {
   return a == 0 ? 2.0 * (x - 0.5) * (x - 1.0) : (a == 1 ? -4.0 * x * (x - 1.0) : 2.0 * x * (x - 0.5));
 }
 __device__ inline double q2d(int a, double x) { return a == 0 ? 4.0 * x - 3.0 : (a == 1 ? 4.0 - 8.0 * x : 4.0 * x - 1.0); }
-
-// x(xi) and J[d][e] = dx_d / dxi_e of the cell map r(zeta) N(xi, eta)
-__device__ inline void shell_map(const double (*N)[3], double r0, double dr, const double xi[3], double x[3],
-                                 double J[3][3]) {
-  double La[3], Da[3], Lb[3], Db[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    La[a] = q2v(a, xi[0]);
-    Da[a] = q2d(a, xi[0]);
-    Lb[a] = q2v(a, xi[1]);
-    Db[a] = q2d(a, xi[1]);
-  }
-  double n[3] = {0, 0, 0}, nx[3] = {0, 0, 0}, ny[3] = {0, 0, 0};
-#pragma unroll
-  for (int b = 0; b < 3; ++b)
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      const double w = La[a] * Lb[b], wx = Da[a] * Lb[b], wy = La[a] * Db[b];
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        const double v = N[a + 3 * b][d];
-        n[d] += w * v;
-        nx[d] += wx * v;
-        ny[d] += wy * v;
-      }
-    }
-  const double r = r0 + dr * xi[2];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    x[d] = r * n[d];
-    J[d][0] = r * nx[d];
-    J[d][1] = r * ny[d];
-    J[d][2] = dr * n[d];
-  }
-}
-
-// det J and Ji[e][d] = dxi_e / dx_d
-__device__ inline double inv3(const double J[3][3], double Ji[3][3]) {
-  const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
-  const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
-  const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
-  const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
-  const double id = 1.0 / det;
-  Ji[0][0] = c00 * id;
-  Ji[1][0] = c01 * id;
-  Ji[2][0] = c02 * id;
-  Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
-  Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
-  Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
-  Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
-  Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
-  Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
-  return det;
-}
-
-// trilinear basis values at xi
-__device__ inline void q1_all(const double xi[3], double phi[8]) {
-#pragma unroll
-  for (int d = 0; d < 8; ++d) phi[d] = q1v(d & 1, xi[0]) * q1v((d >> 1) & 1, xi[1]) * q1v(d >> 2, xi[2]);
-}
-
-// reference point of face f at face Gauss point (gs, gt)
-__device__ inline void face_xi(int f, int gs, int gt, double xi[3]) {
-  const int e = f >> 1;
-  xi[e] = (double)(f & 1);
-  xi[e == 0 ? 1 : 0] = kGQ[gs];
-  xi[e == 2 ? 1 : 2] = kGQ[gt];
-}
 
 __device__ inline double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
 
@@ -181,9 +111,9 @@ struct PhysPres {
   static constexpr bool FACES = true, GRAVITY = true;
   const double* src[NF];
   Weights w;
+  // gravity work terms: weights of the triples 0 .. ST-1 (stage 3) or of the
+  // triples 0 .. ST-2 and the kinetic pair (stages 1, 2)
   double gwt[3];
-  int gsl[3];  // gravity work terms: weight and triple (-1: the kinetic pair)
-  int ngr;
   double dt, Ma2, iFr2, gm, planet2;
   __device__ void cell(const double* v, const double* x, double* V, double (*F)[3]) const {
     const double r = sqrt(dot3(x, x));
@@ -199,14 +129,11 @@ struct PhysPres {
     const double rk = v[5 * ST];
     const double* uk = v + 5 * ST + 1;
     double val = v[0] * E[0] - 0.5 * rk * Ma2 * dot3(uk, uk);
+    double gsum = 0;
 #pragma unroll
-    for (int t = 0; t < 3; ++t) {
-      if (t >= ngr) break;
-      const int sl = gsl[t];
-      const double rr = sl < 0 ? rk : v[5 * sl];
-      const double ue = dot3(sl < 0 ? uk : v + 5 * sl + 1, er);
-      val -= gwt[t] * dt * Ma2 * iFr2 * rr * ue * G;
-    }
+    for (int t = 0; t < ST; ++t) gsum += gwt[t] * v[5 * t] * dot3(v + 5 * t + 1, er);
+    if constexpr (ST < 3) gsum += gwt[ST] * rk * dot3(uk, er);
+    val -= dt * Ma2 * iFr2 * G * gsum;
     V[0] = val;
     double f[3] = {0, 0, 0};
 #pragma unroll
@@ -394,149 +321,265 @@ struct PhysMass {
 };
 
 // ------------------------------------------------------------------ kernel
-// Lateral support directions of the cell (lanes 0-8) into N, its inner radius into r0.
-__device__ inline void cell_directions(const ShellDev& m, int cell, int l, double (*N)[3], double* r0) {
+// Lateral support direction t = a + 3b (a, b in {0, 1/2, 1}) of the cell's panel map.
+__device__ inline void support_dir(const ShellDev& m, int cell, int t, double* out) {
   const int n = m.n;
-  const int i = cell % n, j = (cell / n) % n, k = (cell / (n * n)) % n, p = cell / (n * n * n);
-  if (l < 9) {
-    const int a = l % 3, b = l / 3;
-    const double s = (i + 0.5 * a) / n, t = (j + 0.5 * b) / n;
-    const double* C = m.corners + 12 * p;
-    double v[3];
+  const int i = cell % n, j = (cell / n) % n, p = cell / (n * n * n);
+  const int a = t % 3, b = t / 3;
+  const double s = (i + 0.5 * a) / n, u = (j + 0.5 * b) / n;
+  const double* C = m.corners + 12 * p;
+  double v[3];
 #pragma unroll
-    for (int d = 0; d < 3; ++d)
-      v[d] = (1 - s) * (1 - t) * C[d] + s * (1 - t) * C[3 + d] + (1 - s) * t * C[6 + d] + s * t * C[9 + d];
-    const double nn = sqrt(dot3(v, v));
+  for (int d = 0; d < 3; ++d)
+    v[d] = (1 - s) * (1 - u) * C[d] + s * (1 - u) * C[3 + d] + (1 - s) * u * C[6 + d] + s * u * C[9 + d];
+  const double nn = sqrt(dot3(v, v));
 #pragma unroll
-    for (int d = 0; d < 3; ++d) N[l][d] = v[d] / nn;
-  }
-  if (l == 0) *r0 = m.r_in + m.dr * k;
+  for (int d = 0; d < 3; ++d) out[d] = v[d] / nn;
 }
 
+// Lateral geometry of the map X = r(zeta) N(xi, eta) at (xi, eta): N and the
+// cofactor rows of A = [N_xi | N_eta | N]: C0 = N_eta x N, C1 = N x N_xi,
+// C2 = N_xi x N_eta, det A = N_xi . C0. With J = A diag(r, r, dr):
+// det J = r^2 dr det A, rows of det J * J^{-1}: r dr C0, r dr C1, r^2 C2.
+struct LatGeo {
+  double N[3], C0[3], C1[3], C2[3], det;
+};
+
+__device__ inline void lat_geo(const double (*S)[3], double xi, double eta, LatGeo& g) {
+  double La[3], Da[3], Lb[3], Db[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    La[a] = q2v(a, xi);
+    Da[a] = q2d(a, xi);
+    Lb[a] = q2v(a, eta);
+    Db[a] = q2d(a, eta);
+  }
+  double N0 = 0, N1 = 0, N2 = 0, X0 = 0, X1 = 0, X2 = 0, Y0 = 0, Y1 = 0, Y2 = 0;
+#pragma unroll
+  for (int b = 0; b < 3; ++b)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const double w = La[a] * Lb[b], wx = Da[a] * Lb[b], wy = La[a] * Db[b];
+      const double v0 = S[a + 3 * b][0], v1 = S[a + 3 * b][1], v2 = S[a + 3 * b][2];
+      N0 += w * v0;
+      N1 += w * v1;
+      N2 += w * v2;
+      X0 += wx * v0;
+      X1 += wx * v1;
+      X2 += wx * v2;
+      Y0 += wy * v0;
+      Y1 += wy * v1;
+      Y2 += wy * v2;
+    }
+  g.N[0] = N0;
+  g.N[1] = N1;
+  g.N[2] = N2;
+  g.C0[0] = Y1 * N2 - Y2 * N1;
+  g.C0[1] = Y2 * N0 - Y0 * N2;
+  g.C0[2] = Y0 * N1 - Y1 * N0;
+  g.C1[0] = N1 * X2 - N2 * X1;
+  g.C1[1] = N2 * X0 - N0 * X2;
+  g.C1[2] = N0 * X1 - N1 * X0;
+  g.C2[0] = X1 * Y2 - X2 * Y1;
+  g.C2[1] = X2 * Y0 - X0 * Y2;
+  g.C2[2] = X0 * Y1 - X1 * Y0;
+  g.det = X0 * g.C0[0] + X1 * g.C0[1] + X2 * g.C0[2];
+}
+
+// bilinear values at (xi, eta) of the bottom (zeta = 0) and top (zeta = 1)
+// dofs of a Q1 cell field u[8]
+__device__ inline void lat_vals(const double* u, double xi, double eta, double& vb, double& vt) {
+  const double w0 = (1 - xi) * (1 - eta), w1 = xi * (1 - eta), w2 = (1 - xi) * eta, w3 = xi * eta;
+  vb = u[0] * w0 + u[1] * w1 + u[2] * w2 + u[3] * w3;
+  vt = u[4] * w0 + u[5] * w1 + u[6] * w2 + u[7] * w3;
+}
+
+// Cells per wave: 7 cells x 9 lanes (one per lateral quadrature column (a, b)).
+constexpr int SCPW = 7;
+
+// Lane (cell w, column t = a + 3b) forms the lateral geometry of its column
+// once and evaluates the cell integrand at the column's three radial
+// quadrature points, then the bottom and top face points of the column;
+// the 12 (lateral face, edge point) pairs of the cell go to lanes t and
+// t - 9, each with its three radial points. Face fluxes get the area vector
+// (unit normal times JxW) in place of the normal: every flux is positively
+// homogeneous of degree 1 in the normal. Lanes then integrate (test
+// function, component) items from the LDS records.
 template <class Phys>
 __global__ void __launch_bounds__(64) k_shell(ShellDev m, Phys ph, double* __restrict__ dst, long Ns,
                                               const double* __restrict__ scale) {
-  constexpr int NF = Phys::NF, NO = Phys::NOUT;
-  __shared__ double sdof[2][NF][8];
-  __shared__ double sN[2][9][3];
-  __shared__ double sr0[2];
-  __shared__ double svol[2][NO][27][4];
-  __shared__ double sface[2][NO][Phys::FACES ? 54 : 1];
-  const int lane = threadIdx.x, w = lane >> 5, l = lane & 31;
-  const int c0 = cell_block() * 2;
+  constexpr int NF = Phys::NF, NO = Phys::NOUT, CPW = SCPW;
+  __shared__ double sdof[NF][CPW * 8];
+  __shared__ double sS[CPW][9][3];
+  __shared__ double svol[CPW][NO][27][4];
+  __shared__ double sface[CPW][NO][Phys::FACES ? 54 : 1];
+  const int lane = threadIdx.x;
+  const int w = lane / 9, t = lane - 9 * w;
+  const int c0 = cell_block() * CPW;
   const int cell = c0 + w;
-  const bool active = cell < m.ncell;
-  // stage the dofs of both cells (16 contiguous doubles per field)
+  const bool active = w < CPW && cell < m.ncell;
   const long lim = (long)m.ncell * 8;
-  for (int t = lane; t < 16 * NF; t += 64) {
-    const int f = t >> 4, r = t & 15;
+  for (int q = lane; q < NF * CPW * 8; q += 64) {
+    const int f = q / (CPW * 8), r = q - f * (CPW * 8);
     const long g = (long)c0 * 8 + r;
-    sdof[r >> 3][f][r & 7] = g < lim ? ph.src[f][g] : 0.0;
+    sdof[f][r] = g < lim ? ph.src[f][g] : 0.0;
   }
-  if (active) cell_directions(m, cell, l, sN[w], &sr0[w]);
+  double r0 = 0;
+  if (active) {
+    support_dir(m, cell, t, sS[w][t]);
+    r0 = m.r_in + m.dr * ((cell / (m.n * m.n)) % m.n);
+  }
   __syncthreads();
-  if (active && l < 27) {
-    const double xi[3] = {kGQ[l % 3], kGQ[(l / 3) % 3], kGQ[l / 9]};
-    double x[3], J[3][3], Ji[3][3], phi[8], v[NF];
-    shell_map(sN[w], sr0[w], m.dr, xi, x, J);
-    const double JxW = inv3(J, Ji) * kGW[l % 3] * kGW[(l / 3) % 3] * kGW[l / 9];
-    q1_all(xi, phi);
+  const double dr = m.dr;
+  if (active) {
+    const int a = t % 3, b = t / 3;
+    LatGeo G;
+    lat_geo(sS[w], kGQ[a], kGQ[b], G);
+    double vb[NF], vt[NF];
 #pragma unroll
-    for (int f = 0; f < NF; ++f) {
-      double a = 0;
-#pragma unroll
-      for (int d = 0; d < 8; ++d) a += sdof[w][f][d] * phi[d];
-      v[f] = a;
-    }
-    double V[NO], F[NO][3];
-    ph.cell(v, x, V, F);
-#pragma unroll
-    for (int o = 0; o < NO; ++o) {
-      svol[w][o][l][0] = V[o] * JxW;
-#pragma unroll
-      for (int e = 0; e < 3; ++e) svol[w][o][l][1 + e] = (Ji[e][0] * F[o][0] + Ji[e][1] * F[o][1] + Ji[e][2] * F[o][2]) * JxW;
-    }
-  }
-  if (Phys::FACES && active && l < 27) {
+    for (int f = 0; f < NF; ++f) lat_vals(&sdof[f][w * 8], kGQ[a], kGQ[b], vb[f], vt[f]);
 #pragma unroll 1
-    for (int h = 0; h < 2; ++h) {
-      const int pt = l + 27 * h, f = pt / 9, s = pt % 9, gs = s % 3, gt = s / 3, e = f >> 1;
-      double xi[3], x[3], J[3][3], Ji[3][3], phi[8];
-      face_xi(f, gs, gt, xi);
-      shell_map(sN[w], sr0[w], m.dr, xi, x, J);
-      const double det = inv3(J, Ji);
-      const double sg = (f & 1) ? det : -det;
-      double nv[3] = {sg * Ji[e][0], sg * Ji[e][1], sg * Ji[e][2]};
-      const double area = sqrt(dot3(nv, nv));
-      const double nrm[3] = {nv[0] / area, nv[1] / area, nv[2] / area};
-      const double JxW = area * kGW[gs] * kGW[gt];
-      q1_all(xi, phi);
-      double vp[NF], vm[NF], Fv[NO];
+    for (int c = 0; c < 3; ++c) {
+      const double z = kGQ[c], r = r0 + dr * z, w3 = kGW[a] * kGW[b] * kGW[c];
+      double v[NF], x[3], V[NO], F[NO][3];
 #pragma unroll
-      for (int q = 0; q < NF; ++q) {
-        double a = 0;
+      for (int f = 0; f < NF; ++f) v[f] = vb[f] + z * (vt[f] - vb[f]);
 #pragma unroll
-        for (int d = 0; d < 8; ++d) a += sdof[w][q][d] * phi[d];
-        vp[q] = a;
+      for (int d = 0; d < 3; ++d) x[d] = r * G.N[d];
+      ph.cell(v, x, V, F);
+      const double JxW = r * r * dr * G.det * w3, s01 = r * dr * w3, s2 = r * r * w3;
+      const int q = t + 9 * c;
+#pragma unroll
+      for (int o = 0; o < NO; ++o) {
+        svol[w][o][q][0] = V[o] * JxW;
+        svol[w][o][q][1] = dot3(G.C0, F[o]) * s01;
+        svol[w][o][q][2] = dot3(G.C1, F[o]) * s01;
+        svol[w][o][q][3] = dot3(G.C2, F[o]) * s2;
       }
-      const int nbr = m.nb[(long)cell * 6 + f];
-      if (nbr < 0) {
-        ph.wall(vp, nrm, Fv);
-      } else {
-        const int code = m.nbc[(long)cell * 6 + f];
-        const int g = code & 7, gs2 = (code >> 3) ? 2 - gs : gs;
-        double xn[3], pn[8];
-        face_xi(g, gs2, gt, xn);
-        q1_all(xn, pn);
-        // the neighbour's dofs on its face g: index along axis g/2 equal to g & 1
-        const int ax = g >> 1, side = g & 1;
-#pragma unroll
-        for (int q = 0; q < NF; ++q) {
-          const double* src = ph.src[q] + (long)nbr * 8;
-          double a = 0;
-#pragma unroll
-          for (int d = 0; d < 8; ++d)
-            if (((d >> ax) & 1) == side) a += src[d] * pn[d];
-          vm[q] = a;
-        }
-        ph.face(vp, vm, nrm, Fv);
-      }
-#pragma unroll
-      for (int o = 0; o < NO; ++o) sface[w][o][Phys::FACES ? pt : 0] = Fv[o] * JxW;
-    }
-  }
-  __syncthreads();
-  if (active && l < 8 * NO) {
-    const int i = l & 7, o = l >> 3;
-    const int ia = i & 1, ib = (i >> 1) & 1, ic = i >> 2;
-    double acc = 0;
-#pragma unroll 3
-    for (int q = 0; q < 27; ++q) {
-      const int q0 = q % 3, q1 = (q / 3) % 3, q2 = q / 9;
-      const double la = q1v(ia, kGQ[q0]), lb = q1v(ib, kGQ[q1]), lc = q1v(ic, kGQ[q2]);
-      acc += svol[w][o][q][0] * la * lb * lc + svol[w][o][q][1] * q1d(ia) * lb * lc +
-             svol[w][o][q][2] * la * q1d(ib) * lc + svol[w][o][q][3] * la * lb * q1d(ic);
     }
     if (Phys::FACES) {
-      const int idx[3] = {ia, ib, ic};
+      // bottom (4) and top (5) faces of the column (unrolled: `own` names one register array)
 #pragma unroll
-      for (int f = 0; f < 6; ++f) {
-        const int e = f >> 1;
-        if (idx[e] != (f & 1)) continue;  // phi_i vanishes on this face
-        const int s0 = idx[e == 0 ? 1 : 0], s1 = idx[e == 2 ? 1 : 2];
+      for (int fz = 4; fz < 6; ++fz) {
+        const double rr = fz == 4 ? r0 : r0 + dr, sg = fz == 4 ? -1.0 : 1.0;
+        const double ws = sg * rr * rr * kGW[a] * kGW[b];
+        const double Av[3] = {ws * G.C2[0], ws * G.C2[1], ws * G.C2[2]};
+        const double* own = fz == 4 ? vb : vt;
+        double Fv[NO];
+        const int nbr = m.nb[(long)cell * 6 + fz];
+        if (nbr < 0) {
+          ph.wall(own, Av, Fv);
+        } else {
+          const int code = m.nbc[(long)cell * 6 + fz];
+          const int g = code & 7, a2 = (code >> 3) ? 2 - a : a;
+          double vm[NF];
 #pragma unroll
-        for (int s = 0; s < 9; ++s)
-          acc += sface[w][o][Phys::FACES ? f * 9 + s : 0] * q1v(s0, kGQ[s % 3]) * q1v(s1, kGQ[s / 3]);
+          for (int f = 0; f < NF; ++f) {
+            const double* u = ph.src[f] + (long)nbr * 8 + (g == 5 ? 4 : 0);
+            const double xi = kGQ[a2], eta = kGQ[b];
+            vm[f] = u[0] * (1 - xi) * (1 - eta) + u[1] * xi * (1 - eta) + u[2] * (1 - xi) * eta + u[3] * xi * eta;
+          }
+          ph.face(own, vm, Av, Fv);
+        }
+#pragma unroll
+        for (int o = 0; o < NO; ++o) sface[w][o][Phys::FACES ? fz * 9 + t : 0] = Fv[o];
+      }
+      // lateral faces: pairs (f, e) of face f < 4 and edge point e
+#pragma unroll 1
+      for (int pr = t; pr < 12; pr += 9) {
+        const int f = pr / 3, e = pr % 3;
+        const double xi = f < 2 ? (double)(f & 1) : kGQ[e], eta = f < 2 ? kGQ[e] : (double)(f & 1);
+        LatGeo H;
+        lat_geo(sS[w], xi, eta, H);
+        const double sg = (f & 1) ? 1.0 : -1.0;
+        // area direction C0 (xi faces) or C1 (eta faces), blended arithmetically: a select
+        // between the two member arrays becomes a select of addresses, which spills H
+        const double k0 = f < 2 ? 1.0 : 0.0, k1 = 1.0 - k0;
+        double Cf[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) Cf[d] = k0 * H.C0[d] + k1 * H.C1[d];
+        double ob[NF], ot[NF], nb_[NF], nt[NF];
+#pragma unroll
+        for (int q = 0; q < NF; ++q) lat_vals(&sdof[q][w * 8], xi, eta, ob[q], ot[q]);
+        const int nbr = m.nb[(long)cell * 6 + f];
+        if (nbr >= 0) {
+          // the neighbour's trace along its face g at edge point e2: its two bottom and
+          // two top dofs on that face, linear in the coordinate along the edge
+          const int code = m.nbc[(long)cell * 6 + f];
+          const int g = code & 7, e2 = (code >> 3) ? 2 - e : e;
+          const int i0 = g < 2 ? (g & 1) : 2 * (g & 1), step = g < 2 ? 2 : 1;
+          const double y = kGQ[e2];
+#pragma unroll
+          for (int q = 0; q < NF; ++q) {
+            const double* u = ph.src[q] + (long)nbr * 8 + i0;
+            nb_[q] = (1 - y) * u[0] + y * u[step];
+            nt[q] = (1 - y) * u[4] + y * u[4 + step];
+          }
+        }
+#pragma unroll 1
+        for (int gt = 0; gt < 3; ++gt) {
+          const double z = kGQ[gt], r = r0 + dr * z;
+          const double ws = sg * r * dr * kGW[e] * kGW[gt];
+          const double Av[3] = {ws * Cf[0], ws * Cf[1], ws * Cf[2]};
+          double vp[NF], Fv[NO];
+#pragma unroll
+          for (int q = 0; q < NF; ++q) vp[q] = ob[q] + z * (ot[q] - ob[q]);
+          if (nbr < 0) {
+            ph.wall(vp, Av, Fv);
+          } else {
+            double vm[NF];
+#pragma unroll
+            for (int q = 0; q < NF; ++q) vm[q] = nb_[q] + z * (nt[q] - nb_[q]);
+            ph.face(vp, vm, Av, Fv);
+          }
+#pragma unroll
+          for (int o = 0; o < NO; ++o) sface[w][o][Phys::FACES ? f * 9 + e + 3 * gt : 0] = Fv[o];
+        }
       }
     }
-    const long k = (long)o * Ns + (long)cell * 8 + i;
-    dst[k] = scale ? acc * scale[k] : acc;
+  }
+  __syncthreads();
+  if (active) {
+#pragma unroll 1
+    for (int it = t; it < 8 * NO; it += 9) {
+      const int i = it & 7, o = it >> 3;
+      const int ia = i & 1, ib = (i >> 1) & 1, ic = i >> 2;
+      double acc = 0;
+#pragma unroll 3
+      for (int q = 0; q < 27; ++q) {
+        const int q0 = q % 3, q1 = (q / 3) % 3, q2 = q / 9;
+        const double la = q1v(ia, kGQ[q0]), lb = q1v(ib, kGQ[q1]), lc = q1v(ic, kGQ[q2]);
+        acc += svol[w][o][q][0] * la * lb * lc + svol[w][o][q][1] * q1d(ia) * lb * lc +
+               svol[w][o][q][2] * la * q1d(ib) * lc + svol[w][o][q][3] * la * lb * q1d(ic);
+      }
+      if (Phys::FACES) {
+        // phi_i lives on three faces: xi = ia, eta = ib, zeta = ic; its face
+        // coordinates there are (ib, ic), (ia, ic), (ia, ib)
+#pragma unroll 1
+        for (int k3 = 0; k3 < 3; ++k3) {
+          const int f = k3 == 0 ? ia : (k3 == 1 ? 2 + ib : 4 + ic);
+          const int s0 = k3 == 0 ? ib : ia, s1 = k3 == 2 ? ib : ic;
+          const double* sf = &sface[w][o][Phys::FACES ? f * 9 : 0];
+          double facc = 0;
+#pragma unroll
+          for (int gt = 0; gt < 3; ++gt) {
+            double row = 0;
+#pragma unroll
+            for (int gs = 0; gs < 3; ++gs) row += sf[gs + 3 * gt] * q1v(s0, kGQ[gs]);
+            facc += row * q1v(s1, kGQ[gt]);
+          }
+          acc += facc;
+        }
+      }
+      const long k = (long)o * Ns + (long)cell * 8 + i;
+      dst[k] = scale ? acc * scale[k] : acc;
+    }
   }
 }
 
 template <class Phys>
 void launch(const ShellDev& m, const Phys& ph, double* dst, const double* scale, hipStream_t s) {
-  const int blocks = (m.ncell + 1) / 2;
+  const int blocks = (m.ncell + SCPW - 1) / SCPW;
   hipLaunchKernelGGL(k_shell<Phys>, dim3(blocks), dim3(64), 0, s, m, ph, dst, (long)m.ncell * 8, scale);
 }
 
@@ -549,32 +592,35 @@ __host__ __device__ constexpr int sym8(int a, int b) { return a <= b ? a * (15 -
 __global__ void __launch_bounds__(64) k_shell_blocks(ShellDev m, const double* __restrict__ rho,
                                                      double* __restrict__ inv, double* __restrict__ inv_diag,
                                                      double* __restrict__ bad) {
-  __shared__ double sN[2][9][3];
-  __shared__ double sr0[2];
-  __shared__ double swq[2][27];
-  __shared__ double sB[2][36];
-  const int lane = threadIdx.x, w = lane >> 5, l = lane & 31;
-  const int cell = cell_block() * 2 + w;
-  const bool active = cell < m.ncell;
-  if (active) cell_directions(m, cell, l, sN[w], &sr0[w]);
-  __syncthreads();
-  if (active && l < 27) {
-    const double xi[3] = {kGQ[l % 3], kGQ[(l / 3) % 3], kGQ[l / 9]};
-    double x[3], J[3][3], Ji[3][3], phi[8];
-    shell_map(sN[w], sr0[w], m.dr, xi, x, J);
-    double wq = inv3(J, Ji) * kGW[l % 3] * kGW[(l / 3) % 3] * kGW[l / 9];
-    if (rho) {
-      q1_all(xi, phi);
-      double r = 0;
-#pragma unroll
-      for (int d = 0; d < 8; ++d) r += rho[(long)cell * 8 + d] * phi[d];
-      wq *= r;
-    }
-    swq[w][l] = wq;
+  constexpr int CPW = SCPW;
+  __shared__ double sS[CPW][9][3];
+  __shared__ double swq[CPW][27];
+  __shared__ double sB[CPW][36];
+  const int lane = threadIdx.x;
+  const int w = lane / 9, t = lane - 9 * w;
+  const int cell = blockIdx.x * CPW + w;
+  const bool active = w < CPW && cell < m.ncell;
+  double r0 = 0;
+  if (active) {
+    support_dir(m, cell, t, sS[w][t]);
+    r0 = m.r_in + m.dr * ((cell / (m.n * m.n)) % m.n);
   }
   __syncthreads();
   if (active) {
-    for (int e = l; e < 36; e += 32) {
+    const int a = t % 3, b = t / 3;
+    LatGeo G;
+    lat_geo(sS[w], kGQ[a], kGQ[b], G);
+    double rb = 1.0, rt = 1.0;
+    if (rho) lat_vals(rho + (long)cell * 8, kGQ[a], kGQ[b], rb, rt);
+    for (int c = 0; c < 3; ++c) {
+      const double z = kGQ[c], r = r0 + m.dr * z;
+      const double wq = r * r * m.dr * G.det * kGW[a] * kGW[b] * kGW[c];
+      swq[w][t + 9 * c] = rho ? wq * (rb + z * (rt - rb)) : wq;
+    }
+  }
+  __syncthreads();
+  if (active) {
+    for (int e = t; e < 36; e += 9) {
       int a = 0;
       while (sym8(a, 7) < e) ++a;
       const int b = e - sym8(a, a) + a;
@@ -589,7 +635,7 @@ __global__ void __launch_bounds__(64) k_shell_blocks(ShellDev m, const double* _
     }
   }
   __syncthreads();
-  if (active && l == 0) {
+  if (active && t == 0) {
     double L[8][8];
     bool spd = true;
     for (int j = 0; j < 8; ++j) {
@@ -598,9 +644,9 @@ __global__ void __launch_bounds__(64) k_shell_blocks(ShellDev m, const double* _
       spd = spd && s > 0.0;  // fails for a density that is not positive in the cell (or NaN)
       L[j][j] = sqrt(s);
       for (int i = j + 1; i < 8; ++i) {
-        double t = sB[w][sym8(j, i)];
-        for (int k = 0; k < j; ++k) t -= L[i][k] * L[j][k];
-        L[i][j] = t / L[j][j];
+        double u = sB[w][sym8(j, i)];
+        for (int k = 0; k < j; ++k) u -= L[i][k] * L[j][k];
+        L[i][j] = u / L[j][j];
       }
     }
     // Linv (lower), then inv = Linv^T Linv
@@ -611,15 +657,15 @@ __global__ void __launch_bounds__(64) k_shell_blocks(ShellDev m, const double* _
           Li[i][c] = 0.0;
           continue;
         }
-        double t = i == c ? 1.0 : 0.0;
-        for (int k = c; k < i; ++k) t -= L[i][k] * Li[k][c];
-        Li[i][c] = t / L[i][i];
+        double u = i == c ? 1.0 : 0.0;
+        for (int k = c; k < i; ++k) u -= L[i][k] * Li[k][c];
+        Li[i][c] = u / L[i][i];
       }
     for (int a = 0; a < 8; ++a)
       for (int b = a; b < 8; ++b) {
-        double t = 0;
-        for (int k = b; k < 8; ++k) t += Li[k][a] * Li[k][b];
-        inv[(long)cell * 36 + sym8(a, b)] = t;
+        double u = 0;
+        for (int k = b; k < 8; ++k) u += Li[k][a] * Li[k][b];
+        inv[(long)cell * 36 + sym8(a, b)] = u;
       }
     for (int a = 0; a < 8; ++a) inv_diag[(long)cell * 8 + a] = 1.0 / sB[w][sym8(a, a)];
     if (!spd && bad) *bad = 1.0;
@@ -702,28 +748,17 @@ void shell_rhs_pressure(const ShellDev& m, const ShellConsts& c, int stage, cons
     ph.src[5 * st] = src[3 * st];
     velocity_fields(src[3 * st + 1], Ns, &ph.src[5 * st + 1]);
     ph.w = stage_weights(tb, st);
-    if (st == 1) {
+    if (st == 1) {  // triple 0, then the kinetic pair (rho_tmp_2, u_fixed)
       ph.gwt[0] = tb.a21t;
-      ph.gsl[0] = 0;
       ph.gwt[1] = tb.a22t;
-      ph.gsl[1] = -1;
-      ph.ngr = 2;
-    } else if (st == 2) {
+    } else if (st == 2) {  // triples 0 and 1, then the kinetic pair (rho_tmp_3, u_fixed)
       ph.gwt[0] = tb.a31t;
-      ph.gsl[0] = 0;
       ph.gwt[1] = tb.a32t;
-      ph.gsl[1] = 1;
       ph.gwt[2] = tb.a33t;
-      ph.gsl[2] = -1;
-      ph.ngr = 3;
-    } else {
+    } else {  // triples 0, 1 and 2
       ph.gwt[0] = tb.b1;
-      ph.gsl[0] = 0;
       ph.gwt[1] = tb.b2;
-      ph.gsl[1] = 1;
       ph.gwt[2] = tb.b3;
-      ph.gsl[2] = 2;
-      ph.ngr = 3;
     }
     ph.dt = c.dt;
     ph.Ma2 = c.Ma * c.Ma;
@@ -850,7 +885,7 @@ void shell_mass(const ShellDev& m, int ncomp, const double* rho, double coeff, c
 
 void shell_block_inverse(const ShellDev& m, const double* rho, double* inv, double* inv_diag, double* bad,
                          hipStream_t s) {
-  hipLaunchKernelGGL(k_shell_blocks, dim3((m.ncell + 1) / 2), dim3(64), 0, s, m, rho, inv, inv_diag, bad);
+  hipLaunchKernelGGL(k_shell_blocks, dim3((m.ncell + SCPW - 1) / SCPW), dim3(64), 0, s, m, rho, inv, inv_diag, bad);
 }
 
 void shell_block_apply(const ShellDev& m, int ncomp, const double* inv, double coeff, const double* b, double* x,
