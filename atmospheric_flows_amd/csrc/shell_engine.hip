@@ -244,6 +244,9 @@ class ShellEngine : public Engine {
     vec_copy(Ns, rho_curr.p, rho_old.p, s_);
     vec_copy(3 * Ns, u_curr.p, u_old.p, s_);
     launch_diag_flow<1>(MeshDev{}, rho_old.p, u_old.p, p_old.p, P.gamma_gas, Ns, diag_part.p, diag_rec.p, s_);
+    ATMO_HIP_CHECK(hipMemcpyAsync(hscal + 250, scal.p + 250, sizeof(double), hipMemcpyDeviceToHost, s_));
+    sync();
+    if (hscal[250] != 0.0) throw_bad_density();
   }
 
   void step_diagnostics(double* rec, bool of_state) override {
@@ -348,6 +351,13 @@ class ShellEngine : public Engine {
       tmp1, rhs_u, rhs_p, rhs_u0;
 
   ShellConsts consts() const { return ShellConsts{P.Ma, P.Fr, P.dt, P.gamma_gas}; }
+
+  // A's cell blocks were not all positive definite: the fixed point's density
+  // is not positive everywhere (the reference's CG on A would not converge)
+  [[noreturn]] static void throw_bad_density() {
+    throw std::runtime_error("non-finite A^{-1}: the density of the fixed point is not positive in every cell "
+                             "(A is not positive definite)");
+  }
   void sync() { ATMO_HIP_CHECK(hipStreamSynchronize(s_)); }
 
   // S p = D p - C A^{-1} B p (apply_add NS 2, euler_operator.h:2374-2408), times scale when given
@@ -471,9 +481,7 @@ class ShellEngine : public Engine {
     if (state == -1) {
       ATMO_HIP_CHECK(hipMemcpyAsync(hscal + 250, scal.p + 250, sizeof(double), hipMemcpyDeviceToHost, s_));
       sync();
-      if (hscal[250] != 0.0)
-        throw std::runtime_error("non-finite Schur complement: the density of the fixed point is not positive "
-                                 "in every cell (A is not positive definite)");
+      if (hscal[250] != 0.0) throw_bad_density();
       throw NoConvergence("GMRES: no convergence");
     }
     st_.gmres_its += acc;
@@ -561,7 +569,9 @@ class ShellEngine : public Engine {
       vec_linf(Ns, p_fixed.p, nullptr, scal.p + 2, rb, s_);
       vec_linf(Ns, p_fixed.p, p_fixed_old.p, scal.p + 3, rb, s_);
       ATMO_HIP_CHECK(hipMemcpyAsync(hscal + 2, scal.p + 2, 2 * sizeof(double), hipMemcpyDeviceToHost, s_));
+      ATMO_HIP_CHECK(hipMemcpyAsync(hscal + 250, scal.p + 250, sizeof(double), hipMemcpyDeviceToHost, s_));
       sync();
+      if (hscal[250] != 0.0) throw_bad_density();
       const double err = hscal[3] / (hscal[2] + 1e-10);
       if (P.fp_lock > 0) {
         if (iter + 1 >= P.fp_lock) break;

@@ -403,16 +403,16 @@ constexpr int SCPW = 7;
 // the 12 (lateral face, edge point) pairs of the cell go to lanes t and
 // t - 9, each with its three radial points. Face fluxes get the area vector
 // (unit normal times JxW) in place of the normal: every flux is positively
-// homogeneous of degree 1 in the normal. Lanes then integrate (test
-// function, component) items from the LDS records.
+// homogeneous of degree 1 in the normal. Each lane accumulates its points'
+// contributions to all 8 NOUT (test function, component) integrals in
+// registers; the nine lanes of a cell are then summed through LDS.
 template <class Phys>
 __global__ void __launch_bounds__(64) k_shell(ShellDev m, Phys ph, double* __restrict__ dst, long Ns,
                                               const double* __restrict__ scale) {
-  constexpr int NF = Phys::NF, NO = Phys::NOUT, CPW = SCPW;
+  constexpr int NF = Phys::NF, NO = Phys::NOUT, CPW = SCPW, NI = 8 * NO;
   __shared__ double sdof[NF][CPW * 8];
   __shared__ double sS[CPW][9][3];
-  __shared__ double svol[CPW][NO][27][4];
-  __shared__ double sface[CPW][NO][Phys::FACES ? 54 : 1];
+  __shared__ double sred[CPW][9][NI];
   const int lane = threadIdx.x;
   const int w = lane / 9, t = lane - 9 * w;
   const int c0 = cell_block() * CPW;
@@ -431,8 +431,12 @@ __global__ void __launch_bounds__(64) k_shell(ShellDev m, Phys ph, double* __res
   }
   __syncthreads();
   const double dr = m.dr;
+  double acc[NI];
+#pragma unroll
+  for (int k = 0; k < NI; ++k) acc[k] = 0.0;
   if (active) {
     const int a = t % 3, b = t / 3;
+    const double la[2] = {1.0 - kGQ[a], kGQ[a]}, lb[2] = {1.0 - kGQ[b], kGQ[b]};
     LatGeo G;
     lat_geo(sS[w], kGQ[a], kGQ[b], G);
     double vb[NF], vt[NF];
@@ -441,6 +445,7 @@ __global__ void __launch_bounds__(64) k_shell(ShellDev m, Phys ph, double* __res
 #pragma unroll 1
     for (int c = 0; c < 3; ++c) {
       const double z = kGQ[c], r = r0 + dr * z, w3 = kGW[a] * kGW[b] * kGW[c];
+      const double lc[2] = {1.0 - z, z};
       double v[NF], x[3], V[NO], F[NO][3];
 #pragma unroll
       for (int f = 0; f < NF; ++f) v[f] = vb[f] + z * (vt[f] - vb[f]);
@@ -448,13 +453,17 @@ __global__ void __launch_bounds__(64) k_shell(ShellDev m, Phys ph, double* __res
       for (int d = 0; d < 3; ++d) x[d] = r * G.N[d];
       ph.cell(v, x, V, F);
       const double JxW = r * r * dr * G.det * w3, s01 = r * dr * w3, s2 = r * r * w3;
-      const int q = t + 9 * c;
 #pragma unroll
       for (int o = 0; o < NO; ++o) {
-        svol[w][o][q][0] = V[o] * JxW;
-        svol[w][o][q][1] = dot3(G.C0, F[o]) * s01;
-        svol[w][o][q][2] = dot3(G.C1, F[o]) * s01;
-        svol[w][o][q][3] = dot3(G.C2, F[o]) * s2;
+        const double v0 = V[o] * JxW, g0 = dot3(G.C0, F[o]) * s01, g1 = dot3(G.C1, F[o]) * s01,
+                     g2 = dot3(G.C2, F[o]) * s2;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          const int ia = i & 1, ib = (i >> 1) & 1, ic = i >> 2;
+          const double sa = ia ? 1.0 : -1.0, sb = ib ? 1.0 : -1.0, sc = ic ? 1.0 : -1.0;
+          acc[8 * o + i] += v0 * la[ia] * lb[ib] * lc[ic] + g0 * sa * lb[ib] * lc[ic] + g1 * la[ia] * sb * lc[ic] +
+                            g2 * la[ia] * lb[ib] * sc;
+        }
       }
     }
     if (Phys::FACES) {
@@ -482,7 +491,10 @@ __global__ void __launch_bounds__(64) k_shell(ShellDev m, Phys ph, double* __res
           ph.face(own, vm, Av, Fv);
         }
 #pragma unroll
-        for (int o = 0; o < NO; ++o) sface[w][o][Phys::FACES ? fz * 9 + t : 0] = Fv[o];
+        for (int o = 0; o < NO; ++o)
+#pragma unroll
+          for (int i = (fz - 4) * 4; i < (fz - 4) * 4 + 4; ++i)  // the test functions with ic = fz - 4
+            acc[8 * o + i] += Fv[o] * la[i & 1] * lb[(i >> 1) & 1];
       }
       // lateral faces: pairs (f, e) of face f < 4 and edge point e
 #pragma unroll 1
@@ -498,6 +510,15 @@ __global__ void __launch_bounds__(64) k_shell(ShellDev m, Phys ph, double* __res
         double Cf[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d) Cf[d] = k0 * H.C0[d] + k1 * H.C1[d];
+        // test functions on face f, by value: the lateral factor along the face's edge
+        // coordinate, zero off the face
+        double wl[4];  // i & 3 -> lateral weight of test function (i & 1, (i >> 1) & 1)
+        const double le[2] = {1.0 - kGQ[e], kGQ[e]};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int ia = q & 1, ib = q >> 1;
+          wl[q] = f < 2 ? (ia == (f & 1) ? le[ib] : 0.0) : (ib == (f & 1) ? le[ia] : 0.0);
+        }
         double ob[NF], ot[NF], nb_[NF], nt[NF];
 #pragma unroll
         for (int q = 0; q < NF; ++q) lat_vals(&sdof[q][w * 8], xi, eta, ob[q], ot[q]);
@@ -532,47 +553,26 @@ __global__ void __launch_bounds__(64) k_shell(ShellDev m, Phys ph, double* __res
             for (int q = 0; q < NF; ++q) vm[q] = nb_[q] + z * (nt[q] - nb_[q]);
             ph.face(vp, vm, Av, Fv);
           }
+          const double lz[2] = {1.0 - z, z};
 #pragma unroll
-          for (int o = 0; o < NO; ++o) sface[w][o][Phys::FACES ? f * 9 + e + 3 * gt : 0] = Fv[o];
+          for (int o = 0; o < NO; ++o)
+#pragma unroll
+            for (int i = 0; i < 8; ++i) acc[8 * o + i] += Fv[o] * wl[i & 3] * lz[i >> 2];
         }
       }
     }
+#pragma unroll
+    for (int k = 0; k < NI; ++k) sred[w][t][k] = acc[k];
   }
   __syncthreads();
   if (active) {
 #pragma unroll 1
-    for (int it = t; it < 8 * NO; it += 9) {
-      const int i = it & 7, o = it >> 3;
-      const int ia = i & 1, ib = (i >> 1) & 1, ic = i >> 2;
-      double acc = 0;
-#pragma unroll 3
-      for (int q = 0; q < 27; ++q) {
-        const int q0 = q % 3, q1 = (q / 3) % 3, q2 = q / 9;
-        const double la = q1v(ia, kGQ[q0]), lb = q1v(ib, kGQ[q1]), lc = q1v(ic, kGQ[q2]);
-        acc += svol[w][o][q][0] * la * lb * lc + svol[w][o][q][1] * q1d(ia) * lb * lc +
-               svol[w][o][q][2] * la * q1d(ib) * lc + svol[w][o][q][3] * la * lb * q1d(ic);
-      }
-      if (Phys::FACES) {
-        // phi_i lives on three faces: xi = ia, eta = ib, zeta = ic; its face
-        // coordinates there are (ib, ic), (ia, ic), (ia, ib)
-#pragma unroll 1
-        for (int k3 = 0; k3 < 3; ++k3) {
-          const int f = k3 == 0 ? ia : (k3 == 1 ? 2 + ib : 4 + ic);
-          const int s0 = k3 == 0 ? ib : ia, s1 = k3 == 2 ? ib : ic;
-          const double* sf = &sface[w][o][Phys::FACES ? f * 9 : 0];
-          double facc = 0;
+    for (int it = t; it < NI; it += 9) {
+      double sum = 0.0;
 #pragma unroll
-          for (int gt = 0; gt < 3; ++gt) {
-            double row = 0;
-#pragma unroll
-            for (int gs = 0; gs < 3; ++gs) row += sf[gs + 3 * gt] * q1v(s0, kGQ[gs]);
-            facc += row * q1v(s1, kGQ[gt]);
-          }
-          acc += facc;
-        }
-      }
-      const long k = (long)o * Ns + (long)cell * 8 + i;
-      dst[k] = scale ? acc * scale[k] : acc;
+      for (int l = 0; l < 9; ++l) sum += sred[w][l][it];
+      const long k = (long)(it >> 3) * Ns + (long)cell * 8 + (it & 7);
+      dst[k] = scale ? sum * scale[k] : sum;
     }
   }
 }

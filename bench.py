@@ -296,6 +296,108 @@ def run_amr(args):
     return out
 
 
+SPHERE_SBR = dict(Omega=0.5, a=1.0, g=1.0, RT=1.0)
+
+
+def run_sphere(args):
+    """The reference's own 3D case (solid-body rotation on the spherical
+    shell, FE_DGQ(1), Q2 mapping, radial gravity). Timed workload: the shell
+    of radii 0.9 and 1 (the reference's, in units of its radius) refined
+    --sphere-refinements times (6 panels), in the non-dimensional setting
+    R T = g = a = 1 (Ma = Fr = 1) with Omega = 0.5, where the atmosphere spans
+    a fraction of a scale height and the analytic state is resolved; dt at
+    radial acoustic Courant number 1; GMRES eps 1e-8. Also: the reference's
+    own parameters (parameter-file.prm) on its Earth-size shell, whose first
+    step is reported as it ends; the oracle on the same timed case at 3
+    refinements beside the GPU (CPU baseline and parity)."""
+    import atmospheric_flows_amd as af
+    from oracle.pyoracle_shell import FROUDE, MACH
+    L = args.sphere_refinements
+    dt = (0.1 / 2 ** L) / np.sqrt(1.4)
+    ctx = af.ShellContext(L, 6, r_inner=0.9, r_outer=1.0, planet=1.0, Ma=1.0, Fr=1.0, dt=dt, eps=1e-8)
+    rho, u, p = ctx.sbr_state(**SPHERE_SBR)
+    ctx.upload_state(rho, u, p)
+    ctx.imex_step(1)
+    ctx.reset_stats()
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.imex_step(args.sphere_steps)
+    ctx.sync()
+    el = time.perf_counter() - t0
+    dofs = 5 * ctx.ns
+    err = ctx.l2_errors(**SPHERE_SBR)
+    out = dict(workload=f"sphere_sbr_shell0.9-1_refine{L}_Q1", cells=ctx.ncells, dofs=dofs, dt=dt, steps=args.sphere_steps,
+               dof_steps_per_s=dofs * args.sphere_steps / el, ms_per_step=1e3 * el / args.sphere_steps,
+               stats=ctx.stats(), rel_l2_err_vs_steady_state={k: v[1] for k, v in err.items()},
+               setting="non-dimensional: R T = g = a = 1 (Ma = Fr = 1), Omega = 0.5, shell radii 0.9-1, 6 panels, "
+                       "dt at radial acoustic CFL 1, GMRES eps 1e-8")
+    print(f"[bench] sphere refine {L}: {args.sphere_steps} steps in {el:.3f} s", file=sys.stderr, flush=True)
+    # kernels: id -> (name, algorithmic fields per launch)
+    kern = {}
+    ms_copy, b_copy = ctx.time_kernel(7, args.reps)
+    kern["copy_u"] = dict(ms=ms_copy, bytes=b_copy, gbs=b_copy / ms_copy / 1e6)
+    for kid, nm in ((9, "schur_d_minus_c"), (3, "B"), (4, "C"), (5, "rhs_pressure_stage3"),
+                    (14, "rhs_velocity_stage3"), (15, "rhs_rho_stage3"), (2, "mass")):
+        ms, b = ctx.time_kernel(kid, args.reps)
+        kern[nm] = dict(ms=ms, bytes=b, gbs=b / ms / 1e6)
+    out["kernels"] = kern
+    dom = kern["schur_d_minus_c"]
+    out["roofline"] = dict(bound="hbm", kernel="k_shell<PhysC<true>> (GMRES's scaled D - C)", achieved=dom["gbs"],
+                           peak=HBM_PEAK_GBS, unit="GB/s", frac=dom["gbs"] / HBM_PEAK_GBS,
+                           bytes_per_launch=dom["bytes"], ms_per_launch=dom["ms"])
+    state = ctx.download_state()
+    del ctx
+    # the reference's own parameters on its Earth-size shell (3 refinements of 6 panels)
+    ref = dict(refinements=3, Ma=MACH, Fr=FROUDE, dt=8.64, shell="0.9 a - a, a = 6.37122e6 m")
+    c3 = af.ShellContext(3, 6, Ma=MACH, Fr=FROUDE, dt=8.64, eps=1e-8)
+    c3.upload_state(*c3.sbr_state())
+    try:
+        c3.imex_step(1)
+        ref["status"] = "ok"
+    except af.AtmoError as e:
+        ref["status"] = f"atmo status {e.status}"
+        ref["error"] = str(e)
+    ref["min_density_after_stage1_update"] = c3.step_diagnostics()["min_density"][0]
+    ref["note"] = ("the shell spans ~80 isothermal scale heights (the analytic density grows by e^88 from top to "
+                   "bottom, ~e^11 across one cell at 3 refinements); the first density update of Q1 on such cells is "
+                   "not positive, so the fixed point's A block is not positive definite (the oracle stops at the "
+                   "same point)")
+    del c3
+    out["reference_parameters"] = ref
+    if not args.no_cpu:
+        out["cpu"] = sphere_cpu(args.cpu_threads, min(L, 3), args.sphere_cpu_steps)
+    return out
+
+
+def sphere_cpu(threads, L, steps):
+    """The shell oracle on the timed sphere case at L refinements for
+    `steps` steps (after one untimed), beside the GPU on the same steps."""
+    import atmospheric_flows_amd as af
+    from oracle.pyoracle_shell import ShellOracle, lib
+    lib().orc3_set_threads(threads)
+    dt = (0.1 / 2 ** L) / np.sqrt(1.4)
+    o = ShellOracle(L, r_inner=0.9, r_outer=1.0, planet_radius=1.0)
+    o.set_params(Ma=1.0, Fr=1.0, dt=dt, eps=1e-8)
+    s0 = o.sbr_state(**SPHERE_SBR)
+    o.set_state(*s0)
+    o.step(1)
+    t0 = time.perf_counter()
+    o.step(steps)
+    el = time.perf_counter() - t0
+    want = o.get_state()
+    ctx = af.ShellContext(L, 6, r_inner=0.9, r_outer=1.0, planet=1.0, Ma=1.0, Fr=1.0, dt=dt, eps=1e-8)
+    ctx.upload_state(*s0)
+    ctx.imex_step(steps + 1)
+    got = ctx.download_state()
+    nproc, model = host_cpu()
+    dofs = 5 * o.Ns
+    return dict(value=dofs * steps / el, unit="DoF-steps/s", cores=threads, kind="port", host_nproc=nproc,
+                cpu_model=model, sample=f"{steps} IMEX steps (after 1 untimed) of the sphere case at {L} refinements "
+                                        f"({o.ncell} cells, {dofs} DoF), {el:.1f} s wall",
+                gpu_rel_err_vs_oracle={nm: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+                                       for nm, a, b in zip(("rho", "u", "p"), got, want)})
+
+
 def run_serve(args, precision):
     """Serving stream: Poisson arrivals at --serve-rate req/s for
     --serve-seconds s; each request is one hill-flow member with mesh drawn
@@ -444,6 +546,11 @@ def main():
     # one GPU's share of the host is 16 threads (OMP_NUM_THREADS on the GPU hosts)
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", 0)) or
                     min(16, os.cpu_count() or 1))
+    ap.add_argument("--no-sphere", action="store_true", help="skip the spherical-shell (3D) case")
+    ap.add_argument("--sphere-refinements", type=int, default=5)
+    ap.add_argument("--sphere-steps", type=int, default=5)
+    ap.add_argument("--sphere-cpu-steps", type=int, default=3)
+    ap.add_argument("--sphere-only", action="store_true", help="only the spherical-shell case (profiling runs)")
     ap.add_argument("--cpu-headline-steps", type=int, default=0,
                     help="only the CPU oracle on the headline workload for this many steps (no GPU)")
     args = ap.parse_args()
@@ -454,12 +561,15 @@ def main():
     if args.kernels_only:
         kernels_only(args)
         return
+    if args.sphere_only:
+        print(json.dumps({"sphere": run_sphere(args)}))
+        return
     if args.cpu_headline_steps:
         print(json.dumps({"cpu_headline": cpu_headline(args.cpu_threads, args.cpu_headline_steps)}))
         return
     traffic = None
     if args.mixed_only:
-        args.no_traffic = args.no_ensemble = args.no_amr = args.no_cpu = args.no_serve = True
+        args.no_traffic = args.no_ensemble = args.no_amr = args.no_cpu = args.no_serve = args.no_sphere = True
     if not args.no_traffic:
         # counter passes run in child processes before this process touches the GPU
         nx, nz = (args.nx or WORKLOAD["nx"]), (args.nz or WORKLOAD["nz"])
@@ -495,6 +605,7 @@ def main():
                     ms_per_launch=dom["ms"], traffic_detail=traffic)
     ensemble = None if args.no_ensemble else run_ensemble(args)
     amr = None if args.no_amr else run_amr(args)
+    sphere = None if args.no_sphere else run_sphere(args)
     serve = None if args.no_serve else {"mixed": run_serve(args, 1), "fp64": run_serve(args, 0)}
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_threads)
     out = {
@@ -518,7 +629,7 @@ def main():
         "solver_stats": {f: r[f]["stats"] for f in ("mixed", "fp64_direct", "fp64") if f in r},
         "converged_fp64_seconds": r["tight"]["seconds"] if "tight" in r else None,
         "kernels": r.get("kernels"), "roofline": roof, "cpu_baseline": cpu, "ensemble": ensemble,
-        "serve": serve, "amr": amr,
+        "serve": serve, "amr": amr, "sphere": sphere,
     }
     print(json.dumps(out))
 
