@@ -158,4 +158,44 @@ void Ensemble::run() {
   wall_s_ = now_s() - t0_;
 }
 
+int Ensemble::autotune(const std::vector<int>& candidates, int probe_steps, int sample, std::vector<double>& rates) {
+  if (candidates.empty()) throw std::invalid_argument("autotune: no candidates");
+  if (probe_steps < 1 || sample < 1) throw std::invalid_argument("autotune: probe_steps and sample must be >= 1");
+  for (int c : candidates)
+    if (c < 1 || c > 64) throw std::invalid_argument("autotune: candidate slot counts must be 1..64");
+  const int n = std::min<int>(sample, (int)specs_.size());
+  if (n < 1) throw std::invalid_argument("autotune: submit members first");
+  rates.assign(candidates.size(), 0.0);
+  int best = slots_;
+  double best_rate = -1;
+  for (size_t ci = 0; ci < candidates.size(); ++ci) {
+    Ensemble probe(candidates[ci], budget_, false);
+    double dof_steps = 0;
+    for (int i = 0; i < n; ++i) {
+      MemberSpec m = specs_[i];
+      m.nsteps = probe_steps;
+      m.arrival_s = 0.0;
+      probe.submit(m);
+      const double nd = m.degree + 1;
+      dof_steps += 4.0 * m.mesh.nx * m.mesh.nz * nd * nd * probe_steps;
+    }
+    probe.run();
+    double t_first = 1e300, t_last = 0;
+    bool ok = true;
+    for (int i = 0; i < n; ++i) {
+      const MemberResult& r = probe.result(i);
+      ok = ok && r.status == 0;
+      t_first = std::min(t_first, r.start_s);
+      t_last = std::max(t_last, r.finish_s);
+    }
+    rates[ci] = ok && t_last > t_first ? dof_steps / (t_last - t_first) : 0.0;
+    if (rates[ci] > best_rate) {
+      best_rate = rates[ci];
+      best = candidates[ci];
+    }
+  }
+  slots_ = best;
+  return best;
+}
+
 }  // namespace atmo

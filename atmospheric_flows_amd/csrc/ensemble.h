@@ -47,6 +47,14 @@ class Ensemble {
   const MemberResult& result(int id) const;
   int size() const { return (int)specs_.size(); }
   double wall_s() const { return wall_s_; }
+  int slots() const { return slots_; }
+  // Batch-shape autotuner: runs the first `sample` submitted members (all
+  // arriving at once) for `probe_steps` steps with each candidate number of
+  // slots, measures the aggregate DoF-steps/s of the stepping phase (from the
+  // first member's first step to the last member's last), keeps the best
+  // candidate as this ensemble's slot count and returns it; rates[i] is
+  // candidate i's rate (0 when a member of the probe failed).
+  int autotune(const std::vector<int>& candidates, int probe_steps, int sample, std::vector<double>& rates);
 
  private:
   void worker();

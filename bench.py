@@ -236,12 +236,18 @@ def run_ensemble(args):
         ens.submit(m)
         desc.append(dict(id=i, k=k, nx=nx, nz=nz, h0=h0, U=U, N=Nbv, dt=dt))
         dofs += 4 * nx * nz * (k + 1) ** 2 * args.ens_steps
+    tuned = None
+    if args.ens_autotune:
+        slots, rates = ens.autotune(candidates=(4, 8, 12, 16, 24), probe_steps=8, sample=32)
+        tuned = dict(candidates=rates, chosen=slots, probe="first 32 members, 8 steps each, all arriving at once")
+        args.ens_slots = slots
     print(f"[bench] ensemble: {args.ens_members} members x {args.ens_steps} steps, {args.ens_slots} slots",
           file=sys.stderr, flush=True)
     wall = ens.run()
     res = [ens.result(i) for i in range(args.ens_members)]
     failed = [r["error"] for r in res if r["status"] != 0]
-    out = dict(members=args.ens_members, steps_per_member=args.ens_steps, slots=args.ens_slots, wall_s=wall,
+    out = dict(members=args.ens_members, steps_per_member=args.ens_steps, slots=args.ens_slots, autotune=tuned,
+               wall_s=wall,
                dof_steps_per_s=dofs / wall, failed=len(failed), errors=sorted(set(failed))[:3],
                failed_members=[dict(desc[i], steps_done=r["steps_done"], error=r["error"])
                                for i, r in enumerate(res) if r["status"] != 0][:8])
@@ -543,6 +549,8 @@ def main():
     ap.add_argument("--ens-members", type=int, default=64)
     ap.add_argument("--ens-steps", type=int, default=100)
     ap.add_argument("--ens-slots", type=int, default=8)
+    ap.add_argument("--no-ens-autotune", dest="ens_autotune", action="store_false",
+                    help="keep --ens-slots instead of picking the slot count by measurement")
     # one GPU's share of the host is 16 threads (OMP_NUM_THREADS on the GPU hosts)
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", 0)) or
                     min(16, os.cpu_count() or 1))

@@ -338,6 +338,14 @@ typedef struct {
 atmo_status atmo_ensemble_create(int slots, double hbm_budget_bytes, int keep_states, atmo_ensemble** out);
 atmo_status atmo_ensemble_destroy(atmo_ensemble* e);
 atmo_status atmo_ensemble_submit(atmo_ensemble* e, const atmo_member_desc* m, int* id);
+/* No counterpart in the reference: the batch-shape autotuner. Runs the first
+ * sample_members submitted members (arriving at once) for probe_steps steps
+ * with each of the ncand candidate slot counts (1..64), measures the
+ * aggregate DoF-steps/s of the stepping phase, and keeps the best as the
+ * ensemble's slot count (*best_slots); rates (may be NULL) receives each
+ * candidate's rate (0 when a member of that probe failed). */
+atmo_status atmo_ensemble_autotune(atmo_ensemble* e, const int* candidates, int ncand, int probe_steps,
+                                   int sample_members, int* best_slots, double* rates);
 /* Runs every submitted member; returns when all are done (wall_s may be NULL). */
 atmo_status atmo_ensemble_run(atmo_ensemble* e, double* wall_s);
 /* Result of member id; step_ms (may be NULL) receives up to cap per-step wall times. */

@@ -94,3 +94,23 @@ def test_ensemble_mesh_set_alone_and_in_the_mix():
         alone = run_alone(m)
         for a, b in zip(ens.state(i), alone):
             assert np.array_equal(a, b), (m.nx, m.degree)
+
+
+def test_autotune_picks_a_measured_slot_count():
+    """The batch-shape autotuner runs the first members on each candidate
+    slot count, keeps the fastest, and the ensemble then runs with it."""
+    ens = af.Ensemble(slots=1, keep_states=True)
+    ms = [af.Member(2 + i % 2, 16 * (1 + i % 2), 8 * (1 + i % 2), nsteps=3, seed=40 + i, params=dict(dt=8.0))
+          for i in range(6)]
+    for m in ms:
+        ens.submit(m)
+    best, rates = ens.autotune(candidates=(1, 2, 4), probe_steps=2, sample=6)
+    assert best in (1, 2, 4) and all(r > 0 for r in rates.values())
+    assert rates[best] == max(rates.values())
+    ens.run()
+    for i, m in enumerate(ms):
+        r = ens.result(i)
+        assert r["status"] == 0 and r["steps_done"] == 3
+    # the probes leave the members' own results untouched: member 0 equals its run alone
+    for a, b in zip(ens.state(0), run_alone(ms[0])):
+        assert np.array_equal(a, b)
