@@ -8,6 +8,8 @@
 // reference's face-loop contribution to the cell without any atomics; every
 // face flux is computed twice (once per side).
 #pragma once
+#include <type_traits>
+
 #include "common.h"
 #include "wave.h"
 
@@ -17,6 +19,22 @@ namespace atmo {
 template <int NF>
 struct FieldPtrs {
   const double* p[NF];
+};
+
+// Bottom/top faces of the structured mesh: neighbour face rows of this many
+// fields are loaded together (one memory latency per group; larger groups
+// cost the operators of many fields occupancy).
+constexpr int FACE_ROW_GROUP = 4;
+
+// Number of leading input fields a physics' face flux reads: Ph::NF_FACE
+// where the physics declares it, all NF fields otherwise.
+template <class Ph, class = void>
+struct FaceFields {
+  static constexpr int value = Ph::NF;
+};
+template <class Ph>
+struct FaceFields<Ph, std::void_t<decltype(Ph::NF_FACE)>> {
+  static constexpr int value = Ph::NF_FACE;
 };
 
 // B: euler_operator.h:1510-1617 (cell: submit_divergence, face: {p}[v.n],
@@ -86,6 +104,7 @@ struct PhysC {
 // (:1384-1408) enters as a cell value term, C with the opposite sign.
 struct PhysDminusC {
   static constexpr int NF = 4, NO = 1;
+  static constexpr int NF_FACE = 3;  // the face flux is C's: p enters the cell term only
   static constexpr bool BDRY = false;
   static constexpr bool HAS_V = true;
   static constexpr unsigned GX = 0b1, GZ = 0b1;
@@ -313,7 +332,10 @@ __device__ inline void cellface_columns(const MeshDev& m, const Ph& ph, const Fi
       }
     } else if constexpr (!AMR) {
       const int cj = L.cell / m.nx, ci = L.cell - cj * m.nx;
-      // bottom (j-1) and top (j+1) faces: this lane is face point c
+      // bottom (j-1) and top (j+1) faces: this lane is face point c. Per face, the loads of the neighbour's
+      // face row of a group of fields the flux reads are issued before the first of them is used (one
+      // memory latency per group)
+      constexpr int NFF = FaceFields<Ph>::value, RG = FACE_ROW_GROUP;
 #pragma unroll
       for (int side = 0; side < 2; ++side) {
         const int f = 2 + side;
@@ -325,9 +347,26 @@ __device__ inline void cellface_columns(const MeshDev& m, const Ph& ph, const Fi
         for (int q = 0; q < NF; ++q) own[q] = face_trace<K>(&sh.in[q][off], f, Sc);
         const int nj = side ? cj + 1 : cj - 1;
         if (nj >= 0 && nj < m.nz) {
-          const long nbase = (long)(nj * m.nx + ci) * ND2;
+          // row K of the cell below, row 0 of the cell above; groups of up to RG fields
+          const long nbase = (long)(nj * m.nx + ci) * ND2 + (side ? 0 : K * ND);
 #pragma unroll
-          for (int q = 0; q < NF; ++q) nb[q] = face_trace<K>(in.p[q] + nbase, f ^ 1, Sc);
+          for (int q0 = 0; q0 < NF; q0 += RG) {
+            double row[RG][ND];
+#pragma unroll
+            for (int q = 0; q < RG; ++q)
+#pragma unroll
+              for (int i = 0; i < ND; ++i) row[q][i] = q0 + q < NFF ? in.p[q0 + q < NFF ? q0 + q : 0][nbase + i] : 0.0;
+            // the trace of the row (face_trace's order of terms); fields the flux does not read are 0
+#pragma unroll
+            for (int q = 0; q < RG; ++q) {
+              if (q0 + q >= NF) continue;
+              double t = 0;
+              if (q0 + q < NFF)
+#pragma unroll
+                for (int i = 0; i < ND; ++i) t += Sc[i] * row[q][i];
+              nb[q0 + q] = t;
+            }
+          }
           ph.face(own, nb, fg.n0, fg.n1, F);
         } else if (Ph::BDRY) {
           ph.bdry(own, fg.n0, fg.n1, F);

@@ -455,25 +455,22 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(AMR ? 1 : 4)
   const int blk = cell_block();
   const long base = (long)blk * CPW * ND2;
   const long lim = (long)m.ncell * ND2;
-  // the Jacobi scale of the block's dofs is loaded with the staged fields (in flight during the column phase)
+  cellface_columns<K, PhysDminusC, AMR>(m, ph, in, sh, NoPrefetch(), [&]() {
+    if (!p_last) return;
+#pragma unroll
+    for (int ps = 0; ps < NPASS; ++ps) {
+      const int t = threadIdx.x + 64 * ps;
+      if (t < CPW * ND2) pk[t] = sh.in[3][(t / ND2 + SM::HALO) * SM::CS + t % ND2];
+    }
+  });
+  // the Jacobi scale of the block's dofs, loaded once the column phase is done (loaded with the staged
+  // fields, it stays live through the column phase, whose registers then spill: measured slower)
   double sc[NPASS];
-  cellface_columns<K, PhysDminusC, AMR>(
-      m, ph, in, sh,
-      [&]() {
 #pragma unroll
-        for (int ps = 0; ps < NPASS; ++ps) {
-          const int t = threadIdx.x + 64 * ps;
-          sc[ps] = (t < CPW * ND2 && base + t < lim) ? scale[base + t] : 0.0;
-        }
-      },
-      [&]() {
-        if (!p_last) return;
-#pragma unroll
-        for (int ps = 0; ps < NPASS; ++ps) {
-          const int t = threadIdx.x + 64 * ps;
-          if (t < CPW * ND2) pk[t] = sh.in[3][(t / ND2 + SM::HALO) * SM::CS + t % ND2];
-        }
-      });
+  for (int ps = 0; ps < NPASS; ++ps) {
+    const int t = threadIdx.x + 64 * ps;
+    sc[ps] = (t < CPW * ND2 && base + t < lim) ? scale[base + t] : 0.0;
+  }
   OutRows<K> R = out_rows<K>(threadIdx.x);
   // the block's dofs: full passes of 64 lanes, then a tail of TL < 64 dofs
   constexpr int NFULL = (CPW * ND2) / 64, TL = CPW * ND2 - 64 * NFULL;
