@@ -32,7 +32,7 @@ class Params(ctypes.Structure):
                 ("a_solve_tol", ctypes.c_double), ("fp_max_iter", ctypes.c_int), ("fp_tol", ctypes.c_double),
                 ("fp_lock", ctypes.c_int), ("mg_min_cells", ctypes.c_int), ("cheb_degree", ctypes.c_int),
                 ("cheb_range", ctypes.c_double), ("cheb_eig_its", ctypes.c_int), ("gmres_restart", ctypes.c_int),
-                ("precision", ctypes.c_int)]
+                ("precision", ctypes.c_int), ("mg_matrix_cores", ctypes.c_int)]
 
 
 class ShellDesc(ctypes.Structure):

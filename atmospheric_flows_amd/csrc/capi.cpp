@@ -95,6 +95,7 @@ static EngineParams to_engine(const atmo_params& p) {
   e.cheb_eig_its = p.cheb_eig_its;
   e.gmres_restart = p.gmres_restart;
   e.precision = p.precision;
+  e.mg_matrix_cores = p.mg_matrix_cores;
   return e;
 }
 
@@ -108,6 +109,8 @@ static void check_params(const atmo_params& p) {
   if (p.gmres_restart < 1 || p.gmres_restart > MAX_GMRES - 2)
     throw std::invalid_argument("params: gmres_restart must be in [1, 62]");
   if (p.precision < 0 || p.precision > 2) throw std::invalid_argument("params: precision must be 0, 1 or 2");
+  if (p.mg_matrix_cores != 0 && p.mg_matrix_cores != 1)
+    throw std::invalid_argument("params: mg_matrix_cores must be 0 or 1");
   if (!(p.eps >= 0) || !(p.schur_inner_tol >= 0) || !(p.a_solve_tol >= 0))
     throw std::invalid_argument("params: tolerances must be >= 0");
   if (p.mg_min_cells < 1) throw std::invalid_argument("params: mg_min_cells must be >= 1");
@@ -150,6 +153,7 @@ void atmo_params_default(atmo_params* p) {
   p->cheb_eig_its = e.cheb_eig_its;
   p->gmres_restart = e.gmres_restart;
   p->precision = e.precision;
+  p->mg_matrix_cores = e.mg_matrix_cores;
 }
 
 const char* atmo_version(void) { return "atmo_hip 0.1 (gfx950)"; }

@@ -33,6 +33,8 @@ struct EngineParams {
   // 0: FP64 with the reference's solvers; 1: mixed (FP32 MG levels, cellwise-direct FP64 A^{-1});
   // 2: the mixed form's FP64 comparator (FP64 MG levels, the same cellwise-direct A^{-1})
   int precision = 0;
+  // FP32 multigrid sweeps at k = 2, 3 on the matrix cores (MFMA) instead of the vector ALUs
+  int mg_matrix_cores = 1;
 };
 
 // One cell of a locally refined mesh: base cell (bi, bj), quadtree level l,

@@ -332,8 +332,27 @@ class EngineK : public Engine {
       jcg_init(Ns, m, K, s_t1.p, fine().inv64[BLK_A].p, in_g.p, in_d.p, st, rb, s_);
       jcg_apply<K>(1, m, rho_fix.p, fine().inv64[BLK_A].p, in_g.p, in_d.p, in_h.p, st, rb, Ns, s_);
     }
+    ChebSteps cs_t;  // sweep coefficients for the timing of 14/15 (the kernels' work does not depend on them)
+    if (which == 14 || which == 15) {
+      if (!mg32()) throw std::invalid_argument("time_kernel 14/15 need the mixed form (precision 1)");
+      cs_t.n = P.cheb_degree;
+      for (int s = 0; s < cs_t.n; ++s) {
+        cs_t.f1[s] = s ? 0.5 : 0.0;
+        cs_t.f2[s] = 0.25;
+      }
+      vec_cast<float, double>(2 * Ns, u_old.p, fine().b32.p, s_);
+    }
     auto launch = [&]() {
       switch (which) {
+        case 14:  // FP32 multigrid pre-smoothing sweep + residual of the A block on the finest level: matrix cores
+        case 15:  // the same sweep on the vector ALUs
+        {
+          Level& F = fine();
+          launch_presmooth_residual<K, float>(F.m, 2, true, F.x32.p, F.rho32.p, (float)coeff(BLK_A), F.b32.p,
+                                              F.inv32[BLK_A].p, cs_t, F.t32.p, Ns, which == 14, s_);
+          fields = 0.5 * 9;  // FP32: rho, b (2), P (2) in; x (2), res (2) out
+          break;
+        }
         case 0:
           launch_mass<K, double>(m, 2, true, u_old.p, rho_fix.p, in_h.p, 1.0, nullptr, 0.0, Ns, s_);
           fields = 5;
@@ -876,13 +895,14 @@ class EngineK : public Engine {
     const T* rw = nullptr;
     if (blk == BLK_A) rw = std::is_same<T, double>::value ? (const T*)L.rho64.p : (const T*)L.rho32.p;
     const ChebSteps cs = cheb_steps(l, blk);
-    launch_presmooth_residual<K, T>(L.m, nc, blk == BLK_A, x, rw, (T)coeff(blk), b, Pd, cs, t, L.Ns, s_);
+    launch_presmooth_residual<K, T>(L.m, nc, blk == BLK_A, x, rw, (T)coeff(blk), b, Pd, cs, t, L.Ns, P.mg_matrix_cores,
+                                    s_);
     Level& C = *lv[l - 1];
     launch_restrict<K, T>(L.m, C.m, t, LB<T>(C), nc, L.Ns, C.Ns, s_);
     vcycle<T>(l - 1, blk, LX<T>(C), LB<T>(C), tol, max_its);
     // coarse-grid correction and post-smoothing: one kernel
     launch_prolongate_postsmooth<K, T>(L.m, C.m, nc, blk == BLK_A, x, LX<T>(C), rw, (T)coeff(blk), b, Pd, cs, L.Ns,
-                                       C.Ns, s_);
+                                       C.Ns, P.mg_matrix_cores, s_);
   }
 
   // PreconditionMG::vmult: dst = V-cycle(src) on the fine level

@@ -28,15 +28,17 @@ struct ChebSteps {
 // One Chebyshev sweep of a level from a zero guess (the V-cycle's
 // pre-smoothing) fused with the residual it leaves: x = S(b), res = b - A x.
 // The mass-type operators are cell-local, so every step runs in one kernel.
+// matrix_cores: FP32 sweeps at k = 2, 3 on the matrix cores (k_cheb_sweep_mx);
+// otherwise, and for every other type and degree, on the vector ALUs.
 template <int K, typename T>
 void launch_presmooth_residual(const MeshDev& m, int ncomp, bool weighted, T* x, const T* rho, T coeff, const T* b,
-                               const T* P, const ChebSteps& cs, T* res, long Ns, hipStream_t s);
+                               const T* P, const ChebSteps& cs, T* res, long Ns, bool matrix_cores, hipStream_t s);
 // x += prolongation of the coarse level's xc, then one Chebyshev sweep from
 // that guess (the V-cycle's coarse-grid correction and post-smoothing).
 template <int K, typename T>
 void launch_prolongate_postsmooth(const MeshDev& m, const MeshDev& mc, int ncomp, bool weighted, T* x, const T* xc,
                                   const T* rho, T coeff, const T* b, const T* P, const ChebSteps& cs, long Ns,
-                                  long Nc, hipStream_t s);
+                                  long Nc, bool matrix_cores, hipStream_t s);
 
 // Single-workgroup CG on the coarsest level (FP64, identity preconditioner,
 // absolute tolerance read from *tol, at most max_its iterations). Work

@@ -3,6 +3,7 @@
 // the coarse level's child map and the fine level's parent map say which
 // cells were merged (DG embedding) and which are the same cell (identity).
 #include "mg.h"
+#include "mfma.h"
 #include "wave.h"
 
 namespace atmo {
@@ -302,12 +303,186 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
   }
 }
 
+// The same sweep on the matrix cores (FP32, k = 2 and 3): 16 cells per wave,
+// four lanes per cell, every vector of the sweep (x, d, b, P) and the
+// quadrature weights in registers in the layout of mfma.h, each operator
+// application 33 (k = 2) or 45 (k = 3) MFMA outer-product steps with no LDS
+// and no barrier. Same steps and update expressions as k_cheb_sweep; the
+// contractions sum in another order, so the values differ from it at FP32
+// rounding level.
+constexpr int MX_CELLS = 16;
+
+template <int K, int NC, bool W, bool ZG, bool PRO, bool RES>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_cheb_sweep_mx(MeshDev m, MeshDev mc, float* __restrict__ x,
+                                                      const float* __restrict__ xc, const float* __restrict__ rho,
+                                                      float coeff, const float* __restrict__ b,
+                                                      const float* __restrict__ P, ChebSteps cs,
+                                                      float* __restrict__ res, long Ns, long Nc) {
+  static_assert(K == 2 || K == 3, "matrix-core sweep: k = 2 or 3");
+  constexpr int ND = K + 1, NQ = 2 * K + 1, ND2 = ND * ND;
+  const int lane = threadIdx.x & 63, l4 = lane & 3;
+  const int cell = cell_block() * MX_CELLS + (lane >> 2);
+  const bool live = cell < m.ncell && l4 < ND;  // the lane's dof row exists
+  const long row = live ? (long)cell * ND2 + l4 * ND : 0;
+  const VertsRaw vr = cell_verts_load(m, cell < m.ncell ? cell : 0);
+  float xv[NC][4], bv[NC][4], pv[NC][4], dv[NC][4], rv[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const bool on = live && i < ND;
+    const long k = row + (i < ND ? i : 0);
+#pragma unroll
+    for (int comp = 0; comp < NC; ++comp) {
+      xv[comp][i] = (!ZG && on) ? x[comp * Ns + k] : 0.f;
+      bv[comp][i] = on ? b[comp * Ns + k] : 0.f;
+      pv[comp][i] = on ? P[comp * Ns + k] : 0.f;
+      dv[comp][i] = 0.f;
+    }
+    rv[i] = (W && on) ? rho[k] : 0.f;
+  }
+  const MxTab<K> T(l4);
+  if constexpr (PRO) {
+    // x += prolongation of the parent cell's coarse values (DG embedding), or the identity; the expressions
+    // of k_cheb_sweep
+    if (live) {
+      int cc, a = 0, bb = 0;
+      bool ident = false;
+      if (m.parent) {
+        cc = m.parent[2 * cell];
+        const int code = m.parent[2 * cell + 1];
+        if (code < 0) ident = true;
+        else {
+          a = code & 1;
+          bb = code >> 1;
+        }
+      } else {
+        const int i = cell % m.nx, j = cell / m.nx;
+        cc = (j >> 1) * mc.nx + (i >> 1);
+        a = i & 1;
+        bb = j & 1;
+      }
+      const int jj = l4;
+#pragma unroll
+      for (int comp = 0; comp < NC; ++comp) {
+        const float* u = xc + comp * Nc + (long)cc * ND2;
+#pragma unroll
+        for (int ii = 0; ii < ND; ++ii) {
+          if (ident) {
+            xv[comp][ii] += u[jj * ND + ii];
+          } else {
+            const double* Pa = cTr<K>.P[a][ii];
+            const double* Pb = cTr<K>.P[bb][jj];
+            double acc = 0;
+#pragma unroll
+            for (int mm = 0; mm < ND; ++mm) {
+              const double pb = Pb[mm];
+#pragma unroll
+              for (int n = 0; n < ND; ++n) acc += pb * Pa[n] * (double)u[mm * ND + n];
+            }
+            xv[comp][ii] += (float)acc;
+          }
+        }
+      }
+    }
+  }
+  // quadrature weights of the lane's points (qz = 4tz + r, qx = 4tx + l4): JxW * coeff (column_weights'
+  // expressions), times the density there when W; zero at the padding points
+  mx_f4 w[2][2];
+  {
+    float rv1[1][4] = {{rv[0], rv[1], rv[2], rv[3]}};
+    mx_f4 R[1][2][2];
+    if constexpr (W) mx_interp<K, 1>(rv1, T, R);
+    const CellVerts v = cell_verts_form(m, cell < m.ncell ? cell : 0, vr);
+#pragma unroll
+    for (int tx = 0; tx < 2; ++tx) {
+      const int qx = 4 * tx + l4;
+      float jc = 0.f;
+      if (qx < NQ) {
+        const ColGeo<float> g = col_geo<K, float>(v, qx);
+        jc = g.dx * g.ze * g.wc * coeff;
+      }
+#pragma unroll
+      for (int tz = 0; tz < 2; ++tz)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int qz = 4 * tz + r;
+          float wq = qz < NQ ? jc * (float)cT<K>.gw[qz < NQ ? qz : 0] : 0.f;
+          if constexpr (W) wq *= R[0][tz][tx][r];
+          w[tz][tx][r] = wq;
+        }
+    }
+  }
+  for (int s = 0; s < cs.n; ++s) {
+    const float f1 = (float)cs.f1[s], f2 = (float)cs.f2[s];
+    if (ZG && s == 0) {
+#pragma unroll
+      for (int comp = 0; comp < NC; ++comp)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const float v = pv[comp][i] * bv[comp][i] * f2;
+          dv[comp][i] = v;
+          xv[comp][i] = v;
+        }
+      continue;
+    }
+    mx_f4 ax[NC];
+    mx_mass<K, NC>(xv, w, T, ax);
+#pragma unroll
+    for (int comp = 0; comp < NC; ++comp)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const float v = f1 * dv[comp][i] + f2 * pv[comp][i] * (bv[comp][i] - ax[comp][i]);
+        dv[comp][i] = v;
+        xv[comp][i] = xv[comp][i] + v;
+      }
+  }
+  if constexpr (RES) {
+    mx_f4 ax[NC];
+    mx_mass<K, NC>(xv, w, T, ax);
+    if (live)
+#pragma unroll
+      for (int comp = 0; comp < NC; ++comp)
+#pragma unroll
+        for (int i = 0; i < ND; ++i) res[comp * Ns + row + i] = -ax[comp][i] + bv[comp][i];
+  }
+  if (live)
+#pragma unroll
+    for (int comp = 0; comp < NC; ++comp)
+#pragma unroll
+      for (int i = 0; i < ND; ++i) x[comp * Ns + row + i] = xv[comp][i];
+}
+
+template <int K, bool ZG, bool PRO, bool RES>
+static void sweep_launch_mx(const MeshDev& m, const MeshDev& mc, int ncomp, bool weighted, float* x,
+                            const float* xc, const float* rho, float coeff, const float* b, const float* P,
+                            const ChebSteps& cs, float* res, long Ns, long Nc, hipStream_t s) {
+  const int grid = (m.ncell + MX_CELLS - 1) / MX_CELLS;
+  if (weighted) {
+    if (ncomp != 2) throw std::invalid_argument("weighted mass operator expects 2 components");
+    hipLaunchKernelGGL((k_cheb_sweep_mx<K, 2, true, ZG, PRO, RES>), dim3(grid), dim3(64), 0, s, m, mc, x, xc, rho,
+                       coeff, b, P, cs, res, Ns, Nc);
+  } else if (ncomp == 1) {
+    hipLaunchKernelGGL((k_cheb_sweep_mx<K, 1, false, ZG, PRO, RES>), dim3(grid), dim3(64), 0, s, m, mc, x, xc, rho,
+                       coeff, b, P, cs, res, Ns, Nc);
+  } else {
+    hipLaunchKernelGGL((k_cheb_sweep_mx<K, 2, false, ZG, PRO, RES>), dim3(grid), dim3(64), 0, s, m, mc, x, xc, rho,
+                       coeff, b, P, cs, res, Ns, Nc);
+  }
+  ATMO_HIP_CHECK(hipGetLastError());
+}
+
 template <int K, typename T, bool ZG, bool PRO, bool RES>
 static void sweep_launch(const MeshDev& m, const MeshDev& mc, int ncomp, bool weighted, T* x, const T* xc,
                          const T* rho, T coeff, const T* b, const T* P, const ChebSteps& cs, T* res, long Ns, long Nc,
-                         hipStream_t s) {
+                         bool matrix_cores, hipStream_t s) {
   const int grid = (m.ncell + Cfg<K>::CPW - 1) / Cfg<K>::CPW;
   if (grid == 0) return;
+  if constexpr (std::is_same<T, float>::value && (K == 2 || K == 3)) {
+    if (matrix_cores) {
+      sweep_launch_mx<K, ZG, PRO, RES>(m, mc, ncomp, weighted, x, xc, rho, coeff, b, P, cs, res, Ns, Nc, s);
+      return;
+    }
+  }
+  (void)matrix_cores;
   if (weighted) {
     if (ncomp != 2) throw std::invalid_argument("weighted mass operator expects 2 components");
     hipLaunchKernelGGL((k_cheb_sweep<K, T, 2, true, ZG, PRO, RES>), dim3(grid), dim3(64), 0, s, m, mc, x, xc, rho,
@@ -324,17 +499,19 @@ static void sweep_launch(const MeshDev& m, const MeshDev& mc, int ncomp, bool we
 
 template <int K, typename T>
 void launch_presmooth_residual(const MeshDev& m, int ncomp, bool weighted, T* x, const T* rho, T coeff, const T* b,
-                               const T* P, const ChebSteps& cs, T* res, long Ns, hipStream_t s) {
+                               const T* P, const ChebSteps& cs, T* res, long Ns, bool matrix_cores, hipStream_t s) {
   if (cs.n < 1 || cs.n > ChebSteps::MAX) throw std::invalid_argument("Chebyshev degree out of range");
-  sweep_launch<K, T, true, false, true>(m, m, ncomp, weighted, x, nullptr, rho, coeff, b, P, cs, res, Ns, 0, s);
+  sweep_launch<K, T, true, false, true>(m, m, ncomp, weighted, x, nullptr, rho, coeff, b, P, cs, res, Ns, 0,
+                                        matrix_cores, s);
 }
 
 template <int K, typename T>
 void launch_prolongate_postsmooth(const MeshDev& m, const MeshDev& mc, int ncomp, bool weighted, T* x, const T* xc,
                                   const T* rho, T coeff, const T* b, const T* P, const ChebSteps& cs, long Ns,
-                                  long Nc, hipStream_t s) {
+                                  long Nc, bool matrix_cores, hipStream_t s) {
   if (cs.n < 1 || cs.n > ChebSteps::MAX) throw std::invalid_argument("Chebyshev degree out of range");
-  sweep_launch<K, T, false, true, false>(m, mc, ncomp, weighted, x, xc, rho, coeff, b, P, cs, nullptr, Ns, Nc, s);
+  sweep_launch<K, T, false, true, false>(m, mc, ncomp, weighted, x, xc, rho, coeff, b, P, cs, nullptr, Ns, Nc,
+                                         matrix_cores, s);
 }
 
 // ------------------------------------------------------------ coarse CG
@@ -575,16 +752,17 @@ void launch_eig_cg_batch(const EigBatch& B, int ncomp, bool weighted, double coe
   template void launch_project_down<K>(const MeshDev&, const MeshDev&, const double*, double*, hipStream_t);      \
   template void launch_presmooth_residual<K, float>(const MeshDev&, int, bool, float*, const float*, float,          \
                                                     const float*, const float*, const ChebSteps&, float*, long,      \
-                                                    hipStream_t);                                                    \
+                                                    bool, hipStream_t);                                                    \
   template void launch_presmooth_residual<K, double>(const MeshDev&, int, bool, double*, const double*, double,      \
                                                      const double*, const double*, const ChebSteps&, double*, long,  \
-                                                     hipStream_t);                                                   \
+                                                     bool, hipStream_t);                                                   \
   template void launch_prolongate_postsmooth<K, float>(const MeshDev&, const MeshDev&, int, bool, float*,            \
                                                        const float*, const float*, float, const float*, const float*, \
-                                                       const ChebSteps&, long, long, hipStream_t);                   \
+                                                       const ChebSteps&, long, long, bool, hipStream_t);             \
   template void launch_prolongate_postsmooth<K, double>(const MeshDev&, const MeshDev&, int, bool, double*,          \
                                                         const double*, const double*, double, const double*,         \
-                                                        const double*, const ChebSteps&, long, long, hipStream_t);   \
+                                                        const double*, const ChebSteps&, long, long, bool,           \
+                                                        hipStream_t);                                                \
   template void launch_coarse_cg<K>(const MeshDev&, int, bool, const double*, double, const double*, double*,     \
                                     double*, double*, double*, const double*, int, int*, long, hipStream_t);       \
   template void launch_eig_cg_batch<K>(const EigBatch&, int, bool, double, int, double, hipStream_t);

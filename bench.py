@@ -57,6 +57,8 @@ def run_gpu(args):
         # 1e-14: separates the effect of the reference's inner-solve truncation from that of the mixed form
         forms.append(("fp64_inner", dict(precision=0, a_solve_tol=1e-14, schur_inner_tol=1e-14)))
     if not args.mixed_only:
+        # the mixed form with its FP32 multigrid sweeps on the vector ALUs instead of the matrix cores
+        forms.append(("mixed_vector_alus", dict(precision=1, mg_matrix_cores=0)))
         # the comparator again from the initial density perturbed by a few ulps (relative 1e-15): how far two
         # FP64 runs of one algorithm drift apart at these solver tolerances when only rounding differs
         forms.append(("fp64_direct_ulp", dict(precision=2)))
@@ -106,6 +108,8 @@ def run_gpu(args):
 
     if "fp64" in states:
         res["rel_err_mixed_vs_fp64"] = rel_errs(states["mixed"], states["fp64"])
+    if "mixed_vector_alus" in states:
+        res["rel_err_mixed_vs_vector_alus"] = rel_errs(states["mixed"], states["mixed_vector_alus"])
     if "fp64_direct" in states:
         res["rel_err_mixed_vs_fp64_direct"] = rel_errs(states["mixed"], states["fp64_direct"])
         res["rel_err_fp64_direct_vs_fp64"] = rel_errs(states["fp64_direct"], states["fp64"])
@@ -123,10 +127,31 @@ def run_gpu(args):
 KERNELS = ((8, "k_b_ainv"), (9, "k_d_minus_c"), (12, "k_dmc_dots_14"), (13, "k_dmc_dots_0"), (10, "gm_mdot_14"),
            (11, "gm_maxpy_norm_13"), (1, "k_jcg_apply"),
            (6, "k_jcg_upd"), (0, "k_mass_A"), (2, "k_mass_M"), (3, "k_cellface_B"), (4, "k_cellface_C"),
-           (5, "k_cellface_rhs_p3"), (7, "copy_2field"))
+           (5, "k_cellface_rhs_p3"), (7, "copy_2field"), (14, "k_cheb_sweep_mx_A"), (15, "k_cheb_sweep_A"))
 # dominant kernel of the mixed form (the measured `value`): the Arnoldi step of GMRES, scaled D p - C y with
 # the dot products of its result against the basis (14 vectors: the mean basis size over a restart cycle of 28)
 DOMINANT = (12, "k_dmc_dots_14", "k_dmc_dots<")
+
+
+def matrix_core_summary(r, dofs, steps):
+    """The mixed form with its FP32 multigrid sweeps on the matrix cores (the
+    measured `value`) against the same form with the sweeps on the vector
+    ALUs: step throughput, solver counts, the state difference after the
+    same steps, and the finest-level pre-smoothing sweep of the velocity
+    block timed alone in both forms."""
+    if "mixed_vector_alus" not in r:
+        return None
+    mx = dofs * steps / r["mixed"]["seconds"]
+    va = dofs * steps / r["mixed_vector_alus"]["seconds"]
+    out = dict(value=mx, vector_alus_value=va, speedup=mx / va,
+               vector_alus_stats=r["mixed_vector_alus"]["stats"],
+               rel_err_vs_vector_alus=r.get("rel_err_mixed_vs_vector_alus"))
+    kern = r.get("kernels") or {}
+    if "k_cheb_sweep_mx_A" in kern and "k_cheb_sweep_A" in kern:
+        out["sweep_ms"] = kern["k_cheb_sweep_mx_A"]["ms"]
+        out["vector_alus_sweep_ms"] = kern["k_cheb_sweep_A"]["ms"]
+        out["sweep_speedup"] = kern["k_cheb_sweep_A"]["ms"] / kern["k_cheb_sweep_mx_A"]["ms"]
+    return out
 
 
 def kernels_only(args):
@@ -635,6 +660,7 @@ def main():
         "rel_err_vs_converged_fp64": r.get("rel_err_vs_converged_fp64"),
         "rel_err_vs_fp64_exact_inner": r.get("rel_err_vs_fp64_exact_inner"),
         "solver_stats": {f: r[f]["stats"] for f in ("mixed", "fp64_direct", "fp64") if f in r},
+        "matrix_cores": matrix_core_summary(r, dofs, args.steps),
         "converged_fp64_seconds": r["tight"]["seconds"] if "tight" in r else None,
         "kernels": r.get("kernels"), "roofline": roof, "cpu_baseline": cpu, "ensemble": ensemble,
         "serve": serve, "amr": amr, "sphere": sphere,

@@ -80,6 +80,8 @@ typedef struct {
                                   FP64 Krylov, FP64 cellwise-direct A^{-1} (inner Schur and A solves);
                                   2: the mixed form's FP64 comparator: the same cellwise-direct A^{-1}
                                   with FP64 multigrid (differs from 1 only in the levels' precision) */
+  int mg_matrix_cores;         /* 1: the FP32 multigrid sweeps (precision 1) at degree 2 and 3 run on the
+                                  matrix cores (FP32 MFMA sum factorization); 0: on the vector ALUs */
 } atmo_params;
 
 typedef struct {
@@ -252,7 +254,10 @@ atmo_status atmo_write_vtu_host(const atmo_mesh_desc* mesh, const double* rho, c
  * gmres_restart >= 13 for both), 12 the GMRES Arnoldi step's scaled D - C
  * with the dot products of its result against 14 basis vectors
  * (k_dmc_dots, partial sums only; applied to the newest basis vector, whose
- * staged values serve its own dot product), 13 the same kernel with no basis vector.
+ * staged values serve its own dot product), 13 the same kernel with no basis vector,
+ * 14 the FP32 multigrid pre-smoothing sweep with its residual on the finest level
+ * (velocity block A, mixed form only) on the matrix cores (k_cheb_sweep_mx), 15 the
+ * same sweep on the vector ALUs (k_cheb_sweep).
  * Also reports the kernel's algorithmic HBM bytes per launch. */
 atmo_status atmo_time_kernel(atmo_ctx* ctx, int which, int nrep, double* ms_per_launch, double* bytes_per_launch);
 
