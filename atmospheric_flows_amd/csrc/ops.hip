@@ -438,10 +438,9 @@ __device__ inline double wave_sum8(const double (&v)[8]) {
 // the partial dot products of w with V_0 .. V_{nv-1} (column k at V + k*ldv)
 // and w.w: slot k of block b at part[k*pstride + b], slot nv is w.w. The
 // Arnoldi step of GMRES: its Schur product and the classical Gram-Schmidt dot
-// products without a second pass over w. Slots go in chunks of 8 whose loads
-// are all in flight at once (slots past nv load nothing). With p_last, p is
-// column nv-1 of V (GMRES applies the operator to its newest basis vector):
-// that slot's values come from the staged copy of p instead of a second read.
+// products without a second pass over w. With p_last, p is column nv-1 of V
+// (GMRES applies the operator to its newest basis vector): that slot's values
+// come from the staged copy of p instead of a second read.
 template <int K, bool AMR>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(AMR ? 1 : 4))) void k_dmc_dots(MeshDev m, PhysDminusC ph, FieldPtrs<4> in, double* __restrict__ dst,
                                                  const double* __restrict__ scale, long Ns,
@@ -491,11 +490,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(AMR ? 1 : 4)
     }
   }
   const int l = threadIdx.x & 63;
-  // Dot products in chunks of 8 slots (slot k: column k of V, slot nv: w itself). Full passes: every lane
-  // multiplies its own dofs. Tail (TL dofs, TL/8 per slot when TL divides into 8 lanes' pairs): lanes 8s ..
-  // 8s+7 take slot s's tail as one 16-byte load of two dofs, with w's tail values fetched from the lanes
-  // that formed them, so a chunk takes 8*NFULL + 1 loads per lane instead of 8*NPASS. The basis is read with
-  // non-temporal loads (streamed, not kept in the caches).
+  // Dot products. Slots 0 .. nload-1 are basis columns read from memory, in chunks of 8 whose loads are
+  // all in flight at once (slots past nload load nothing and multiply zeros). Full passes: every lane
+  // multiplies its own dofs. Tail (TL dofs, TL/8 per slot when TL divides into 8 lanes' pairs): lanes
+  // 8s .. 8s+7 take slot s's tail as one 16-byte load of two dofs, with w's tail values fetched from the
+  // lanes that formed them, so a chunk takes 8*NFULL + 1 loads per lane instead of 8*NPASS. The basis is
+  // read with non-temporal loads (streamed, not kept in the caches). The other two slots need no loads
+  // and are reduced together after the chunks: with p_last, slot nv-1 is p (the staged copy), and slot
+  // nv is w itself.
   constexpr bool PAIRS = TL == 16;
   constexpr int NL = PAIRS ? NFULL : NPASS;  // per-lane single loads per slot
   double wt0 = 0.0, wt1 = 0.0;
@@ -507,16 +509,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(AMR ? 1 : 4)
     gt = base + 64 * NFULL + src;
     if (gt + 1 >= lim) gt = base;  // past the last cell: w's values there are zero
   }
+  const int nload = p_last ? nv - 1 : nv;  // the slots read from memory
+  typedef double nt_d2 __attribute__((ext_vector_type(2)));
   struct Chunk {
     double y[8][NL];
-    double2 t;
+    nt_d2 t;
   };
-  // the last slot that is read from memory
-  const int nload = p_last ? nv - 1 : nv;
   auto load = [&](int k0, Chunk& c) {
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
-      if (k0 + s < nload) {  // uniform: slots past the basis (and the staged p) load nothing
+      if (k0 + s < nload) {  // uniform
         const long col = (long)(k0 + s) * ldv;
 #pragma unroll
         for (int ps = 0; ps < NL; ++ps) c.y[s][ps] = __builtin_nontemporal_load(V + col + gi[ps]);
@@ -525,48 +527,61 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(AMR ? 1 : 4)
         for (int ps = 0; ps < NL; ++ps) c.y[s][ps] = 0.0;
       }
     }
+    c.t = nt_d2{0.0, 0.0};
     if constexpr (PAIRS) {
-      const int s = l >> 3;
-      c.t = make_double2(0.0, 0.0);
-      if (k0 + s < nload) {
-        typedef double nt_d2 __attribute__((ext_vector_type(2)));
-        const nt_d2 tv = __builtin_nontemporal_load(reinterpret_cast<const nt_d2*>(V + (long)(k0 + s) * ldv + gt));
-        c.t = make_double2(tv.x, tv.y);
-      }
+      const int k = k0 + (l >> 3);
+      if (k < nload) c.t = __builtin_nontemporal_load(reinterpret_cast<const nt_d2*>(V + (long)k * ldv + gt));
     }
-  };
-  // slot k's value at full pass ps: the loaded column, the staged p, w itself, or nothing
-  auto yv = [&](int k, const Chunk& c, int s, int ps) {
-    return k < nload ? c.y[s][ps] : (k < nv ? pk[threadIdx.x + 64 * ps] : (k == nv ? wv[ps] : 0.0));
   };
   auto reduce = [&](int k0, const Chunk& c) {
     double v[8];
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
-      const int k = k0 + s;
       double acc = 0.0;
 #pragma unroll
-      for (int ps = 0; ps < NL; ++ps) acc += wv[ps] * yv(k, c, s, ps);
+      for (int ps = 0; ps < NL; ++ps) acc += wv[ps] * c.y[s][ps];
       v[s] = acc;
     }
     if constexpr (PAIRS) {
-      const int k = k0 + (l >> 3);
-      const int tl = 64 * NFULL + 2 * (l & 7);  // the block-local dofs of this lane's tail pair
-      const double t0 = k < nload ? c.t.x : (k < nv ? pk[tl] : (k == nv ? wt0 : 0.0));
-      const double t1 = k < nload ? c.t.y : (k < nv ? pk[tl + 1] : (k == nv ? wt1 : 0.0));
-      const double tail = wt0 * t0 + wt1 * t1;
+      const double tail = wt0 * c.t.x + wt1 * c.t.y;
 #pragma unroll
       for (int s = 0; s < 8; ++s)
         if (s == (l >> 3)) v[s] += tail;
     }
     const double sum = wave_sum8(v);
     const int slot = k0 + (l >> 3);
-    if ((l & 7) == 0 && slot <= nv) part[(long)slot * pstride + blk] = sum;
+    if ((l & 7) == 0 && slot < nload) part[(long)slot * pstride + blk] = sum;
   };
-  for (int k0 = 0; k0 <= nv; k0 += 8) {
-    Chunk ca;
-    load(k0, ca);
-    reduce(k0, ca);
+  // w.p (with p_last) and w.w: this lane's parts from LDS and registers, reduced together while the first
+  // chunk's loads are in flight
+  auto specials = [&]() {
+    double v[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int ps = 0; ps < NL; ++ps) {
+      const int t = threadIdx.x + 64 * ps;  // past the block's dofs w is zero and pk has no entry
+      if (p_last && t < CPW * ND2) v[0] += wv[ps] * pk[t];
+      v[1] += wv[ps] * wv[ps];
+    }
+    if constexpr (PAIRS) {
+      if (l < 8) {
+        const int tl = 64 * NFULL + 2 * l;  // the block-local dofs of this lane's tail pair
+        if (p_last) v[0] += wt0 * pk[tl] + wt1 * pk[tl + 1];
+        v[1] += wt0 * wt0 + wt1 * wt1;
+      }
+    }
+    const double sum = wave_sum8(v);
+    if (l == 0 && p_last) part[(long)(nv - 1) * pstride + blk] = sum;
+    if (l == 8) part[(long)nv * pstride + blk] = sum;
+  };
+  if (nload == 0) {
+    specials();
+    return;
+  }
+  for (int k0 = 0; k0 < nload; k0 += 8) {
+    Chunk c;
+    load(k0, c);
+    if (k0 == 0) specials();
+    reduce(k0, c);
   }
 }
 
