@@ -419,10 +419,44 @@ __global__ void __launch_bounds__(64) k_shell(ShellDev m, Phys ph, double* __res
   const int cell = c0 + w;
   const bool active = w < CPW && cell < m.ncell;
   const long lim = (long)m.ncell * 8;
-  for (int q = lane; q < NF * CPW * 8; q += 64) {
-    const int f = q / (CPW * 8), r = q - f * (CPW * 8);
-    const long g = (long)c0 * 8 + r;
-    sdof[f][r] = g < lim ? ph.src[f][g] : 0.0;
+  // neighbour indices and orientation codes of the faces this lane evaluates (bottom, top and its one or
+  // two lateral faces), loaded with the staged fields so they share that memory latency
+  const int pr0 = t, pr1 = t + 9;  // the lane's (lateral face, edge point) pairs; pr1 only when < 12
+  int nbz[2] = {-1, -1}, nbl[2] = {-1, -1};
+  unsigned char cbz[2] = {0, 0}, cbl[2] = {0, 0};
+  if (Phys::FACES && active) {
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      nbz[k] = m.nb[(long)cell * 6 + 4 + k];
+      cbz[k] = m.nbc[(long)cell * 6 + 4 + k];
+    }
+    nbl[0] = m.nb[(long)cell * 6 + pr0 / 3];
+    cbl[0] = m.nbc[(long)cell * 6 + pr0 / 3];
+    if (pr1 < 12) {
+      nbl[1] = m.nb[(long)cell * 6 + pr1 / 3];
+      cbl[1] = m.nbc[(long)cell * 6 + pr1 / 3];
+    }
+  }
+  // staged fields: fixed trip counts, every load of a group of passes issued before its first LDS write
+  {
+    constexpr int NST = NF * CPW * 8, NSP = (NST + 63) / 64, G = 8;
+#pragma unroll
+    for (int g0 = 0; g0 < NSP; g0 += G) {
+      double sv[G];
+#pragma unroll
+      for (int k = 0; k < G; ++k) {
+        const int q = lane + 64 * (g0 + k);
+        const int f = q / (CPW * 8), r = q - f * (CPW * 8);
+        const long g = (long)c0 * 8 + r;
+        sv[k] = (g0 + k < NSP && q < NST && g < lim) ? ph.src[f < NF ? f : 0][g] : 0.0;
+      }
+#pragma unroll
+      for (int k = 0; k < G; ++k) {
+        const int q = lane + 64 * (g0 + k);
+        const int f = q / (CPW * 8), r = q - f * (CPW * 8);
+        if (g0 + k < NSP && q < NST) sdof[f][r] = sv[k];
+      }
+    }
   }
   double r0 = 0;
   if (active) {
@@ -468,6 +502,19 @@ __global__ void __launch_bounds__(64) k_shell(ShellDev m, Phys ph, double* __res
     }
     if (Phys::FACES) {
       // bottom (4) and top (5) faces of the column (unrolled: `own` names one register array)
+      // the neighbours' traces at this column's points of both faces, their loads issued together
+      double vmz[2][NF];
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const int nbr = nbz[k] < 0 ? cell : nbz[k];  // a wall reads its own cell (the value is not used)
+        const int g = cbz[k] & 7, a2 = (cbz[k] >> 3) ? 2 - a : a;
+        const double xi = kGQ[a2], eta = kGQ[b];
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+          const double* u = ph.src[f] + (long)nbr * 8 + (g == 5 ? 4 : 0);
+          vmz[k][f] = u[0] * (1 - xi) * (1 - eta) + u[1] * xi * (1 - eta) + u[2] * (1 - xi) * eta + u[3] * xi * eta;
+        }
+      }
 #pragma unroll
       for (int fz = 4; fz < 6; ++fz) {
         const double rr = fz == 4 ? r0 : r0 + dr, sg = fz == 4 ? -1.0 : 1.0;
@@ -475,20 +522,10 @@ __global__ void __launch_bounds__(64) k_shell(ShellDev m, Phys ph, double* __res
         const double Av[3] = {ws * G.C2[0], ws * G.C2[1], ws * G.C2[2]};
         const double* own = fz == 4 ? vb : vt;
         double Fv[NO];
-        const int nbr = m.nb[(long)cell * 6 + fz];
-        if (nbr < 0) {
+        if (nbz[fz - 4] < 0) {
           ph.wall(own, Av, Fv);
         } else {
-          const int code = m.nbc[(long)cell * 6 + fz];
-          const int g = code & 7, a2 = (code >> 3) ? 2 - a : a;
-          double vm[NF];
-#pragma unroll
-          for (int f = 0; f < NF; ++f) {
-            const double* u = ph.src[f] + (long)nbr * 8 + (g == 5 ? 4 : 0);
-            const double xi = kGQ[a2], eta = kGQ[b];
-            vm[f] = u[0] * (1 - xi) * (1 - eta) + u[1] * xi * (1 - eta) + u[2] * (1 - xi) * eta + u[3] * xi * eta;
-          }
-          ph.face(own, vm, Av, Fv);
+          ph.face(own, vmz[fz - 4], Av, Fv);
         }
 #pragma unroll
         for (int o = 0; o < NO; ++o)
@@ -498,7 +535,7 @@ __global__ void __launch_bounds__(64) k_shell(ShellDev m, Phys ph, double* __res
       }
       // lateral faces: pairs (f, e) of face f < 4 and edge point e
 #pragma unroll 1
-      for (int pr = t; pr < 12; pr += 9) {
+      for (int pr = pr0; pr < 12; pr += 9) {
         const int f = pr / 3, e = pr % 3;
         const double xi = f < 2 ? (double)(f & 1) : kGQ[e], eta = f < 2 ? kGQ[e] : (double)(f & 1);
         LatGeo H;
@@ -522,11 +559,11 @@ __global__ void __launch_bounds__(64) k_shell(ShellDev m, Phys ph, double* __res
         double ob[NF], ot[NF], nb_[NF], nt[NF];
 #pragma unroll
         for (int q = 0; q < NF; ++q) lat_vals(&sdof[q][w * 8], xi, eta, ob[q], ot[q]);
-        const int nbr = m.nb[(long)cell * 6 + f];
+        const int nbr = pr == pr0 ? nbl[0] : nbl[1];
         if (nbr >= 0) {
           // the neighbour's trace along its face g at edge point e2: its two bottom and
           // two top dofs on that face, linear in the coordinate along the edge
-          const int code = m.nbc[(long)cell * 6 + f];
+          const int code = pr == pr0 ? cbl[0] : cbl[1];
           const int g = code & 7, e2 = (code >> 3) ? 2 - e : e;
           const int i0 = g < 2 ? (g & 1) : 2 * (g & 1), step = g < 2 ? 2 : 1;
           const double y = kGQ[e2];
