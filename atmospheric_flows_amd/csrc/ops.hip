@@ -442,7 +442,7 @@ __device__ inline double wave_sum8(const double (&v)[8]) {
 // (GMRES applies the operator to its newest basis vector): that slot's values
 // come from the staged copy of p instead of a second read.
 template <int K, bool AMR>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(AMR ? 1 : 4))) void k_dmc_dots(MeshDev m, PhysDminusC ph, FieldPtrs<4> in, double* __restrict__ dst,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(AMR ? 1 : (K == 2 ? 3 : 4)))) void k_dmc_dots(MeshDev m, PhysDminusC ph, FieldPtrs<4> in, double* __restrict__ dst,
                                                  const double* __restrict__ scale, long Ns,
                                                  const double* __restrict__ V, long ldv, int nv, int p_last,
                                                  double* __restrict__ part, long pstride) {
