@@ -607,10 +607,107 @@ __global__ __launch_bounds__(64) void k_coarse_cg(MeshDev m, const double* __res
   if (threadIdx.x == 0) *its = status == 1 ? it : -1;
 }
 
+// The same CG with every vector of the level (x, g, d, h and the density)
+// held in LDS for the whole solve: an iteration makes no global memory
+// access. Same operations in the same order as k_coarse_cg (one wave; the
+// operator is applied one component at a time, with the same column
+// weights), so the results are those of k_coarse_cg.
+constexpr size_t COARSE_LDS_MAX = 60 * 1024;  // bytes of level vectors a coarse solve keeps in LDS
+
+template <int K, int NC, bool W>
+__global__ __launch_bounds__(64) void k_coarse_cg_lds(MeshDev m, const double* __restrict__ rho, double coeff,
+                                                      const double* __restrict__ b, double* __restrict__ x_out,
+                                                      const double* tolp, int max_its, int* its, long Ns) {
+  using C = Cfg<K>;
+  constexpr int ND2 = C::ND2, CPW = C::CPW;
+  extern __shared__ double lv[];
+  __shared__ double s_red[CPW * C::ND * C::NQ];
+  const long n = (long)NC * Ns;
+  double *x = lv, *g = lv + n, *d = lv + 2 * n, *h = lv + 3 * n, *r = lv + 4 * n;
+  const double tol = *tolp;
+  double gg = 0;
+  for (long i = threadIdx.x; i < n; i += 64) {
+    const double gi = -b[i];
+    x[i] = 0.0;
+    g[i] = gi;
+    d[i] = -gi;
+    gg += gi * gi;
+  }
+  if (W)
+    for (long i = threadIdx.x; i < Ns; i += 64) r[i] = rho[i];
+  gg = wsum(gg);
+  double res = sqrt(gg);
+  int it = 0;
+  int status = 0;
+  if (res <= tol) status = 1;
+  else if (max_its <= 0) status = -1;
+  double gh = gg;
+  wave_sync();
+  const int passes = (m.ncell + CPW - 1) / CPW;
+  double Si[C::NQ];
+  lane_cols_S<K, double>(threadIdx.x % C::ND, Si);
+  while (status == 0) {
+    ++it;
+    // h = A d, pass by pass and component by component
+    for (int p = 0; p < passes; ++p) {
+      const int cell0 = p * CPW;
+      const long base = (long)cell0 * ND2, lim = (long)m.ncell * ND2;
+#pragma unroll
+      for (int comp = 0; comp < NC; ++comp) {
+        wave_mass_core<K, double, 1, W>(m, cell0, d + comp * Ns + base, r + base, coeff, s_red);
+        wave_sync();
+        for (int t = threadIdx.x; t < CPW * ND2; t += 64) {
+          if (base + t >= lim) break;
+          if (64 % C::ND != 0) lane_cols_S<K, double>(t % C::ND, Si);
+          h[comp * Ns + base + t] = wave_mass_out<K, double>(s_red, t, Si);
+        }
+        if (64 % C::ND != 0) lane_cols_S<K, double>(threadIdx.x % C::ND, Si);
+        wave_sync();
+      }
+    }
+    double dh = 0;
+    for (long i = threadIdx.x; i < n; i += 64) dh += d[i] * h[i];
+    dh = wsum(dh);
+    const double alpha = gh / dh;
+    gg = 0;
+    for (long i = threadIdx.x; i < n; i += 64) {
+      x[i] += alpha * d[i];
+      const double gi = g[i] + alpha * h[i];
+      g[i] = gi;
+      gg += gi * gi;
+    }
+    gg = wsum(gg);
+    res = sqrt(gg);
+    if (res <= tol) { status = 1; break; }
+    if (it >= max_its || isnan(res)) { status = -1; break; }
+    const double beta = gg / gh;
+    gh = gg;
+    for (long i = threadIdx.x; i < n; i += 64) d[i] = -g[i] + beta * d[i];
+    wave_sync();
+  }
+  wave_sync();
+  for (long i = threadIdx.x; i < n; i += 64) x_out[i] = x[i];
+  if (threadIdx.x == 0) *its = status == 1 ? it : -1;
+}
+
 template <int K>
 void launch_coarse_cg(const MeshDev& m, int ncomp, bool weighted, const double* rho, double coeff, const double* b,
                       double* x, double* g, double* d, double* h, const double* tol, int max_its, int* its, long Ns,
                       hipStream_t s) {
+  const size_t lds = (size_t)(4 * ncomp + (weighted ? 1 : 0)) * Ns * sizeof(double);
+  if (lds <= COARSE_LDS_MAX) {
+    if (weighted)
+      hipLaunchKernelGGL((k_coarse_cg_lds<K, 2, true>), dim3(1), dim3(64), lds, s, m, rho, coeff, b, x, tol, max_its,
+                         its, Ns);
+    else if (ncomp == 1)
+      hipLaunchKernelGGL((k_coarse_cg_lds<K, 1, false>), dim3(1), dim3(64), lds, s, m, rho, coeff, b, x, tol,
+                         max_its, its, Ns);
+    else
+      hipLaunchKernelGGL((k_coarse_cg_lds<K, 2, false>), dim3(1), dim3(64), lds, s, m, rho, coeff, b, x, tol,
+                         max_its, its, Ns);
+    ATMO_HIP_CHECK(hipGetLastError());
+    return;
+  }
   if (weighted)
     hipLaunchKernelGGL((k_coarse_cg<K, 2, true>), dim3(1), dim3(64), 0, s, m, rho, coeff, b, x, g, d, h, tol, max_its,
                        its, Ns);
