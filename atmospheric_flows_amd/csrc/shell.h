@@ -99,6 +99,17 @@ void shell_rhs_velocity(const ShellDev& m, const ShellConsts& c, int stage, cons
 void shell_B(const ShellDev& m, const ShellConsts& c, double coeff, const double* p, double* dst_u, hipStream_t s);
 void shell_C(const ShellDev& m, const ShellConsts& c, double coeff, const double* pfix, const double* u,
              double* dst_p, hipStream_t s);
+// y = A^{-1} B p with A's packed cell inverse blocks (shell_block_inverse), one pass
+void shell_b_ainv(const ShellDev& m, const ShellConsts& c, double coeff, const double* p, const double* inv,
+                  double* dst_u, hipStream_t s);
+// dst = scale * (D p - C u) and, in the same pass, per-block partial dot products of dst with the columns
+// V_0 .. V_{nv-1} of V (leading dimension ldv) and with itself: slot k of block b at part[k * pstride + b],
+// slot nv is dst.dst (GMRES's Arnoldi step; when p is column nv-1 its staged values serve that slot)
+void shell_d_minus_c_dots(const ShellDev& m, const ShellConsts& c, double coeff, const double* pfix, const double* u,
+                          const double* p, const double* scale, double* dst, const double* V, long ldv, int nv,
+                          double* part, long pstride, hipStream_t s);
+// number of blocks (partials per slot) of shell_d_minus_c_dots
+inline int shell_blocks(int ncell) { return (ncell + 6) / 7; }
 // dst = scale * (D p - C u) (scale may be null)
 void shell_d_minus_c(const ShellDev& m, const ShellConsts& c, double coeff, const double* pfix, const double* u,
                      const double* p, const double* scale, double* dst, hipStream_t s);

@@ -81,6 +81,7 @@ class ShellEngine : public Engine {
     gm_coef.alloc(MAX_GMRES + 2);
     gm_rec.alloc((size_t)MAX_GMRES * GM_REC);
     gm_sig.alloc(MAX_GMRES + 2);
+    gm_part.alloc((size_t)RED_MAX_SLOTS * shell_blocks(m_.ncell));
     for (auto* b : {&rho_old, &rho2, &rho3, &rho_curr, &p_old, &p2, &p3, &p_fixed, &p_fixed_old, &rhs_p}) b->alloc(Ns);
     for (auto* b : {&u_old, &u2, &u3, &u_curr, &u_fixed, &tmp1, &rhs_u, &rhs_u0}) b->alloc(3 * Ns);
     for (auto* b : {&rho_old, &rho2, &rho3, &rho_curr, &p_old, &p2, &p3, &p_fixed, &p_fixed_old, &rhs_p, &u_old, &u2,
@@ -298,6 +299,21 @@ class ShellEngine : public Engine {
           shell_d_minus_c(m_, oc_, bc_coeff(), pres_fix.p, u_old.p, p_old.p, ddiag.p, gm_p.p, s_);
           fields = 7;
           break;
+        case 8:  // B with the cellwise A^{-1} in one pass: p in, y (3 components) out, the packed blocks
+          shell_b_ainv(m_, oc_, bc_coeff(), p_old.p, ainv.p, s_t2.p, s_);
+          fields = 4 + 36.0 / 8.0;
+          break;
+        case 12: {  // the Arnoldi step's scaled D - C with the dot products against 14 basis vectors
+          if (P.gmres_restart < 13) throw std::invalid_argument("time_kernel 12 needs gmres_restart >= 13");
+          // as GMRES runs it: applied to the newest basis vector V_13, whose staged values serve its own
+          // dot product (p, y (3 components), p_fix, the Jacobi scale and 13 other basis vectors in; w out)
+          const long ld = gm_ld();
+          const int G = shell_blocks(m_.ncell);
+          shell_d_minus_c_dots(m_, oc_, bc_coeff(), pres_fix.p, u_old.p, gm_V.p + 13 * ld, ddiag.p,
+                               gm_V.p + 14 * ld, gm_V.p, ld, 14, gm_part.p, G, s_);
+          fields = 7 + 13;
+          break;
+        }
         case 14:
           shell_rhs_velocity(m_, oc_, 3, s11, s_t1.p, s_);
           fields = 18;
@@ -347,6 +363,7 @@ class ShellEngine : public Engine {
   DBuf<double> minv, mdiag, ainv, adiag, ddiag, rho_fix, pres_fix;
   DBuf<double> s_t1, s_t2, s_t3;
   DBuf<double> gm_V, gm_p, gm_coef, gm_rec, gm_sig;
+  DBuf<double> gm_part;  // per-block partial dot products of the fused Arnoldi step, [slot][block]
   DBuf<double> rho_old, rho2, rho3, rho_curr, u_old, u2, u3, u_curr, u_fixed, p_old, p2, p3, p_fixed, p_fixed_old,
       tmp1, rhs_u, rhs_p, rhs_u0;
 
@@ -363,8 +380,7 @@ class ShellEngine : public Engine {
   // S p = D p - C A^{-1} B p (apply_add NS 2, euler_operator.h:2374-2408), times scale when given
   void schur_vmult(const double* src, double* dst, const double* scale = nullptr) {
     ++st_.schur_matvecs;
-    shell_B(m_, oc_, bc_coeff(), src, s_t1.p, s_);
-    shell_block_apply(m_, 3, ainv.p, 1.0, s_t1.p, s_t2.p, s_);
+    shell_b_ainv(m_, oc_, bc_coeff(), src, ainv.p, s_t2.p, s_);
     shell_d_minus_c(m_, oc_, bc_coeff(), pres_fix.p, s_t2.p, src, scale, dst, s_);
   }
 
@@ -383,9 +399,14 @@ class ShellEngine : public Engine {
     double* V = gm_V.p;
     double* W = V + (long)(j + 1) * ld;
     double* rec = gm_rec.p + (long)j * GM_REC;
-    schur_vmult(V + (long)j * ld, W, ddiag.p);
-    gm_mdot(Ns, W, V, ld, j + 1, rec, rb, s_);
-    vec_dot(Ns, W, W, rec + j + 1, false, rb, s_);
+    // W = P S V_j (the Schur product: A^{-1} B in one pass, then D - C) with the dot products W.V_i (i <= j)
+    // and W.W formed in the D - C pass
+    const double* Vj = V + (long)j * ld;
+    ++st_.schur_matvecs;
+    shell_b_ainv(m_, oc_, bc_coeff(), Vj, ainv.p, s_t2.p, s_);
+    const int G = shell_blocks(m_.ncell);
+    shell_d_minus_c_dots(m_, oc_, bc_coeff(), pres_fix.p, s_t2.p, Vj, ddiag.p, W, V, ld, j + 1, gm_part.p, G, s_);
+    gm_sum_slots(gm_part.p, G, G, j + 2, rec, s_);
     gm_maxpy_norm(Ns, W, V, ld, j + 1, rec, gm_sig.p, rec + GM_REC - 1, gm_sig.p + j + 1, rb, s_);
     ATMO_HIP_CHECK(hipMemcpyAsync(hgm + (long)j * GM_REC, rec, sizeof(double) * GM_REC, hipMemcpyDeviceToHost, s_));
     ATMO_HIP_CHECK(hipEventRecord(gm_ev[j], s_));
@@ -599,6 +620,7 @@ size_t shell_device_bytes(const ShellDesc& d, const EngineParams& p) {
   b += (nc * 36 * 2 + nc * 8 * 2 + Ns * 3) * 8;                         // blocks, diagonals, rho_fix, pres_fix
   b += (Ns * 3 * 2 + Ns) * 8;                                           // Schur temporaries
   b += ((size_t)(p.gmres_restart + 2) * (Ns + 32) + Ns + 2 * (MAX_GMRES + 2) + (size_t)MAX_GMRES * GM_REC) * 8;
+  b += (size_t)RED_MAX_SLOTS * shell_blocks((int)nc) * 8;                // Arnoldi partial dot products
   b += (Ns * 10 + Ns * 3 * 8) * 8;                                      // IMEX state and work vectors
   return b;
 }

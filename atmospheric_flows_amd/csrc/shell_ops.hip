@@ -15,6 +15,7 @@
 
 #include "ops.h"
 #include "shell.h"
+#include "wave.h"
 
 namespace atmo {
 
@@ -394,6 +395,9 @@ __device__ inline void lat_vals(const double* u, double xi, double eta, double& 
   vt = u[4] * w0 + u[5] * w1 + u[6] * w2 + u[7] * w3;
 }
 
+// Packed upper triangle of a symmetric 8 x 8 block: entry (a, b), a <= b.
+__host__ __device__ constexpr int sym8(int a, int b) { return a <= b ? a * (15 - a) / 2 + b : b * (15 - b) / 2 + a; }
+
 // Cells per wave: 7 cells x 9 lanes (one per lateral quadrature column (a, b)).
 constexpr int SCPW = 7;
 
@@ -406,13 +410,31 @@ constexpr int SCPW = 7;
 // homogeneous of degree 1 in the normal. Each lane accumulates its points'
 // contributions to all 8 NOUT (test function, component) integrals in
 // registers; the nine lanes of a cell are then summed through LDS.
-template <class Phys>
+// What a k_shell launch does with its result (EPI): 0 stores it (times
+// scale when given); 1 stores it through the cells' inverse blocks (packed,
+// 36 per cell: B p -> A^{-1} B p in one pass); 2 stores it times scale and,
+// in the same pass, forms the per-block partial dot products of the result
+// with the basis vectors V_0 .. V_{nv-1} and with itself (GMRES's Arnoldi
+// step; NOUT = 1).
+struct ShellEpi {
+  const double* inv = nullptr;
+  const double* V = nullptr;
+  long ldv = 0;
+  int nv = 0, p_last = 0;
+  double* part = nullptr;
+  long pstride = 0;
+};
+
+template <class Phys, int EPI>
 __global__ void __launch_bounds__(64) k_shell(ShellDev m, Phys ph, double* __restrict__ dst, long Ns,
-                                              const double* __restrict__ scale) {
+                                              const double* __restrict__ scale, ShellEpi ep) {
   constexpr int NF = Phys::NF, NO = Phys::NOUT, CPW = SCPW, NI = 8 * NO;
+  static_assert(EPI != 2 || NO == 1, "Arnoldi epilogue: one output field");
   __shared__ double sdof[NF][CPW * 8];
   __shared__ double sS[CPW][9][3];
   __shared__ double sred[CPW][9][NI];
+  __shared__ double sinv[EPI == 1 ? CPW * 36 : 1];   // EPI 1: the wave's inverse blocks
+  __shared__ double sbp[EPI == 1 ? CPW : 1][NI];     // EPI 1: the cells' results before the inverse
   const int lane = threadIdx.x;
   const int w = lane / 9, t = lane - 9 * w;
   const int c0 = cell_block() * CPW;
@@ -437,6 +459,16 @@ __global__ void __launch_bounds__(64) k_shell(ShellDev m, Phys ph, double* __res
       cbl[1] = m.nbc[(long)cell * 6 + pr1 / 3];
     }
   }
+  // EPI 1: the wave's inverse blocks (independent of the result), loaded with the staged fields
+  constexpr int NIV = (CPW * 36 + 63) / 64;
+  double ivr[EPI == 1 ? NIV : 1];
+  if constexpr (EPI == 1) {
+#pragma unroll
+    for (int k = 0; k < NIV; ++k) {
+      const int q = lane + 64 * k;
+      ivr[k] = (q < CPW * 36 && c0 + q / 36 < m.ncell) ? ep.inv[(long)c0 * 36 + q] : 0.0;
+    }
+  }
   // staged fields: fixed trip counts, every load of a group of passes issued before its first LDS write
   {
     constexpr int NST = NF * CPW * 8, NSP = (NST + 63) / 64, G = 8;
@@ -456,6 +488,13 @@ __global__ void __launch_bounds__(64) k_shell(ShellDev m, Phys ph, double* __res
         const int f = q / (CPW * 8), r = q - f * (CPW * 8);
         if (g0 + k < NSP && q < NST) sdof[f][r] = sv[k];
       }
+    }
+  }
+  if constexpr (EPI == 1) {
+#pragma unroll
+    for (int k = 0; k < NIV; ++k) {
+      const int q = lane + 64 * k;
+      if (q < CPW * 36) sinv[q] = ivr[k];
     }
   }
   double r0 = 0;
@@ -602,27 +641,88 @@ __global__ void __launch_bounds__(64) k_shell(ShellDev m, Phys ph, double* __res
     for (int k = 0; k < NI; ++k) sred[w][t][k] = acc[k];
   }
   __syncthreads();
-  if (active) {
+  if constexpr (EPI == 0) {
+    if (active) {
 #pragma unroll 1
-    for (int it = t; it < NI; it += 9) {
+      for (int it = t; it < NI; it += 9) {
+        double sum = 0.0;
+#pragma unroll
+        for (int l = 0; l < 9; ++l) sum += sred[w][l][it];
+        const long k = (long)(it >> 3) * Ns + (long)cell * 8 + (it & 7);
+        dst[k] = scale ? sum * scale[k] : sum;
+      }
+    }
+  } else if constexpr (EPI == 1) {
+    // dof a of output o: row a of the cell's inverse block times the cell's result of that output (the
+    // expression of k_shell_block_apply with coeff 1)
+    if (active) {
+#pragma unroll 1
+      for (int it = t; it < NI; it += 9) {
+        double sum = 0.0;
+#pragma unroll
+        for (int l = 0; l < 9; ++l) sum += sred[w][l][it];
+        sbp[w][it] = sum;
+      }
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll 1
+      for (int it = t; it < NI; it += 9) {
+        const int o = it >> 3, a = it & 7;
+        double acc = 0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc += sinv[w * 36 + sym8(a, j)] * sbp[w][8 * o + j];
+        dst[(long)o * Ns + (long)cell * 8 + a] = 1.0 * acc;
+      }
+    }
+  } else {
+    // lane t < 8 of a cell holds dof t of the result; the dot products with the basis go in chunks of 8
+    // slots (slot k: column k of V) whose loads are all in flight at once, then w.p (with p_last: p is
+    // the staged field 0, column nv-1 of V) and w.w
+    const bool own = active && t < 8;
+    const long k = (long)cell * 8 + t;
+    double wv = 0.0;
+    if (own) {
       double sum = 0.0;
 #pragma unroll
-      for (int l = 0; l < 9; ++l) sum += sred[w][l][it];
-      const long k = (long)(it >> 3) * Ns + (long)cell * 8 + (it & 7);
-      dst[k] = scale ? sum * scale[k] : sum;
+      for (int l = 0; l < 9; ++l) sum += sred[w][l][t];
+      wv = scale ? sum * scale[k] : sum;
+      dst[k] = wv;
     }
+    const long gk = own ? k : 0;
+    const int nload = ep.p_last ? ep.nv - 1 : ep.nv;
+    const int blk = cell_block();
+    const int l = lane & 63;
+    for (int k0 = 0; k0 < nload; k0 += 8) {
+      double v[8];
+#pragma unroll
+      for (int s8 = 0; s8 < 8; ++s8) {
+        const bool ld = k0 + s8 < nload;  // uniform
+        const double y = ld ? __builtin_nontemporal_load(ep.V + (long)(k0 + s8) * ep.ldv + gk) : 0.0;
+        v[s8] = wv * y;
+      }
+      const double sum = wave_sum8(v);
+      const int slot = k0 + (l >> 3);
+      if ((l & 7) == 0 && slot < nload) ep.part[(long)slot * ep.pstride + blk] = sum;
+    }
+    double v[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (ep.p_last && own) v[0] = wv * sdof[0][w * 8 + t];
+    v[1] = wv * wv;
+    const double sum = wave_sum8(v);
+    if (l == 0 && ep.p_last) ep.part[(long)(ep.nv - 1) * ep.pstride + blk] = sum;
+    if (l == 8) ep.part[(long)ep.nv * ep.pstride + blk] = sum;
   }
 }
 
-template <class Phys>
-void launch(const ShellDev& m, const Phys& ph, double* dst, const double* scale, hipStream_t s) {
+template <class Phys, int EPI = 0>
+void launch(const ShellDev& m, const Phys& ph, double* dst, const double* scale, hipStream_t s,
+            const ShellEpi& ep = ShellEpi()) {
   const int blocks = (m.ncell + SCPW - 1) / SCPW;
-  hipLaunchKernelGGL(k_shell<Phys>, dim3(blocks), dim3(64), 0, s, m, ph, dst, (long)m.ncell * 8, scale);
+  hipLaunchKernelGGL((k_shell<Phys, EPI>), dim3(blocks), dim3(64), 0, s, m, ph, dst, (long)m.ncell * 8, scale, ep);
+  ATMO_HIP_CHECK(hipGetLastError());
 }
 
 // ------------------------------------------------------------ cell blocks
-// Packed upper triangle of a symmetric 8 x 8 block: entry (a, b), a <= b.
-__host__ __device__ constexpr int sym8(int a, int b) { return a <= b ? a * (15 - a) / 2 + b : b * (15 - b) / 2 + a; }
 
 // Inverse of each cell's mass block (weighted by rho when given), by
 // Cholesky in one lane per cell; the inverse diagonal of the block itself too.
@@ -894,6 +994,39 @@ void shell_d_minus_c(const ShellDev& m, const ShellConsts& c, double coeff, cons
   ph.dt = c.dt;
   ph.gm = c.gamma_gas;
   launch(m, ph, dst, scale, s);
+}
+
+void shell_b_ainv(const ShellDev& m, const ShellConsts& c, double coeff, const double* p, const double* inv,
+                  double* dst_u, hipStream_t s) {
+  PhysB ph{};
+  ph.src[0] = p;
+  ph.cf = coeff * c.dt / (c.Ma * c.Ma);
+  ShellEpi ep;
+  ep.inv = inv;
+  launch<PhysB, 1>(m, ph, dst_u, nullptr, s, ep);
+}
+
+void shell_d_minus_c_dots(const ShellDev& m, const ShellConsts& c, double coeff, const double* pfix, const double* u,
+                          const double* p, const double* scale, double* dst, const double* V, long ldv, int nv,
+                          double* part, long pstride, hipStream_t s) {
+  const long Ns = (long)m.ncell * 8;
+  const int blocks = (m.ncell + SCPW - 1) / SCPW;
+  if (blocks > pstride) throw std::invalid_argument("shell_d_minus_c_dots: partial buffer too small");
+  PhysC<true> ph{};
+  ph.src[0] = p;
+  ph.src[1] = pfix;
+  velocity_fields(u, Ns, &ph.src[2]);
+  ph.coeff = coeff;
+  ph.dt = c.dt;
+  ph.gm = c.gamma_gas;
+  ShellEpi ep;
+  ep.V = V;
+  ep.ldv = ldv;
+  ep.nv = nv;
+  ep.p_last = nv > 0 && p == V + (long)(nv - 1) * ldv;
+  ep.part = part;
+  ep.pstride = pstride;
+  launch<PhysC<true>, 2>(m, ph, dst, scale, s, ep);
 }
 
 void shell_mass(const ShellDev& m, int ncomp, const double* rho, double coeff, const double* src, double* dst,

@@ -149,4 +149,29 @@ __device__ inline T wave_mass_out(const T* s_red_comp, int t, const T (&Si)[Cfg<
   return acc;
 }
 
+// Eight per-lane values summed over the wave by a transposing butterfly
+// (10 shuffles for the eight sums, fixed order): every lane returns the sum
+// of slot (lane >> 3) & 7.
+__device__ inline double wave_sum8(const double (&v)[8]) {
+  const int l = threadIdx.x & 63;
+  const bool b5 = l & 32, b4 = l & 16, b3 = l & 8;
+  double u[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const double keep = b5 ? v[4 + i] : v[i], give = b5 ? v[i] : v[4 + i];
+    u[i] = keep + __shfl_xor(give, 32, 64);
+  }
+  double w[2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const double keep = b4 ? u[2 + i] : u[i], give = b4 ? u[i] : u[2 + i];
+    w[i] = keep + __shfl_xor(give, 16, 64);
+  }
+  double x = (b3 ? w[1] : w[0]) + __shfl_xor(b3 ? w[0] : w[1], 8, 64);
+  x += __shfl_xor(x, 4, 64);
+  x += __shfl_xor(x, 2, 64);
+  x += __shfl_xor(x, 1, 64);
+  return x;
+}
+
 }  // namespace atmo
