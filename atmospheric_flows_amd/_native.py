@@ -32,7 +32,13 @@ class Params(ctypes.Structure):
                 ("a_solve_tol", ctypes.c_double), ("fp_max_iter", ctypes.c_int), ("fp_tol", ctypes.c_double),
                 ("fp_lock", ctypes.c_int), ("mg_min_cells", ctypes.c_int), ("cheb_degree", ctypes.c_int),
                 ("cheb_range", ctypes.c_double), ("cheb_eig_its", ctypes.c_int), ("gmres_restart", ctypes.c_int),
-                ("precision", ctypes.c_int), ("mg_matrix_cores", ctypes.c_int)]
+                ("precision", ctypes.c_int), ("mg_matrix_cores", ctypes.c_int),
+                ("arnoldi_waves_per_simd", ctypes.c_int)]
+
+
+class KernelTuning(ctypes.Structure):
+    _fields_ = [("arnoldi_us", ctypes.c_double * 2), ("sweep_us", ctypes.c_double * 2),
+                ("arnoldi_waves_per_simd", ctypes.c_int), ("mg_matrix_cores", ctypes.c_int)]
 
 
 class ShellDesc(ctypes.Structure):
@@ -120,6 +126,9 @@ SIGNATURES = [
     ("atmo_write_vtu", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]),
     ("atmo_write_vtu_host", ctypes.c_int, [ctypes.POINTER(MeshDesc), _dp, _dp, _dp, ctypes.c_char_p, ctypes.c_int]),
     ("atmo_time_kernel", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp]),
+    ("atmo_tune_kernels", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(KernelTuning)]),
+    ("atmo_ensemble_tune_kernels", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(KernelTuning),
+                                                  ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     ("atmo_device_bytes", ctypes.c_int, [ctypes.POINTER(MeshDesc), ctypes.POINTER(Params),
                                          ctypes.POINTER(ctypes.c_size_t)]),
     ("atmo_ctx_device_bytes", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)]),

@@ -96,6 +96,7 @@ static EngineParams to_engine(const atmo_params& p) {
   e.gmres_restart = p.gmres_restart;
   e.precision = p.precision;
   e.mg_matrix_cores = p.mg_matrix_cores;
+  e.arnoldi_waves = p.arnoldi_waves_per_simd;
   return e;
 }
 
@@ -111,6 +112,8 @@ static void check_params(const atmo_params& p) {
   if (p.precision < 0 || p.precision > 2) throw std::invalid_argument("params: precision must be 0, 1 or 2");
   if (p.mg_matrix_cores != 0 && p.mg_matrix_cores != 1)
     throw std::invalid_argument("params: mg_matrix_cores must be 0 or 1");
+  if (p.arnoldi_waves_per_simd != 0 && p.arnoldi_waves_per_simd != 3 && p.arnoldi_waves_per_simd != 4)
+    throw std::invalid_argument("params: arnoldi_waves_per_simd must be 0, 3 or 4");
   if (!(p.eps >= 0) || !(p.schur_inner_tol >= 0) || !(p.a_solve_tol >= 0))
     throw std::invalid_argument("params: tolerances must be >= 0");
   if (p.mg_min_cells < 1) throw std::invalid_argument("params: mg_min_cells must be >= 1");
@@ -154,6 +157,7 @@ void atmo_params_default(atmo_params* p) {
   p->gmres_restart = e.gmres_restart;
   p->precision = e.precision;
   p->mg_matrix_cores = e.mg_matrix_cores;
+  p->arnoldi_waves_per_simd = e.arnoldi_waves;
 }
 
 const char* atmo_version(void) { return "atmo_hip 0.1 (gfx950)"; }
@@ -640,6 +644,23 @@ atmo_status atmo_time_kernel(atmo_ctx* ctx, int which, int nrep, double* ms_per_
   });
 }
 
+static void to_tuning(const KernelTuning& kt, atmo_kernel_tuning* o) {
+  for (int i = 0; i < 2; ++i) {
+    o->arnoldi_us[i] = kt.arnoldi_us[i];
+    o->sweep_us[i] = kt.sweep_us[i];
+  }
+  o->arnoldi_waves_per_simd = kt.arnoldi_waves;
+  o->mg_matrix_cores = kt.mg_matrix_cores;
+}
+
+atmo_status atmo_tune_kernels(atmo_ctx* ctx, int nrep, atmo_kernel_tuning* out) {
+  return guarded(ctx, [&]() {
+    if (!ctx || !out || nrep < 1) throw std::invalid_argument("bad argument");
+    to_tuning(ctx->eng->tune_kernels(nrep), out);
+    return ATMO_OK;
+  });
+}
+
 
 // ------------------------------------------------------------ spherical shell
 static ShellDesc to_shell(const atmo_shell_desc* d) {
@@ -793,6 +814,16 @@ atmo_status atmo_ensemble_autotune(atmo_ensemble* e, const int* candidates, int 
     *best_slots = best;
     if (rates)
       for (int i = 0; i < ncand; ++i) rates[i] = r[i];
+    return ATMO_OK;
+  });
+}
+
+atmo_status atmo_ensemble_tune_kernels(atmo_ensemble* e, int nrep, atmo_kernel_tuning* out, int cap, int* nclasses) {
+  return guarded(nullptr, [&]() {
+    if (!e || !nclasses || (cap > 0 && !out)) throw std::invalid_argument("null argument");
+    const auto cls = e->ens->tune_kernels(nrep);
+    *nclasses = (int)cls.size();
+    for (int c = 0; c < (int)cls.size() && c < cap; ++c) to_tuning(cls[c].kt, out + c);
     return ATMO_OK;
   });
 }

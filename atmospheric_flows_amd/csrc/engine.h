@@ -33,8 +33,20 @@ struct EngineParams {
   // 0: FP64 with the reference's solvers; 1: mixed (FP32 MG levels, cellwise-direct FP64 A^{-1});
   // 2: the mixed form's FP64 comparator (FP64 MG levels, the same cellwise-direct A^{-1})
   int precision = 0;
-  // FP32 multigrid sweeps at k = 2, 3 on the matrix cores (MFMA) instead of the vector ALUs
+  // FP32 multigrid sweeps at k = 2, 3 on the matrix cores (MFMA, 16 cells per wave) instead of the vector
+  // ALUs (9 or 12 cells per wave)
   int mg_matrix_cores = 1;
+  // waves per SIMD the Arnoldi kernel is compiled for (3 or 4; 0: the default for the degree)
+  int arnoldi_waves = 0;
+};
+
+// Kernel shapes the tuner times on a context's own mesh (Engine::tune_kernels):
+// microseconds per launch of each candidate and the choice made.
+struct KernelTuning {
+  double arnoldi_us[2] = {0, 0};  // the Arnoldi step (14 basis vectors) at 3 and 4 waves per SIMD
+  double sweep_us[2] = {0, 0};    // the FP32 A-block sweep on the vector ALUs and on the matrix cores
+  int arnoldi_waves = 0;          // chosen (0: not tuned, the form does not run the kernel)
+  int mg_matrix_cores = -1;       // chosen (-1: not tuned, no FP32 multigrid at k = 2, 3)
 };
 
 // One cell of a locally refined mesh: base cell (bi, bj), quadtree level l,
@@ -115,6 +127,9 @@ class Engine {
 
   // ---- measurement: mean duration of one kernel on the engine's data
   virtual void time_kernel(int which, int nrep, double* ms_per_launch, double* bytes_per_launch) = 0;
+  // ---- autotuner: times the tunable kernel shapes on this engine's mesh and sets the fastest (the
+  // results do not depend on the choice); the terrain engines of the cellwise-direct forms
+  virtual KernelTuning tune_kernels(int nrep) { return KernelTuning(); }
 };
 
 // Device bytes an engine of this mesh and parameters requests (equals

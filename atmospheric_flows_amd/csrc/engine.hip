@@ -317,6 +317,36 @@ class EngineK : public Engine {
   }
   double* state_ptr(int which) override { return which == 0 ? rho_old.p : (which == 1 ? u_old.p : p_old.p); }
 
+  // The tunable kernel shapes timed on this mesh (two alternating rounds, the faster round of each
+  // candidate counts): the Arnoldi kernel's waves per SIMD (structured meshes of the cellwise-direct forms)
+  // and the FP32 sweeps' form (matrix cores, 16 cells per wave, or vector ALUs, 64 / (2k+1) cells per wave).
+  KernelTuning tune_kernels(int nrep) override {
+    KernelTuning kt;
+    double ms = 0, bytes = 0;
+    if (direct_ainv() && !fine().m.cface && P.gmres_restart >= 13) {
+      kt.arnoldi_us[0] = kt.arnoldi_us[1] = 1e300;
+      for (int round = 0; round < 2; ++round)
+        for (int i = 0; i < 2; ++i) {
+          P.arnoldi_waves = 3 + i;
+          time_kernel(12, nrep, &ms, &bytes);
+          kt.arnoldi_us[i] = std::min(kt.arnoldi_us[i], 1e3 * ms);
+        }
+      kt.arnoldi_waves = kt.arnoldi_us[0] < kt.arnoldi_us[1] ? 3 : 4;
+      P.arnoldi_waves = kt.arnoldi_waves;
+    }
+    if (mg32() && (K == 2 || K == 3)) {
+      kt.sweep_us[0] = kt.sweep_us[1] = 1e300;
+      for (int round = 0; round < 2; ++round)
+        for (int i = 0; i < 2; ++i) {
+          time_kernel(i ? 14 : 15, nrep, &ms, &bytes);
+          kt.sweep_us[i] = std::min(kt.sweep_us[i], 1e3 * ms);
+        }
+      kt.mg_matrix_cores = kt.sweep_us[1] <= kt.sweep_us[0] ? 1 : 0;
+      P.mg_matrix_cores = kt.mg_matrix_cores;
+    }
+    return kt;
+  }
+
   void time_kernel(int which, int nrep, double* ms, double* bytes) override {
     const long Ns = fine().Ns;
     const MeshDev& m = fine().m;
@@ -421,7 +451,7 @@ class EngineK : public Engine {
           const int G = cell_blocks<K>(fine().ncell);
           const double* p = which == 12 ? gm_V.p + 13 * gm_ld() : p_old.p;
           launch_d_minus_c_dots<K>(m, oc_, bc_coeff(), pres_fix.p, u_old.p, p, fine().inv64[BLK_D].p,
-                                   gm_V.p + 14 * gm_ld(), Ns, gm_V.p, gm_ld(), nv, gm_part.p, G, s_);
+                                   gm_V.p + 14 * gm_ld(), Ns, gm_V.p, gm_ld(), nv, gm_part.p, G, P.arnoldi_waves, s_);
           fields = nv ? 6 + nv - 1 : 6;
           break;
         }
@@ -1054,7 +1084,7 @@ class EngineK : public Engine {
     schur_velocity(V + (long)j * ld);
     const int G = cell_blocks<K>(fine().ncell);
     launch_d_minus_c_dots<K>(fine().m, oc_, bc_coeff(), pres_fix.p, s_t2.p, V + (long)j * ld, PD, W, Ns, V, ld, j + 1,
-                             gm_part.p, G, s_);
+                             gm_part.p, G, P.arnoldi_waves, s_);
     gm_sum_slots(gm_part.p, G, G, j + 2, rec, s_);
     gm_maxpy_norm(Ns, W, V, ld, j + 1, rec, gm_sig.p, rec + GM_REC - 1, gm_sig.p + j + 1, rb, s_);
     ATMO_HIP_CHECK(hipMemcpyAsync(hgm + (long)j * GM_REC, rec, sizeof(double) * GM_REC, hipMemcpyDeviceToHost, s_));

@@ -198,4 +198,30 @@ int Ensemble::autotune(const std::vector<int>& candidates, int probe_steps, int 
   return best;
 }
 
+std::vector<Ensemble::ClassTuning> Ensemble::tune_kernels(int nrep) {
+  if (nrep < 1) throw std::invalid_argument("tune_kernels: nrep must be >= 1");
+  std::vector<ClassTuning> out;
+  std::vector<int> cls(specs_.size(), -1);
+  for (size_t i = 0; i < specs_.size(); ++i) {
+    const MemberSpec& m = specs_[i];
+    for (size_t c = 0; c < out.size(); ++c)
+      if (out[c].degree == m.degree && out[c].nx == m.mesh.nx && out[c].nz == m.mesh.nz &&
+          out[c].precision == m.params.precision)
+        cls[i] = (int)c;
+    if (cls[i] < 0) {
+      cls[i] = (int)out.size();
+      out.push_back(ClassTuning{m.degree, m.mesh.nx, m.mesh.nz, m.params.precision, 0, KernelTuning()});
+      auto eng = Engine::create(m.degree, m.mesh, m.params);
+      out.back().kt = eng->tune_kernels(nrep);
+    }
+    ++out[cls[i]].members;
+  }
+  for (size_t i = 0; i < specs_.size(); ++i) {
+    const KernelTuning& kt = out[cls[i]].kt;
+    if (kt.arnoldi_waves) specs_[i].params.arnoldi_waves = kt.arnoldi_waves;
+    if (kt.mg_matrix_cores >= 0) specs_[i].params.mg_matrix_cores = kt.mg_matrix_cores;
+  }
+  return out;
+}
+
 }  // namespace atmo

@@ -55,6 +55,15 @@ class Ensemble {
   // candidate as this ensemble's slot count and returns it; rates[i] is
   // candidate i's rate (0 when a member of the probe failed).
   int autotune(const std::vector<int>& candidates, int probe_steps, int sample, std::vector<double>& rates);
+  // Kernel-shape autotuner: for every class of submitted members with the same degree, mesh size and
+  // precision, creates one engine of the class (the first member's mesh and parameters), times the
+  // tunable kernel shapes on it (Engine::tune_kernels) and sets the choices in the parameters of every
+  // member of the class. Returns one record per class, in order of first appearance.
+  struct ClassTuning {
+    int degree, nx, nz, precision, members;
+    KernelTuning kt;
+  };
+  std::vector<ClassTuning> tune_kernels(int nrep);
 
  private:
   void worker();

@@ -44,11 +44,12 @@ void launch_d_minus_c(const MeshDev& m, const OpConsts& oc, double coeff, const 
 // The same product (scale required) and, in the same pass, per-block partial
 // dot products of dst with V_0 .. V_{nv-1} (column k at V + k*ldv) and
 // dst.dst: slot k of block b at part[k*pstride + b] (slot nv: dst.dst);
-// blocks = ceil(ncell / CPW) <= pstride.
+// blocks = ceil(ncell / CPW) <= pstride. waves: the waves per SIMD the
+// kernel was compiled for (3 or 4, 0: the default for k), results identical.
 template <int K>
 void launch_d_minus_c_dots(const MeshDev& m, const OpConsts& oc, double coeff, const double* pfix, const double* u,
                            const double* p, const double* scale, double* dst, long Ns, const double* V, long ldv,
-                           int nv, double* part, long pstride, hipStream_t s);
+                           int nv, double* part, long pstride, int waves, hipStream_t s);
 // number of blocks of the cell kernels (the partial count of launch_d_minus_c_dots)
 template <int K>
 inline int cell_blocks(int ncell) {

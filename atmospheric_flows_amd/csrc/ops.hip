@@ -416,8 +416,10 @@ void launch_d_minus_c(const MeshDev& m, const OpConsts& oc, double coeff, const 
 // products without a second pass over w. With p_last, p is column nv-1 of V
 // (GMRES applies the operator to its newest basis vector): that slot's values
 // come from the staged copy of p instead of a second read.
-template <int K, bool AMR>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(AMR ? 1 : (K == 2 ? 3 : 4)))) void k_dmc_dots(MeshDev m, PhysDminusC ph, FieldPtrs<4> in, double* __restrict__ dst,
+// WPE: the waves per SIMD the kernel is compiled for (its register budget), a tuning choice with identical
+// results (arnoldi_waves_per_simd; 0 picks the measured default: 3 at k = 2, where 4 spills, else 4).
+template <int K, bool AMR, int WPE>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(AMR ? 1 : WPE))) void k_dmc_dots(MeshDev m, PhysDminusC ph, FieldPtrs<4> in, double* __restrict__ dst,
                                                  const double* __restrict__ scale, long Ns,
                                                  const double* __restrict__ V, long ldv, int nv, int p_last,
                                                  double* __restrict__ part, long pstride) {
@@ -563,7 +565,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(AMR ? 1 : (K
 template <int K>
 void launch_d_minus_c_dots(const MeshDev& m, const OpConsts& oc, double coeff, const double* pfix, const double* u,
                            const double* p, const double* scale, double* dst, long Ns, const double* V, long ldv,
-                           int nv, double* part, long pstride, hipStream_t s) {
+                           int nv, double* part, long pstride, int waves, hipStream_t s) {
   // GMRES applies the operator to its newest basis vector: then slot nv-1 is p itself
   const int p_last = nv > 0 && p == V + (long)(nv - 1) * ldv;
   const int grid = ceil_div(m.ncell, Cfg<K>::CPW);
@@ -573,12 +575,17 @@ void launch_d_minus_c_dots(const MeshDev& m, const OpConsts& oc, double coeff, c
   ph.c.set(coeff * oc.dt, oc.gamma_gas);
   ph.cD = 1.0 / (oc.gamma_gas - 1.0);
   FieldPtrs<4> in{{pfix, u, u + Ns, p}};
+  if (waves == 0) waves = K == 2 ? 3 : 4;
+  if (waves != 3 && waves != 4) throw std::invalid_argument("d_minus_c_dots: waves per SIMD must be 0, 3 or 4");
   if (m.cface)
-    hipLaunchKernelGGL((k_dmc_dots<K, true>), dim3(grid), dim3(64), 0, s, m, ph, in, dst, scale, Ns, V, ldv, nv,
+    hipLaunchKernelGGL((k_dmc_dots<K, true, 1>), dim3(grid), dim3(64), 0, s, m, ph, in, dst, scale, Ns, V, ldv, nv,
                        p_last, part, pstride);
+  else if (waves == 3)
+    hipLaunchKernelGGL((k_dmc_dots<K, false, 3>), dim3(grid), dim3(64), 0, s, m, ph, in, dst, scale, Ns, V, ldv,
+                       nv, p_last, part, pstride);
   else
-    hipLaunchKernelGGL((k_dmc_dots<K, false>), dim3(grid), dim3(64), 0, s, m, ph, in, dst, scale, Ns, V, ldv, nv,
-                       p_last, part, pstride);
+    hipLaunchKernelGGL((k_dmc_dots<K, false, 4>), dim3(grid), dim3(64), 0, s, m, ph, in, dst, scale, Ns, V, ldv,
+                       nv, p_last, part, pstride);
   ATMO_HIP_CHECK(hipGetLastError());
 }
 
@@ -703,7 +710,7 @@ void launch_rhs_velocity(const MeshDev& m, const OpConsts& oc, const Tableau& tb
                                     const double*, const double*, double*, long, hipStream_t);                   \
   template void launch_d_minus_c_dots<K>(const MeshDev&, const OpConsts&, double, const double*, const double*,    \
                                          const double*, const double*, double*, long, const double*, long, int,  \
-                                         double*, long, hipStream_t);                                            \
+                                         double*, long, int, hipStream_t);                                       \
   template void launch_rhs_rho<K>(const MeshDev&, const OpConsts&, const Tableau&, int, const double* const*,     \
                                   double*, long, hipStream_t);                                                  \
   template void launch_rhs_pressure<K>(const MeshDev&, const OpConsts&, const Tableau&, int,                     \

@@ -12,7 +12,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _native as N
-from .operator import _ptr, default_params, witch_of_agnesi
+from .operator import _ptr, default_params, tuning_dict, witch_of_agnesi
 
 
 def _mesh(degree, nx, nz, h, periodic=True, Lx=40000.0, H=20000.0):
@@ -77,6 +77,17 @@ class Ensemble:
         N.check(N.load().atmo_ensemble_autotune(self.handle, c, len(candidates), probe_steps, sample,
                                                 ctypes.byref(best), _ptr(rates)))
         return best.value, dict(zip(candidates, rates.tolist()))
+
+    def tune_kernels(self, nrep=20):
+        """Kernel-shape autotuner per class of members (same degree, mesh size
+        and precision): times the tunable kernel shapes on one engine of the
+        class and applies the choices to every member of it. Returns one dict
+        per class, in order of first appearance."""
+        cap = 64
+        out = (N.KernelTuning * cap)()
+        n = ctypes.c_int()
+        N.check(N.load().atmo_ensemble_tune_kernels(self.handle, nrep, out, cap, ctypes.byref(n)))
+        return [tuning_dict(out[c]) for c in range(min(n.value, cap))]
 
     def run(self):
         w = ctypes.c_double()

@@ -34,6 +34,11 @@ def default_params(**overrides):
     return p
 
 
+def tuning_dict(t):
+    return dict(arnoldi_us=list(t.arnoldi_us), sweep_us_vector_alus=t.sweep_us[0], sweep_us_matrix_cores=t.sweep_us[1],
+                arnoldi_waves_per_simd=t.arnoldi_waves_per_simd, mg_matrix_cores=t.mg_matrix_cores)
+
+
 def _ptr(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
@@ -191,6 +196,18 @@ class DeviceContext:
         ms, b = ctypes.c_double(), ctypes.c_double()
         N.check(N.load().atmo_time_kernel(self.handle, which, nrep, ctypes.byref(ms), ctypes.byref(b)), self.handle)
         return ms.value, b.value
+
+    def tune_kernels(self, nrep=20):
+        """Kernel-shape autotuner (atmo_tune_kernels): times the Arnoldi kernel
+        at 3 and 4 waves per SIMD and the FP32 multigrid sweep on the vector
+        ALUs and the matrix cores on this context's mesh, keeps the fastest in
+        the context's parameters and returns the timings and choices."""
+        t = N.KernelTuning()
+        N.check(N.load().atmo_tune_kernels(self.handle, nrep, ctypes.byref(t)), self.handle)
+        self.params.arnoldi_waves_per_simd = t.arnoldi_waves_per_simd or self.params.arnoldi_waves_per_simd
+        if t.mg_matrix_cores >= 0:
+            self.params.mg_matrix_cores = t.mg_matrix_cores
+        return tuning_dict(t)
 
 
 # EquationData (equation_data.h:23-33) of the reference's 3D test

@@ -64,6 +64,11 @@ def run_gpu(args):
         forms.append(("fp64_direct_ulp", dict(precision=2)))
     for name, prm in forms:
         ctx = af.DeviceContext(k, nx, nz, h=h, Ma=1.0, Fr=fr, dt=W["dt"], **{"eps": 1e-10, **prm})
+        if name in ("mixed", "fp64_direct"):
+            # kernel shapes timed on this mesh before the timed steps (results do not depend on them)
+            t = ctx.tune_kernels(nrep=20)
+            if name == "mixed":
+                res["kernel_tuning"] = t
         rho, u, p = ctx.init_hill(seed=20240601)
         if name == "fp64_direct_ulp":
             rho = rho * (1.0 + 1e-15 * np.random.default_rng(7).uniform(-1.0, 1.0, rho.shape))
@@ -263,8 +268,16 @@ def run_ensemble(args):
         dofs += 4 * nx * nz * (k + 1) ** 2 * args.ens_steps
     tuned = None
     if args.ens_autotune:
+        # kernel shapes per member class (degree x mesh), then the number of concurrent members
+        classes = ens.tune_kernels(nrep=20)
+        keys = []
+        for d in desc:
+            if (d["k"], d["nx"], d["nz"]) not in keys:
+                keys.append((d["k"], d["nx"], d["nz"]))
+        kernel_shapes = {f"{nx}x{nz}_Q{k}": c for (k, nx, nz), c in zip(keys, classes)}
         slots, rates = ens.autotune(candidates=(4, 8, 12, 16, 24), probe_steps=8, sample=32)
-        tuned = dict(candidates=rates, chosen=slots, probe="first 32 members, 8 steps each, all arriving at once")
+        tuned = dict(candidates=rates, chosen=slots, probe="first 32 members, 8 steps each, all arriving at once",
+                     kernel_shapes=kernel_shapes)
         args.ens_slots = slots
     print(f"[bench] ensemble: {args.ens_members} members x {args.ens_steps} steps, {args.ens_slots} slots",
           file=sys.stderr, flush=True)
@@ -663,6 +676,7 @@ def main():
         "rel_err_vs_fp64_exact_inner": r.get("rel_err_vs_fp64_exact_inner"),
         "solver_stats": {f: r[f]["stats"] for f in ("mixed", "fp64_direct", "fp64") if f in r},
         "matrix_cores": matrix_core_summary(r, dofs, args.steps),
+        "kernel_tuning": r.get("kernel_tuning"),
         "converged_fp64_seconds": r["tight"]["seconds"] if "tight" in r else None,
         "kernels": r.get("kernels"), "roofline": roof, "cpu_baseline": cpu, "ensemble": ensemble,
         "serve": serve, "amr": amr, "sphere": sphere,

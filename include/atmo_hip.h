@@ -82,7 +82,17 @@ typedef struct {
                                   with FP64 multigrid (differs from 1 only in the levels' precision) */
   int mg_matrix_cores;         /* 1: the FP32 multigrid sweeps (precision 1) at degree 2 and 3 run on the
                                   matrix cores (FP32 MFMA sum factorization); 0: on the vector ALUs */
+  int arnoldi_waves_per_simd;  /* waves per SIMD the GMRES Arnoldi kernel of the cellwise-direct forms is
+                                  compiled for: 3 or 4 (0: 3 at degree 2, else 4); the same results */
 } atmo_params;
+
+/* Kernel shapes timed by the autotuners (microseconds per launch) and the choice made. */
+typedef struct {
+  double arnoldi_us[2];        /* the Arnoldi step (14 basis vectors) at 3 and 4 waves per SIMD */
+  double sweep_us[2];          /* the FP32 A-block multigrid sweep on the vector ALUs and on the matrix cores */
+  int arnoldi_waves_per_simd;  /* chosen (0: not tuned: AMR mesh, precision 0 or gmres_restart < 13) */
+  int mg_matrix_cores;         /* chosen (-1: not tuned: no FP32 multigrid at degree 2 or 3) */
+} atmo_kernel_tuning;
 
 typedef struct {
   long cg_its, gmres_its, inner_cg_its, fp_its, mg_cycles, coarse_its, schur_matvecs, host_syncs;
@@ -260,6 +270,12 @@ atmo_status atmo_write_vtu_host(const atmo_mesh_desc* mesh, const double* rho, c
  * same sweep on the vector ALUs (k_cheb_sweep).
  * Also reports the kernel's algorithmic HBM bytes per launch. */
 atmo_status atmo_time_kernel(atmo_ctx* ctx, int which, int nrep, double* ms_per_launch, double* bytes_per_launch);
+/* No counterpart in the reference: the kernel-shape autotuner. Times the tunable
+ * kernel shapes on the context's own mesh (nrep launches each, two alternating
+ * rounds) — the Arnoldi kernel's waves per SIMD and the FP32 multigrid sweep's
+ * form (matrix cores with 16 cells per wave, or vector ALUs) — and sets the
+ * fastest in the context's parameters (results do not depend on the choice). */
+atmo_status atmo_tune_kernels(atmo_ctx* ctx, int nrep, atmo_kernel_tuning* out);
 
 /* ------------------------------------------------------------ spherical shell
  * The reference's own 3D case (EulerSolver<3>, Atmospheric_Flow.cc:1277):
@@ -351,6 +367,13 @@ atmo_status atmo_ensemble_submit(atmo_ensemble* e, const atmo_member_desc* m, in
  * candidate's rate (0 when a member of that probe failed). */
 atmo_status atmo_ensemble_autotune(atmo_ensemble* e, const int* candidates, int ncand, int probe_steps,
                                    int sample_members, int* best_slots, double* rates);
+/* No counterpart in the reference: the kernel-shape autotuner of an ensemble. For
+ * every class of submitted members with one degree, mesh size and precision, times
+ * the tunable kernel shapes on one engine of the class (as atmo_tune_kernels) and
+ * sets the choices for every member of the class; out[c] (c < cap) receives class
+ * c's timings and choices, classes in order of first appearance, and *nclasses
+ * their number. */
+atmo_status atmo_ensemble_tune_kernels(atmo_ensemble* e, int nrep, atmo_kernel_tuning* out, int cap, int* nclasses);
 /* Runs every submitted member; returns when all are done (wall_s may be NULL). */
 atmo_status atmo_ensemble_run(atmo_ensemble* e, double* wall_s);
 /* Result of member id; step_ms (may be NULL) receives up to cap per-step wall times. */
