@@ -12,6 +12,7 @@
 // memory (L2) at the face point its orientation code maps to; no per-point
 // geometry is stored.
 #include <cmath>
+#include <type_traits>
 
 #include "ops.h"
 #include "shell.h"
@@ -425,9 +426,22 @@ struct ShellEpi {
   long pstride = 0;
 };
 
+// Register budget: the Schur complement's kernels (C, D - C, B through the inverse blocks) are compiled
+// for 3 waves per SIMD (168 VGPRs, 8-16 bytes of spills per lane; measured faster than 2 waves at 182-206
+// VGPRs); the residual kernels keep the compiler's choice (at 168 VGPRs they spill hundreds of bytes).
 template <class Phys, int EPI>
-__global__ void __launch_bounds__(64) k_shell(ShellDev m, Phys ph, double* __restrict__ dst, long Ns,
-                                              const double* __restrict__ scale, ShellEpi ep) {
+constexpr int shell_waves() {
+  if constexpr (std::is_same_v<Phys, PhysB>)
+    return EPI == 1 ? 3 : 1;
+  else if constexpr (std::is_same_v<Phys, PhysC<false>> || std::is_same_v<Phys, PhysC<true>>)
+    return 3;
+  else
+    return 1;
+}
+
+template <class Phys, int EPI>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(shell_waves<Phys, EPI>())))
+k_shell(ShellDev m, Phys ph, double* __restrict__ dst, long Ns, const double* __restrict__ scale, ShellEpi ep) {
   constexpr int NF = Phys::NF, NO = Phys::NOUT, CPW = SCPW, NI = 8 * NO;
   static_assert(EPI != 2 || NO == 1, "Arnoldi epilogue: one output field");
   __shared__ double sdof[NF][CPW * 8];
@@ -577,15 +591,35 @@ __global__ void __launch_bounds__(64) k_shell(ShellDev m, Phys ph, double* __res
       for (int pr = pr0; pr < 12; pr += 9) {
         const int f = pr / 3, e = pr % 3;
         const double xi = f < 2 ? (double)(f & 1) : kGQ[e], eta = f < 2 ? kGQ[e] : (double)(f & 1);
-        LatGeo H;
-        lat_geo(sS[w], xi, eta, H);
         const double sg = (f & 1) ? 1.0 : -1.0;
-        // area direction C0 (xi faces) or C1 (eta faces), blended arithmetically: a select
-        // between the two member arrays becomes a select of addresses, which spills H
+        // area direction on the face: C0 = N_eta x N (xi faces) or C1 = N x N_xi (eta faces). On a face
+        // one Lagrange factor of the map is a unit vector, so N and its derivative along the face's edge
+        // are sums over the three support directions of that edge alone: the same nonzero terms, in the
+        // same order, as lat_geo's full sums, whose other terms are exact zeros
+        double Nf[3] = {0.0, 0.0, 0.0}, Nd[3] = {0.0, 0.0, 0.0};
+        {
+          const int s0 = f < 2 ? 2 * (f & 1) : 6 * (f & 1), st = f < 2 ? 3 : 1;
+          const double ue = kGQ[e];
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            const double Lk = q2v(k, ue), Dk = q2d(k, ue);
+            const double* T = sS[w][s0 + st * k];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+              Nf[d] += Lk * T[d];
+              Nd[d] += Dk * T[d];
+            }
+          }
+        }
+        // blended arithmetically: a per-lane select between the two arrays becomes a select of addresses
         const double k0 = f < 2 ? 1.0 : 0.0, k1 = 1.0 - k0;
+        const double C0[3] = {Nd[1] * Nf[2] - Nd[2] * Nf[1], Nd[2] * Nf[0] - Nd[0] * Nf[2],
+                              Nd[0] * Nf[1] - Nd[1] * Nf[0]};
+        const double C1[3] = {Nf[1] * Nd[2] - Nf[2] * Nd[1], Nf[2] * Nd[0] - Nf[0] * Nd[2],
+                              Nf[0] * Nd[1] - Nf[1] * Nd[0]};
         double Cf[3];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) Cf[d] = k0 * H.C0[d] + k1 * H.C1[d];
+        for (int d = 0; d < 3; ++d) Cf[d] = k0 * C0[d] + k1 * C1[d];
         // test functions on face f, by value: the lateral factor along the face's edge
         // coordinate, zero off the face
         double wl[4];  // i & 3 -> lateral weight of test function (i & 1, (i >> 1) & 1)
