@@ -720,43 +720,74 @@ void launch_coarse_cg(const MeshDev& m, int ncomp, bool weighted, const double* 
   ATMO_HIP_CHECK(hipGetLastError());
 }
 
+// CG of one level's eigenvalue estimate per workgroup (1024 threads), all iterations inside: the
+// coefficients alpha_k, beta_k go to E.hist for the Lanczos tridiagonal matrix (the iterate itself is
+// not formed: nothing reads it). Every sum has a fixed order: a thread takes elements tid + 1024 j in
+// increasing j, then each wave's shuffle tree, then the waves in order. Iterations after the stopping
+// test succeeds or fails record nothing, so the loop ends there.
 template <int K, int NC, bool W>
 __global__ __launch_bounds__(1024) void k_eig_cg(EigBatch B, double coeff, int nit, double tol) {
   using C = Cfg<K>;
-  constexpr int ND2 = C::ND2, CPW = C::CPW, NW = 16;
+  constexpr int ND2 = C::ND2, CPW = C::CPW, NW = 16, NT = NW * 64;
+  constexpr int NU = NC * CPW * ND2, NS = NU + (W ? CPW * ND2 : 0), NLD = (NS + 63) / 64;
+  constexpr int U = 4;  // element loops: batches of U elements per thread, every load issued before use
   __shared__ double s_u[NW][NC * CPW * ND2];
   __shared__ double s_r[NW][W ? CPW * ND2 : 1];
   __shared__ double s_red[NW][NC * CPW * C::ND * C::NQ];
-  __shared__ double s_sum[NW];
+  __shared__ double s_sum[2][NW];
   const EigLevel& E = B.L[blockIdx.x];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const long n = (long)NC * E.Ns;
-  // block sum in a fixed order: each wave's shuffle tree, then the waves in order
-  auto bsum = [&](double v) {
-    v = wsum(v);
-    if (lane == 0) s_sum[w] = v;
+  // block sums of a and (two) b
+  auto bsum = [&](double& a, double& b, bool two) {
+    a = wsum(a);
+    if (two) b = wsum(b);
+    if (lane == 0) {
+      s_sum[0][w] = a;
+      if (two) s_sum[1][w] = b;
+    }
     __syncthreads();
-    double t = 0;
+    double ta = 0, tb = 0;
 #pragma unroll
-    for (int i = 0; i < NW; ++i) t += s_sum[i];
+    for (int i = 0; i < NW; ++i) {
+      ta += s_sum[0][i];
+      if (two) tb += s_sum[1][i];
+    }
     __syncthreads();
-    return t;
+    a = ta;
+    b = tb;
   };
-  // dst = A src on the level: the waves take interleaved passes of CPW cells, each in its own LDS
+  // dst = A src on the level: the waves take interleaved passes of CPW cells, each in its own LDS; the
+  // next pass's values are loaded while the current pass is computed
+  auto stage_load = [&](int p, const double* src, double (&v)[NLD]) {
+    const long base = (long)p * CPW * ND2, lim = (long)E.m.ncell * ND2;
+#pragma unroll
+    for (int k = 0; k < NLD; ++k) {
+      const int q = lane + 64 * k;
+      const int comp = q < NU ? q / (CPW * ND2) : 0;
+      const int t = q < NU ? q - comp * CPW * ND2 : q - NU;
+      const long gi = base + t;
+      v[k] = 0.0;
+      if (q < NS && gi < lim) v[k] = q < NU ? src[comp * E.Ns + gi] : E.rho[gi];
+    }
+  };
   auto apply = [&](const double* src, double* dst) {
     const int passes = (E.m.ncell + CPW - 1) / CPW;
+    double v[NLD];
+    if (w < passes) stage_load(w, src, v);
     for (int p = w; p < passes; p += NW) {
-      const int cell0 = p * CPW;
-      const long base = (long)cell0 * ND2, lim = (long)E.m.ncell * ND2;
-      for (int t = lane; t < CPW * ND2; t += 64) {
-        const long gi = base + t;
 #pragma unroll
-        for (int comp = 0; comp < NC; ++comp) s_u[w][comp * CPW * ND2 + t] = gi < lim ? src[comp * E.Ns + gi] : 0.0;
-        if (W) s_r[w][t] = gi < lim ? E.rho[gi] : 0.0;
+      for (int k = 0; k < NLD; ++k) {
+        const int q = lane + 64 * k;
+        if (q < NU) s_u[w][q] = v[k];
+        else if (W && q < NS) s_r[w][q - NU] = v[k];
       }
+      if (p + NW < passes) stage_load(p + NW, src, v);
       wave_sync();
+      const int cell0 = p * CPW;
       wave_mass_core<K, double, NC, W>(E.m, cell0, s_u[w], s_r[w], coeff, s_red[w]);
       wave_sync();
+      const long base = (long)cell0 * ND2, lim = (long)E.m.ncell * ND2;
       double Si[C::NQ];
       lane_cols_S<K, double>(lane % C::ND, Si);
       for (int t = lane; t < CPW * ND2; t += 64) {
@@ -770,53 +801,103 @@ __global__ __launch_bounds__(1024) void k_eig_cg(EigBatch B, double coeff, int n
       wave_sync();
     }
   };
-  double part = 0;
-  for (long i = threadIdx.x; i < n; i += blockDim.x) {
-    const double gi = -E.b[i];
-    const double hi = gi * E.P[i];
-    E.g[i] = gi;
-    E.x[i] = 0.0;
-    E.h[i] = hi;
-    E.d[i] = -hi;
-    part += gi * hi;
+  double* __restrict__ g = E.g;
+  double* __restrict__ d = E.d;
+  double* __restrict__ h = E.h;
+  const double* __restrict__ P = E.P;
+  const double* __restrict__ b = E.b;
+  double part = 0, unused = 0;
+  for (long i0 = threadIdx.x; i0 < n; i0 += U * NT) {
+    double bv[U], pv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long i = i0 + (long)u * NT;
+      bv[u] = i < n ? b[i] : 0.0;
+      pv[u] = i < n ? P[i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long i = i0 + (long)u * NT;
+      if (i >= n) break;
+      const double gi = -bv[u];
+      const double hi = gi * pv[u];
+      g[i] = gi;
+      h[i] = hi;
+      d[i] = -hi;
+      part += gi * hi;
+    }
   }
-  double gh = bsum(part);
+  bsum(part, unused, false);
+  double gh = part;
   int it = 0, done = 0;
   for (int k = 0; k < nit; ++k) {
     __syncthreads();
-    apply(E.d, E.h);
+    apply(d, h);
     __syncthreads();
     part = 0;
-    for (long i = threadIdx.x; i < n; i += blockDim.x) part += E.d[i] * E.h[i];
-    const double dh = bsum(part);
-    if (!done) {
-      const double alpha = gh / dh;
-      if (threadIdx.x == 0) E.hist[2 * k] = alpha;
-      part = 0;
-      for (long i = threadIdx.x; i < n; i += blockDim.x) {
-        E.x[i] += alpha * E.d[i];
-        const double gi = E.g[i] + alpha * E.h[i];
-        E.g[i] = gi;
-        part += gi * gi;
+    for (long i0 = threadIdx.x; i0 < n; i0 += U * NT) {
+      double dv[U], hv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const long i = i0 + (long)u * NT;
+        dv[u] = i < n ? d[i] : 0.0;
+        hv[u] = i < n ? h[i] : 0.0;
       }
-      const double res = sqrt(bsum(part));
-      ++it;
-      if (res <= tol) done = 1;
-      else if (it >= nit || isnan(res)) done = -1;
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (i0 + (long)u * NT < n) part += dv[u] * hv[u];
     }
-    if (k + 1 == nit) break;
-    part = 0;
-    for (long i = threadIdx.x; i < n; i += blockDim.x) {
-      const double hi = E.g[i] * E.P[i];
-      E.h[i] = hi;
-      part += E.g[i] * hi;
+    bsum(part, unused, false);
+    const double alpha = gh / part;
+    if (threadIdx.x == 0) E.hist[2 * k] = alpha;
+    // g += alpha h with |g|^2, and (unless this is the last iteration) h = P g with g.h
+    const bool more = k + 1 < nit;
+    double pgg = 0, pgh = 0;
+    for (long i0 = threadIdx.x; i0 < n; i0 += U * NT) {
+      double gv[U], hv[U], pv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const long i = i0 + (long)u * NT;
+        gv[u] = i < n ? g[i] : 0.0;
+        hv[u] = i < n ? h[i] : 0.0;
+        pv[u] = i < n && more ? P[i] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const long i = i0 + (long)u * NT;
+        if (i >= n) break;
+        const double gi = gv[u] + alpha * hv[u];
+        g[i] = gi;
+        pgg += gi * gi;
+        if (more) {
+          const double hi = gi * pv[u];
+          h[i] = hi;
+          pgh += gi * hi;
+        }
+      }
     }
-    const double ggh = bsum(part);
-    if (!done) {
-      const double beta = ggh / gh;
-      gh = ggh;
-      if (threadIdx.x == 0) E.hist[2 * k + 1] = beta;
-      for (long i = threadIdx.x; i < n; i += blockDim.x) E.d[i] = -E.h[i] + beta * E.d[i];
+    bsum(pgg, pgh, more);
+    const double res = sqrt(pgg);
+    ++it;
+    if (res <= tol) done = 1;
+    else if (it >= nit || isnan(res)) done = -1;
+    if (done) break;
+    const double beta = pgh / gh;
+    gh = pgh;
+    if (threadIdx.x == 0) E.hist[2 * k + 1] = beta;
+    for (long i0 = threadIdx.x; i0 < n; i0 += U * NT) {
+      double hv[U], dv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const long i = i0 + (long)u * NT;
+        hv[u] = i < n ? h[i] : 0.0;
+        dv[u] = i < n ? d[i] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const long i = i0 + (long)u * NT;
+        if (i < n) d[i] = -hv[u] + beta * dv[u];
+      }
     }
   }
   if (threadIdx.x == 0) {
