@@ -608,22 +608,33 @@ __global__ __launch_bounds__(64) void k_coarse_cg(MeshDev m, const double* __res
 }
 
 // The same CG with every vector of the level (x, g, d, h and the density)
-// held in LDS for the whole solve: an iteration makes no global memory
-// access. Same operations in the same order as k_coarse_cg (one wave; the
-// operator is applied one component at a time, with the same column
-// weights), so the results are those of k_coarse_cg.
-constexpr size_t COARSE_LDS_MAX = 60 * 1024;  // bytes of level vectors a coarse solve keeps in LDS
+// held in LDS for the whole solve, and each lane's column weights (the
+// Jacobian, quadrature weights and interpolated density of its quadrature
+// column in each pass, which do not change across iterations) formed once
+// into LDS: an iteration makes no global memory access. Same operations in
+// the same order as k_coarse_cg (one wave; the operator is applied one
+// component at a time with wave_mass_core's arithmetic), so the results are
+// those of k_coarse_cg.
+constexpr size_t COARSE_LDS_MAX = 60 * 1024;  // bytes of level vectors and weights a coarse solve keeps in LDS
+
+template <int K>
+size_t coarse_lds_bytes(int ncell, int ncomp, bool weighted, long Ns) {
+  const int passes = (ncell + Cfg<K>::CPW - 1) / Cfg<K>::CPW;
+  return ((size_t)(4 * ncomp + (weighted ? 1 : 0)) * Ns + (size_t)passes * 64 * Cfg<K>::NQ) * sizeof(double);
+}
 
 template <int K, int NC, bool W>
 __global__ __launch_bounds__(64) void k_coarse_cg_lds(MeshDev m, const double* __restrict__ rho, double coeff,
                                                       const double* __restrict__ b, double* __restrict__ x_out,
                                                       const double* tolp, int max_its, int* its, long Ns) {
   using C = Cfg<K>;
-  constexpr int ND2 = C::ND2, CPW = C::CPW;
+  constexpr int ND = C::ND, NQ = C::NQ, ND2 = C::ND2, CPW = C::CPW;
+  constexpr auto& tb = basis::kT<K>;
   extern __shared__ double lv[];
   __shared__ double s_red[CPW * C::ND * C::NQ];
   const long n = (long)NC * Ns;
   double *x = lv, *g = lv + n, *d = lv + 2 * n, *h = lv + 3 * n, *r = lv + 4 * n;
+  double* wql = lv + 4 * n + (W ? Ns : 0);  // [pass][lane][NQ]
   const double tol = *tolp;
   double gg = 0;
   for (long i = threadIdx.x; i < n; i += 64) {
@@ -644,6 +655,23 @@ __global__ __launch_bounds__(64) void k_coarse_cg_lds(MeshDev m, const double* _
   double gh = gg;
   wave_sync();
   const int passes = (m.ncell + CPW - 1) / CPW;
+  const int lane = threadIdx.x & 63, cw = lane / NQ, c = lane - cw * NQ;
+  double Sc[ND];
+  {
+    double Dc[ND];
+    lane_rows<K, double>(c, Sc, Dc);
+  }
+  // the lane's column weights in every pass (wave_mass_core's column_weights, once per solve)
+  for (int p = 0; p < passes; ++p) {
+    const int cell = p * CPW + cw;
+    if (cw < CPW && cell < m.ncell) {
+      double wq[NQ];
+      column_weights<K, double, W>(cell_verts(m, cell), c, cw, r + (long)p * CPW * ND2, coeff, wq, Sc);
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) wql[((long)p * 64 + lane) * NQ + q] = wq[q];
+    }
+  }
+  wave_sync();
   double Si[C::NQ];
   lane_cols_S<K, double>(threadIdx.x % C::ND, Si);
   while (status == 0) {
@@ -654,7 +682,33 @@ __global__ __launch_bounds__(64) void k_coarse_cg_lds(MeshDev m, const double* _
       const long base = (long)cell0 * ND2, lim = (long)m.ncell * ND2;
 #pragma unroll
       for (int comp = 0; comp < NC; ++comp) {
-        wave_mass_core<K, double, 1, W>(m, cell0, d + comp * Ns + base, r + base, coeff, s_red);
+        // wave_mass_core's column phase with the stored weights
+        if (cw < CPW && cell0 + cw < m.ncell) {
+          const double* u = d + comp * Ns + base + cw * ND2;
+          const double* wq = wql + ((long)p * 64 + lane) * NQ;
+          double tv[ND];
+#pragma unroll
+          for (int j = 0; j < ND; ++j) {
+            double sum = 0;
+#pragma unroll
+            for (int i = 0; i < ND; ++i) sum += Sc[i] * u[j * ND + i];
+            tv[j] = sum;
+          }
+          double Tv[ND];
+#pragma unroll
+          for (int j = 0; j < ND; ++j) Tv[j] = 0;
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) {
+            double val = 0;
+#pragma unroll
+            for (int j = 0; j < ND; ++j) val += (double)tb.S[q][j] * tv[j];
+            val *= wq[q];
+#pragma unroll
+            for (int j = 0; j < ND; ++j) Tv[j] += (double)tb.S[q][j] * val;
+          }
+#pragma unroll
+          for (int j = 0; j < ND; ++j) s_red[cw * ND * NQ + j * NQ + c] = Tv[j];
+        }
         wave_sync();
         for (int t = threadIdx.x; t < CPW * ND2; t += 64) {
           if (base + t >= lim) break;
@@ -694,7 +748,7 @@ template <int K>
 void launch_coarse_cg(const MeshDev& m, int ncomp, bool weighted, const double* rho, double coeff, const double* b,
                       double* x, double* g, double* d, double* h, const double* tol, int max_its, int* its, long Ns,
                       hipStream_t s) {
-  const size_t lds = (size_t)(4 * ncomp + (weighted ? 1 : 0)) * Ns * sizeof(double);
+  const size_t lds = coarse_lds_bytes<K>(m.ncell, ncomp, weighted, Ns);
   if (lds <= COARSE_LDS_MAX) {
     if (weighted)
       hipLaunchKernelGGL((k_coarse_cg_lds<K, 2, true>), dim3(1), dim3(64), lds, s, m, rho, coeff, b, x, tol, max_its,
