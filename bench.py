@@ -599,6 +599,8 @@ def main():
     ap.add_argument("--sphere-steps", type=int, default=5)
     ap.add_argument("--sphere-cpu-steps", type=int, default=3)
     ap.add_argument("--sphere-only", action="store_true", help="only the spherical-shell case (profiling runs)")
+    ap.add_argument("--ensemble-only", action="store_true",
+                    help="only the ensemble leg, with --ens-slots/--ens-steps as given (profiling runs)")
     ap.add_argument("--cpu-headline-steps", type=int, default=0,
                     help="only the CPU oracle on the headline workload for this many steps (no GPU)")
     args = ap.parse_args()
@@ -611,6 +613,9 @@ def main():
         return
     if args.sphere_only:
         print(json.dumps({"sphere": run_sphere(args)}))
+        return
+    if args.ensemble_only:
+        print(json.dumps({"ensemble": run_ensemble(args)}))
         return
     if args.cpu_headline_steps:
         print(json.dumps({"cpu_headline": cpu_headline(args.cpu_threads, args.cpu_headline_steps)}))
