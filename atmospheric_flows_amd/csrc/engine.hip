@@ -904,19 +904,12 @@ class EngineK : public Engine {
     Level& L = *lv[l];
     const int nc = ncomp(blk);
     if (l == 0) {
-      const double* b64;
-      double* x64;
-      if constexpr (std::is_same<T, double>::value) {
-        b64 = b;
-        x64 = x;
-      } else {
-        vec_cast<double, T>((long)nc * L.Ns, b, L.b64.p, s_);
-        b64 = L.b64.p;
-        x64 = L.x64.p;
-      }
-      launch_coarse_cg<K>(L.m, nc, blk == BLK_A, L.rho64.p, coeff(blk), b64, x64, L.cg_g.p, L.cg_d.p, L.cg_h.p, tol,
-                          max_its, coarse_its.p, L.Ns, s_);
-      if constexpr (!std::is_same<T, double>::value) vec_cast<T, double>((long)nc * L.Ns, x64, x, s_);
+      if constexpr (std::is_same<T, double>::value)
+        launch_coarse_cg<K>(L.m, nc, blk == BLK_A, L.rho64.p, coeff(blk), b, x, L.cg_g.p, L.cg_d.p, L.cg_h.p, tol,
+                            max_its, coarse_its.p, L.Ns, s_);
+      else
+        launch_coarse_cg_f32<K>(L.m, nc, blk == BLK_A, L.rho64.p, coeff(blk), b, x, L.b64.p, L.x64.p, L.cg_g.p,
+                                L.cg_d.p, L.cg_h.p, tol, max_its, coarse_its.p, L.Ns, s_);
       return;
     }
     // pre-smoothing from a zero guess and the residual t = b - A x: one kernel

@@ -48,6 +48,13 @@ template <int K>
 void launch_coarse_cg(const MeshDev& m, int ncomp, bool weighted, const double* rho, double coeff, const double* b,
                       double* x, double* g, double* d, double* h, const double* tol, int max_its, int* its, long Ns,
                       hipStream_t s);
+// The same for FP32 level vectors b, x (the mixed form): the solve runs in FP64 as above, b and x
+// converted as by vec_cast (inside the kernel when the level fits LDS; b64, x64 are the FP64 copies
+// used otherwise).
+template <int K>
+void launch_coarse_cg_f32(const MeshDev& m, int ncomp, bool weighted, const double* rho, double coeff, const float* b,
+                          float* x, double* b64, double* x64, double* g, double* d, double* h, const double* tol,
+                          int max_its, int* its, long Ns, hipStream_t s);
 
 // Chebyshev eigenvalue estimates of several small levels in one launch: one
 // workgroup of 16 waves per level runs all nit CG iterations (Jacobi

@@ -623,9 +623,11 @@ size_t coarse_lds_bytes(int ncell, int ncomp, bool weighted, long Ns) {
   return ((size_t)(4 * ncomp + (weighted ? 1 : 0)) * Ns + (size_t)passes * 64 * Cfg<K>::NQ) * sizeof(double);
 }
 
-template <int K, int NC, bool W>
+// TB, TX: the types b is read and x written in (float in the mixed form, whose levels are FP32: the
+// conversions are those of a cast kernel before and after the solve).
+template <int K, int NC, bool W, typename TB, typename TX>
 __global__ __launch_bounds__(64) void k_coarse_cg_lds(MeshDev m, const double* __restrict__ rho, double coeff,
-                                                      const double* __restrict__ b, double* __restrict__ x_out,
+                                                      const TB* __restrict__ b, TX* __restrict__ x_out,
                                                       const double* tolp, int max_its, int* its, long Ns) {
   using C = Cfg<K>;
   constexpr int ND = C::ND, NQ = C::NQ, ND2 = C::ND2, CPW = C::CPW;
@@ -638,7 +640,7 @@ __global__ __launch_bounds__(64) void k_coarse_cg_lds(MeshDev m, const double* _
   const double tol = *tolp;
   double gg = 0;
   for (long i = threadIdx.x; i < n; i += 64) {
-    const double gi = -b[i];
+    const double gi = -(double)b[i];
     x[i] = 0.0;
     g[i] = gi;
     d[i] = -gi;
@@ -740,28 +742,33 @@ __global__ __launch_bounds__(64) void k_coarse_cg_lds(MeshDev m, const double* _
     wave_sync();
   }
   wave_sync();
-  for (long i = threadIdx.x; i < n; i += 64) x_out[i] = x[i];
+  for (long i = threadIdx.x; i < n; i += 64) x_out[i] = (TX)x[i];
   if (threadIdx.x == 0) *its = status == 1 ? it : -1;
+}
+
+template <int K, typename TB, typename TX>
+static bool coarse_cg_lds(const MeshDev& m, int ncomp, bool weighted, const double* rho, double coeff, const TB* b,
+                          TX* x, const double* tol, int max_its, int* its, long Ns, hipStream_t s) {
+  const size_t lds = coarse_lds_bytes<K>(m.ncell, ncomp, weighted, Ns);
+  if (lds > COARSE_LDS_MAX) return false;
+  if (weighted)
+    hipLaunchKernelGGL((k_coarse_cg_lds<K, 2, true, TB, TX>), dim3(1), dim3(64), lds, s, m, rho, coeff, b, x, tol,
+                       max_its, its, Ns);
+  else if (ncomp == 1)
+    hipLaunchKernelGGL((k_coarse_cg_lds<K, 1, false, TB, TX>), dim3(1), dim3(64), lds, s, m, rho, coeff, b, x, tol,
+                       max_its, its, Ns);
+  else
+    hipLaunchKernelGGL((k_coarse_cg_lds<K, 2, false, TB, TX>), dim3(1), dim3(64), lds, s, m, rho, coeff, b, x, tol,
+                       max_its, its, Ns);
+  ATMO_HIP_CHECK(hipGetLastError());
+  return true;
 }
 
 template <int K>
 void launch_coarse_cg(const MeshDev& m, int ncomp, bool weighted, const double* rho, double coeff, const double* b,
                       double* x, double* g, double* d, double* h, const double* tol, int max_its, int* its, long Ns,
                       hipStream_t s) {
-  const size_t lds = coarse_lds_bytes<K>(m.ncell, ncomp, weighted, Ns);
-  if (lds <= COARSE_LDS_MAX) {
-    if (weighted)
-      hipLaunchKernelGGL((k_coarse_cg_lds<K, 2, true>), dim3(1), dim3(64), lds, s, m, rho, coeff, b, x, tol, max_its,
-                         its, Ns);
-    else if (ncomp == 1)
-      hipLaunchKernelGGL((k_coarse_cg_lds<K, 1, false>), dim3(1), dim3(64), lds, s, m, rho, coeff, b, x, tol,
-                         max_its, its, Ns);
-    else
-      hipLaunchKernelGGL((k_coarse_cg_lds<K, 2, false>), dim3(1), dim3(64), lds, s, m, rho, coeff, b, x, tol,
-                         max_its, its, Ns);
-    ATMO_HIP_CHECK(hipGetLastError());
-    return;
-  }
+  if (coarse_cg_lds<K, double, double>(m, ncomp, weighted, rho, coeff, b, x, tol, max_its, its, Ns, s)) return;
   if (weighted)
     hipLaunchKernelGGL((k_coarse_cg<K, 2, true>), dim3(1), dim3(64), 0, s, m, rho, coeff, b, x, g, d, h, tol, max_its,
                        its, Ns);
@@ -772,6 +779,17 @@ void launch_coarse_cg(const MeshDev& m, int ncomp, bool weighted, const double* 
     hipLaunchKernelGGL((k_coarse_cg<K, 2, false>), dim3(1), dim3(64), 0, s, m, rho, coeff, b, x, g, d, h, tol,
                        max_its, its, Ns);
   ATMO_HIP_CHECK(hipGetLastError());
+}
+
+template <int K>
+void launch_coarse_cg_f32(const MeshDev& m, int ncomp, bool weighted, const double* rho, double coeff, const float* b,
+                          float* x, double* b64, double* x64, double* g, double* d, double* h, const double* tol,
+                          int max_its, int* its, long Ns, hipStream_t s) {
+  if (coarse_cg_lds<K, float, float>(m, ncomp, weighted, rho, coeff, b, x, tol, max_its, its, Ns, s)) return;
+  const long n = (long)ncomp * Ns;
+  vec_cast<double, float>(n, b, b64, s);
+  launch_coarse_cg<K>(m, ncomp, weighted, rho, coeff, b64, x64, g, d, h, tol, max_its, its, Ns, s);
+  vec_cast<float, double>(n, x64, x, s);
 }
 
 // CG of one level's eigenvalue estimate per workgroup (1024 threads), all iterations inside: the
@@ -995,6 +1013,9 @@ void launch_eig_cg_batch(const EigBatch& B, int ncomp, bool weighted, double coe
                                                         const double*, const double*, double, const double*,         \
                                                         const double*, const ChebSteps&, long, long, bool,           \
                                                         hipStream_t);                                                \
+  template void launch_coarse_cg_f32<K>(const MeshDev&, int, bool, const double*, double, const float*, float*,   \
+                                        double*, double*, double*, double*, double*, const double*, int, int*, long, \
+                                        hipStream_t);                                                              \
   template void launch_coarse_cg<K>(const MeshDev&, int, bool, const double*, double, const double*, double*,     \
                                     double*, double*, double*, const double*, int, int*, long, hipStream_t);       \
   template void launch_eig_cg_batch<K>(const EigBatch&, int, bool, double, int, double, hipStream_t);
