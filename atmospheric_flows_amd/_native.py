@@ -145,7 +145,8 @@ SIGNATURES = [
     ("atmo_ensemble_destroy", ctypes.c_int, [ctypes.c_void_p]),
     ("atmo_ensemble_submit", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(MemberDesc), ctypes.POINTER(ctypes.c_int)]),
     ("atmo_ensemble_autotune", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.c_int,
-                                              ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _dp]),
+                                              ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                              ctypes.POINTER(ctypes.c_int), _dp]),
     ("atmo_ensemble_run", ctypes.c_int, [ctypes.c_void_p, _dp]),
     ("atmo_ensemble_result", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(MemberResult), _dp,
                                             ctypes.c_int]),

@@ -806,11 +806,12 @@ atmo_status atmo_ensemble_submit(atmo_ensemble* e, const atmo_member_desc* m, in
 }
 
 atmo_status atmo_ensemble_autotune(atmo_ensemble* e, const int* candidates, int ncand, int probe_steps,
-                                   int sample_members, int* best_slots, double* rates) {
+                                   int sample_members, double rate_tolerance, int* best_slots, double* rates) {
   return guarded(nullptr, [&]() {
     if (!e || !candidates || ncand < 1 || !best_slots) throw std::invalid_argument("null argument");
     std::vector<double> r;
-    const int best = e->ens->autotune(std::vector<int>(candidates, candidates + ncand), probe_steps, sample_members, r);
+    const int best = e->ens->autotune(std::vector<int>(candidates, candidates + ncand), probe_steps, sample_members,
+                                      rate_tolerance, r);
     *best_slots = best;
     if (rates)
       for (int i = 0; i < ncand; ++i) rates[i] = r[i];

@@ -158,16 +158,18 @@ void Ensemble::run() {
   wall_s_ = now_s() - t0_;
 }
 
-int Ensemble::autotune(const std::vector<int>& candidates, int probe_steps, int sample, std::vector<double>& rates) {
+int Ensemble::autotune(const std::vector<int>& candidates, int probe_steps, int sample, double rate_tolerance,
+                       std::vector<double>& rates) {
   if (candidates.empty()) throw std::invalid_argument("autotune: no candidates");
   if (probe_steps < 1 || sample < 1) throw std::invalid_argument("autotune: probe_steps and sample must be >= 1");
+  if (!(rate_tolerance >= 0.0 && rate_tolerance < 1.0))
+    throw std::invalid_argument("autotune: rate_tolerance must be in [0, 1)");
   for (int c : candidates)
     if (c < 1 || c > 64) throw std::invalid_argument("autotune: candidate slot counts must be 1..64");
   const int n = std::min<int>(sample, (int)specs_.size());
   if (n < 1) throw std::invalid_argument("autotune: submit members first");
   rates.assign(candidates.size(), 0.0);
-  int best = slots_;
-  double best_rate = -1;
+  double best_rate = 0;
   for (size_t ci = 0; ci < candidates.size(); ++ci) {
     Ensemble probe(candidates[ci], budget_, false);
     double dof_steps = 0;
@@ -189,10 +191,15 @@ int Ensemble::autotune(const std::vector<int>& candidates, int probe_steps, int 
       t_last = std::max(t_last, r.finish_s);
     }
     rates[ci] = ok && t_last > t_first ? dof_steps / (t_last - t_first) : 0.0;
-    if (rates[ci] > best_rate) {
-      best_rate = rates[ci];
-      best = candidates[ci];
-    }
+    best_rate = std::max(best_rate, rates[ci]);
+  }
+  // the fewest slots within the tolerance of the best rate (the slot count unchanged if every probe failed)
+  int best = slots_;
+  if (best_rate > 0) {
+    best = 0;
+    for (size_t ci = 0; ci < candidates.size(); ++ci)
+      if (rates[ci] > 0 && rates[ci] >= (1.0 - rate_tolerance) * best_rate && (best == 0 || candidates[ci] < best))
+        best = candidates[ci];
   }
   slots_ = best;
   return best;

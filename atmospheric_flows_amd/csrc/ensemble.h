@@ -51,10 +51,13 @@ class Ensemble {
   // Batch-shape autotuner: runs the first `sample` submitted members (all
   // arriving at once) for `probe_steps` steps with each candidate number of
   // slots, measures the aggregate DoF-steps/s of the stepping phase (from the
-  // first member's first step to the last member's last), keeps the best
-  // candidate as this ensemble's slot count and returns it; rates[i] is
-  // candidate i's rate (0 when a member of the probe failed).
-  int autotune(const std::vector<int>& candidates, int probe_steps, int sample, std::vector<double>& rates);
+  // first member's first step to the last member's last), keeps as this
+  // ensemble's slot count the fewest slots whose rate is within
+  // rate_tolerance (a fraction, 0 <= t < 1) of the best rate (0: the fastest
+  // candidate; fewer concurrent members step with lower latency) and returns
+  // it; rates[i] is candidate i's rate (0 when a member of the probe failed).
+  int autotune(const std::vector<int>& candidates, int probe_steps, int sample, double rate_tolerance,
+               std::vector<double>& rates);
   // Kernel-shape autotuner: for every class of submitted members with the same degree, mesh size and
   // precision, creates one engine of the class (the first member's mesh and parameters), times the
   // tunable kernel shapes on it (Engine::tune_kernels) and sets the choices in the parameters of every

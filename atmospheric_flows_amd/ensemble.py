@@ -66,16 +66,17 @@ class Ensemble:
         self.members.append(m)
         return i.value
 
-    def autotune(self, candidates=(4, 8, 12, 16), probe_steps=8, sample=32):
+    def autotune(self, candidates=(4, 8, 12, 16), probe_steps=8, sample=32, rate_tolerance=0.0):
         """Picks the number of concurrent members (slots) by measurement:
         the first `sample` submitted members run `probe_steps` steps with each
-        candidate; the fastest becomes this ensemble's slot count. Returns
-        (best, {candidate: DoF-steps/s})."""
+        candidate; the fewest slots whose aggregate DoF-steps/s is within
+        `rate_tolerance` (a fraction; 0 = the fastest) of the best rate becomes
+        this ensemble's slot count. Returns (best, {candidate: DoF-steps/s})."""
         c = (ctypes.c_int * len(candidates))(*candidates)
         rates = np.zeros(len(candidates))
         best = ctypes.c_int()
         N.check(N.load().atmo_ensemble_autotune(self.handle, c, len(candidates), probe_steps, sample,
-                                                ctypes.byref(best), _ptr(rates)))
+                                                float(rate_tolerance), ctypes.byref(best), _ptr(rates)))
         return best.value, dict(zip(candidates, rates.tolist()))
 
     def tune_kernels(self, nrep=20):

@@ -275,9 +275,11 @@ def run_ensemble(args):
             if (d["k"], d["nx"], d["nz"]) not in keys:
                 keys.append((d["k"], d["nx"], d["nz"]))
         kernel_shapes = {f"{nx}x{nz}_Q{k}": c for (k, nx, nz), c in zip(keys, classes)}
-        slots, rates = ens.autotune(candidates=(4, 8, 12, 16, 24), probe_steps=8, sample=32)
+        # the fewest concurrent members within 5 % of the best aggregate rate: more slots than that add member
+        # latency for little throughput
+        slots, rates = ens.autotune(candidates=(4, 8, 12, 16, 24), probe_steps=8, sample=32, rate_tolerance=0.05)
         tuned = dict(candidates=rates, chosen=slots, probe="first 32 members, 8 steps each, all arriving at once",
-                     kernel_shapes=kernel_shapes)
+                     rule="fewest slots within 5 % of the best probe rate", kernel_shapes=kernel_shapes)
         args.ens_slots = slots
     print(f"[bench] ensemble: {args.ens_members} members x {args.ens_steps} steps, {args.ens_slots} slots",
           file=sys.stderr, flush=True)

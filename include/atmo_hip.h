@@ -362,11 +362,13 @@ atmo_status atmo_ensemble_submit(atmo_ensemble* e, const atmo_member_desc* m, in
 /* No counterpart in the reference: the batch-shape autotuner. Runs the first
  * sample_members submitted members (arriving at once) for probe_steps steps
  * with each of the ncand candidate slot counts (1..64), measures the
- * aggregate DoF-steps/s of the stepping phase, and keeps the best as the
- * ensemble's slot count (*best_slots); rates (may be NULL) receives each
- * candidate's rate (0 when a member of that probe failed). */
+ * aggregate DoF-steps/s of the stepping phase, and keeps as the ensemble's
+ * slot count (*best_slots) the fewest slots whose rate is within
+ * rate_tolerance (a fraction in [0, 1); 0 = the fastest candidate) of the
+ * best rate: fewer concurrent members step with lower latency. rates (may be
+ * NULL) receives each candidate's rate (0 when a member of that probe failed). */
 atmo_status atmo_ensemble_autotune(atmo_ensemble* e, const int* candidates, int ncand, int probe_steps,
-                                   int sample_members, int* best_slots, double* rates);
+                                   int sample_members, double rate_tolerance, int* best_slots, double* rates);
 /* No counterpart in the reference: the kernel-shape autotuner of an ensemble. For
  * every class of submitted members with one degree, mesh size and precision, times
  * the tunable kernel shapes on one engine of the class (as atmo_tune_kernels) and

@@ -114,3 +114,18 @@ def test_autotune_picks_a_measured_slot_count():
     # the probes leave the members' own results untouched: member 0 equals its run alone
     for a, b in zip(ens.state(0), run_alone(ms[0])):
         assert np.array_equal(a, b)
+
+
+def test_autotune_rate_tolerance_picks_fewest_slots_near_the_best():
+    """With a rate tolerance the autotuner keeps the fewest slots whose
+    measured rate is within that fraction of the best rate; out-of-range
+    tolerances are refused."""
+    ens = af.Ensemble(slots=3)
+    for i in range(6):
+        ens.submit(af.Member(2, 16, 8, nsteps=2, seed=60 + i, params=dict(dt=8.0)))
+    best, rates = ens.autotune(candidates=(4, 1, 2), probe_steps=2, sample=6, rate_tolerance=0.5)
+    top = max(rates.values())
+    assert top > 0
+    assert best == min(c for c, r in rates.items() if r >= 0.5 * top)
+    with pytest.raises(af.AtmoError):
+        ens.autotune(candidates=(1, 2), probe_steps=1, sample=2, rate_tolerance=1.0)
