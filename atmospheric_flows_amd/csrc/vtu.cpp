@@ -12,6 +12,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -89,6 +90,13 @@ void write(const MeshDesc& m, const std::vector<double>& cgeo, const double* rho
   const long ncell = cgeo.empty() ? (long)m.nx * m.nz : (long)(cgeo.size() / 6);
   const long Ns = ncell * ND2;
   const int ppc = (n + 1) * (n + 1);  // points per mesh cell
+  // the file first: an unwritable path is refused before any field value is read
+  struct Closer {
+    void operator()(FILE* fp) const { std::fclose(fp); }
+  };
+  std::unique_ptr<FILE, Closer> file(std::fopen(path.c_str(), "w"));
+  if (!file) throw std::invalid_argument("write_vtu: cannot open " + path);
+  FILE* f = file.get();
   std::vector<double> pts, vr, vu, vp;
   pts.reserve(3 * ncell * ppc);
   vr.reserve(ncell * ppc);
@@ -164,8 +172,6 @@ void write(const MeshDesc& m, const std::vector<double>& cgeo, const double* rho
           types.push_back(9);  // VTK_QUAD
         }
   }
-  FILE* f = std::fopen(path.c_str(), "w");
-  if (!f) throw std::invalid_argument("write_vtu: cannot open " + path);
   std::fprintf(f, "<?xml version=\"1.0\"?>\n");
   std::fprintf(f, "<VTKFile type=\"UnstructuredGrid\" version=\"1.0\" byte_order=\"LittleEndian\" "
                   "header_type=\"UInt32\">\n  <UnstructuredGrid>\n");
@@ -181,8 +187,7 @@ void write(const MeshDesc& m, const std::vector<double>& cgeo, const double* rho
   data_array(f, "Float64", "u", 3, vu);
   data_array(f, "Float64", "p", 1, vp);
   std::fprintf(f, "      </PointData>\n    </Piece>\n  </UnstructuredGrid>\n</VTKFile>\n");
-  const bool ok = std::ferror(f) == 0;
-  std::fclose(f);
+  const bool ok = std::ferror(f) == 0 && std::fclose(file.release()) == 0;
   if (!ok) throw std::runtime_error("write_vtu: write error on " + path);
 }
 

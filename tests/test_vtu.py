@@ -109,7 +109,7 @@ def test_host_vtu_values_and_cells(tmp_path, k, high_order):
 def test_host_vtu_rejects_bad_arguments(tmp_path):
     h = np.zeros(3)
     mesh = N.MeshDesc(4, 2, 2, 1, 1.0, 1.0, h.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
-    a = np.zeros(64)
+    a = np.zeros(2 * 4 * 16)  # two components of the largest mesh below (4 cells, 16 dofs each at degree 3)
     dp = a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
     assert N.load().atmo_write_vtu_host(ctypes.byref(mesh), dp, dp, dp, str(tmp_path / "x.vtu").encode(), 0) \
         == N.ATMO_INVALID_ARG
