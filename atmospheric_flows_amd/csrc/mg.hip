@@ -307,9 +307,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
 // four lanes per cell, every vector of the sweep (x, d, b, P) and the
 // quadrature weights in registers in the layout of mfma.h, each operator
 // application 33 (k = 2) or 45 (k = 3) MFMA outer-product steps with no LDS
-// and no barrier. Same steps and update expressions as k_cheb_sweep; the
-// contractions sum in another order, so the values differ from it at FP32
-// rounding level.
+// and no barrier. Same steps and update expressions as k_cheb_sweep; every
+// MFMA step is one fused multiply-add per entry and each contraction is
+// accumulated in k_cheb_sweep's order, so the values are bitwise those of
+// k_cheb_sweep.
 constexpr int MX_CELLS = 16;
 
 template <int K, int NC, bool W, bool ZG, bool PRO, bool RES>
